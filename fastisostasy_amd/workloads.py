@@ -1,0 +1,174 @@
+"""Parameter sets and seeded synthetic inputs for the LV-ELVA workloads.
+
+The defaults follow the reference namelist par/test_isostasy_test1d.nml
+(lines 1-79) after the unit conversions of isos_par_load
+(src/fastisostasy.f90:971-979: km -> m for r_earth, pad, zl, dl).  The five
+workloads are the benchmark configurations of this project:
+
+  plain_disc     128x128 ice grid at 25 km, pad 1600 km -> N = 256, uniform
+                 88 km lithosphere, eta = 1e21, disc load (r = 1000 km,
+                 h = 1000 m), 0 -> 50 kyr.
+  lv_rheology    256x256 at 16 km, pad 2048 km -> N = 512, Gaussian
+                 lithosphere (150 km edge -> 50 km centre), two mantle layers
+                 with freqdomain lumping, elastic Green's function on,
+                 parabolic dome (1200 km, 3000 m), 0 -> 20 kyr.
+  ensemble       64 members, 256x256 at 16 km, pad 2048 km -> N = 512,
+                 random log10-viscosity field, random lithosphere thickness
+                 and dome, 10 kyr.
+  sealevel       384x384 at 16 km, pad 1024 km -> N = 512, fractal bed,
+                 sawtooth glacial cycle, interactive sea level, 120 kyr.
+  serving        request stream on 128/256/384/512 ice grids (pad 1024 km).
+
+All fields are float64 arrays in [ny][nx] (x fastest) order.
+"""
+import math
+
+import numpy as np
+
+DEFAULT_PARAMS = dict(
+    heterogeneous_ssh=0, interactive_sealevel=0, include_elastic=0,
+    correct_compression=0, correct_distortion=0, method=3,
+    dt_diagnostics=10.0, min_pad=0.0,
+    mantle="uniform", lithosphere="uniform", layering="equalized", lumping="freqdomain",
+    viscosity_scaling_method="scale", viscosity_scaling=1.0,
+    nl=1, zl=[88.0e3], viscosities=[1e21], dl=0.0, eta_ref=1.e21,
+    rho_water=1000.0, rho_ice=910.0, rho_seawater=1028.0, rho_uppermantle=3400.0,
+    rho_litho=0.0, E=66.0, nu=0.28, g=9.81, r_earth=6.378e3 * 1e3, m_earth=5.972e24,
+    dt_method="tsit54", atol=1e-2, rtol=1e-4, dt_min=1e-10, dt_max=10.0, dt_init=1.0, q=1.0,
+)
+
+
+def params(**overrides):
+    p = dict(DEFAULT_PARAMS)
+    p.update(overrides)
+    return p
+
+
+def ice_coords(nx, dx):
+    """Cell-centre coordinates of an ice grid centred on 0 (metres)."""
+    return (np.arange(nx) - (nx - 1) / 2.0) * dx
+
+
+def padded_coords(n, dx):
+    """init_dims (src/isos_utils.f90:221-240): xc(i) = (i - n/2 - 1) dx."""
+    return (np.arange(n) + 1 - n // 2 - 1) * float(dx)
+
+
+def disc_load(nx, dx, r0=1000e3, h0=1000.0):
+    x = ice_coords(nx, dx)
+    X, Y = np.meshgrid(x, x)
+    return np.where(X * X + Y * Y <= r0 * r0, h0, 0.0)
+
+
+def parabolic_dome(nx, dx, radius, peak, xc=0.0, yc=0.0):
+    x = ice_coords(nx, dx)
+    X, Y = np.meshgrid(x - xc, x - yc)
+    return peak * np.maximum(0.0, 1.0 - (X * X + Y * Y) / (radius * radius))
+
+
+def gaussian_random_field(n, dx, corr_len, rng):
+    """Zero-mean, unit-variance stationary field with Gaussian covariance
+    exp(-r^2 / (2 L^2)), by spectral synthesis on an n x n periodic grid."""
+    k = 2 * np.pi * np.fft.fftfreq(n, d=dx)
+    KX, KY = np.meshgrid(k, k)
+    amp = np.exp(-0.25 * (KX ** 2 + KY ** 2) * corr_len ** 2)
+    noise = rng.standard_normal((n, n))
+    f = np.fft.ifft2(np.fft.fft2(noise) * amp).real
+    return (f - f.mean()) / f.std()
+
+
+def fractal_field(n, dx, slope, rng):
+    """Zero-mean, unit-variance field with power spectrum ~ k^slope."""
+    k = np.fft.fftfreq(n, d=dx)
+    KX, KY = np.meshgrid(k, k)
+    kk = np.sqrt(KX ** 2 + KY ** 2)
+    kk[0, 0] = np.inf
+    amp = kk ** (slope / 2.0)
+    amp[0, 0] = 0.0
+    noise = rng.standard_normal((n, n))
+    f = np.fft.ifft2(np.fft.fft2(noise) * amp).real
+    return (f - f.mean()) / f.std()
+
+
+def padded_size(nx_ice, dx, pad):
+    """init_domain_size (src/isos_utils.f90:285-354) for a square ice grid."""
+    nxt = nx_ice + (nx_ice % 2)
+    return nxt + 2 * int(math.floor(pad / dx + 0.5))
+
+
+# ---------------------------------------------------------------- workloads
+def plain_disc():
+    """FP64 plain LV-ELVA: 128^2 at 25 km, N = 256, disc load, 50 kyr."""
+    nx, dx = 128, 25e3
+    p = params(min_pad=1600e3, rho_uppermantle=3300.0, zl=[88e3], viscosities=[1e21])
+    H = disc_load(nx, dx)
+    z_bed = np.full((nx, nx), 1e6)
+    return dict(name="plain_disc", params=p, nx=nx, dx=dx, H_ice=H, z_bed=z_bed,
+                H_init=np.zeros_like(H), t0=0.0, t1=50e3, dtt=10.0, dt_out=1e3,
+                he_field=None, eta_field=None)
+
+
+def lv_rheology():
+    """Laterally variable rheology: 256^2 at 16 km, N = 512, 20 kyr."""
+    nx, dx, pad = 256, 16e3, 2048e3
+    n = padded_size(nx, dx, pad)
+    xc = padded_coords(n, dx)
+    X, Y = np.meshgrid(xc, xc)
+    r2 = X * X + Y * Y
+    g = np.exp(-r2 / (2 * 500e3 ** 2))
+    he = 150e3 - 100e3 * g
+    eta = np.empty((2, n, n))
+    eta[0] = 10.0 ** (21.0 - 2.0 * g)
+    eta[1] = 1e21
+    p = params(min_pad=pad, lithosphere="fields", mantle="fields", nl=2,
+               zl=[150e3, 300e3], viscosities=[1e21, 1e21], lumping="freqdomain",
+               layering="equalized", include_elastic=1)
+    H = parabolic_dome(nx, dx, 1200e3, 3000.0)
+    return dict(name="lv_rheology", params=p, nx=nx, dx=dx, H_ice=H,
+                z_bed=np.full((nx, nx), 1e6), H_init=np.zeros_like(H), t0=0.0, t1=20e3,
+                dtt=10.0, dt_out=1e3, he_field=he, eta_field=eta)
+
+
+def ensemble_member(m, nx=256, dx=16e3, pad=2048e3):
+    """Member m of the ensemble (seed 1000 + m)."""
+    rng = np.random.default_rng(1000 + m)
+    n = padded_size(nx, dx, pad)
+    grf = gaussian_random_field(n, dx, 300e3, rng)
+    log10_eta = np.clip(20.5 + 0.7 * grf, 19.0, 22.0)
+    he = rng.uniform(50e3, 150e3)
+    radius = rng.uniform(600e3, 1500e3)
+    peak = rng.uniform(1000.0, 3500.0)
+    p = params(min_pad=pad, mantle="fields", lithosphere="fields", nl=1, zl=[he],
+               viscosities=[1e21])
+    H = parabolic_dome(nx, dx, radius, peak)
+    return dict(name=f"ensemble_{m}", params=p, nx=nx, dx=dx, H_ice=H,
+                z_bed=np.full((nx, nx), 1e6), H_init=np.zeros_like(H), t0=0.0, t1=10e3,
+                dtt=10.0, dt_out=1e3, he_field=np.full((n, n), he),
+                eta_field=(10.0 ** log10_eta)[None])
+
+
+def sealevel_cycle(nx=384, dx=16e3, pad=1024e3, seed=4242):
+    """Interactive sea level and masks, 120 kyr sawtooth cycle."""
+    rng = np.random.default_rng(seed)
+    bed = -500.0 + 1000.0 * fractal_field(nx, dx, -2.0, rng)
+    H0 = parabolic_dome(nx, dx, 0.35 * nx * dx, 3500.0)
+    p = params(min_pad=pad, interactive_sealevel=1, heterogeneous_ssh=1, include_elastic=1,
+               zl=[100e3], viscosities=[1e21], rho_uppermantle=3300.0)
+    return dict(name="sealevel", params=p, nx=nx, dx=dx, H_max=H0, z_bed=bed,
+                H_init=H0 * sawtooth(0.0), t0=0.0, t1=120e3, dtt=10.0, dt_out=1e3,
+                he_field=None, eta_field=None, bsl_method="fastiso")
+
+
+def sawtooth(t, period=100e3, grow=90e3):
+    """Glacial-cycle factor: 0 -> 1 linearly over `grow` years, 1 -> 0 over the rest."""
+    ph = np.mod(t, period)
+    return np.where(ph < grow, ph / grow, 1.0 - (ph - grow) / (period - grow))
+
+
+def step_on_load(H, t):
+    """interp_2d between (t=0, H=0) and (t=1e-9, H) as in tests/test_isostasy.f90:233-238."""
+    if t < 0.0:
+        return np.zeros_like(H)
+    if t > 1e-9:
+        return H
+    return H * (t / 1e-9)
