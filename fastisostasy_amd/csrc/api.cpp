@@ -1,0 +1,840 @@
+// api.cpp -- host side of the C ABI (include/fastiso.h).
+//
+// Owns device memory, FFT plans and twiddle tables, and sequences the kernels
+// of isos_init / isos_init_state / isos_update (src/fastisostasy.f90) for one
+// domain or a batch of domains.  There is no CPU compute path: every field
+// operation runs on the GPU, and the library refuses to create a handle when
+// no HIP device is present.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/fastiso.h"
+#include "fi_kernels.h"
+
+using namespace fi;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(x)                                                                          \
+    do {                                                                                   \
+        hipError_t e_ = (x);                                                               \
+        if (e_ != hipSuccess) return fail(FI_E_HIP, "%s: %s", #x, hipGetErrorString(e_)); \
+    } while (0)
+
+std::mutex g_mu;
+hipStream_t g_stream = nullptr;
+
+int ensure_device() {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (g_stream) return 0;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return fail(FI_E_HIP, "no HIP device available: the LV-ELVA library has no CPU path");
+    HIPCHK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+    return 0;
+}
+
+bool make_plan(int n, FftPlan& p) {
+    p.n = n;
+    p.nf = 0;
+    int m = n;
+    const int order[] = {8, 4, 2, 3, 5, 7};
+    for (int r : order)
+        while (m % r == 0 && p.nf < 24) { p.radix[p.nf++] = r; m /= r; }
+    return m == 1 && n >= 4;
+}
+
+struct Twiddles {
+    double2* d64 = nullptr;
+    float2* d32 = nullptr;
+};
+std::map<int, Twiddles> g_tw;
+
+int get_twiddles(int n, Twiddles& out) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_tw.find(n);
+    if (it != g_tw.end()) { out = it->second; return 0; }
+    std::vector<double2> h64(n);
+    std::vector<float2> h32(n);
+    for (int k = 0; k < n; ++k) {
+        const long double a = 2.0L * 3.141592653589793238462643383279502884L * (long double)k / (long double)n;
+        const double c = (double)cosl(a), s = -(double)sinl(a);
+        h64[k] = make_double2(c, s);
+        h32[k] = make_float2((float)c, (float)s);
+    }
+    Twiddles t;
+    HIPCHK(hipMalloc(&t.d64, sizeof(double2) * n));
+    HIPCHK(hipMalloc(&t.d32, sizeof(float2) * n));
+    HIPCHK(hipMemcpy(t.d64, h64.data(), sizeof(double2) * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(t.d32, h32.data(), sizeof(float2) * n, hipMemcpyHostToDevice));
+    g_tw[n] = t;
+    out = t;
+    return 0;
+}
+
+int fft_threads(const FftPlan& p, int lines, int& threads) {
+    int nb = 0;
+    for (int s = 0; s < p.nf; ++s) nb = std::max(nb, (p.n / p.radix[s]) * lines);
+    threads = ((nb / 2 + 63) / 64) * 64;
+    threads = std::min(512, std::max(64, threads));
+    if (nb > FFT_MAXIT * threads) return fail(FI_E_UNSUPPORTED, "FFT block too large (N=%d, lines=%d)", p.n, lines);
+    return 0;
+}
+
+// Row / column blocking for a batch of `nmem` domains of size n.
+int make_shape(int n, int nmem, bool fp64, bool uniform, RhsShape& s) {
+    if (!make_plan(n, s.plan))
+        return fail(FI_E_UNSUPPORTED, "padded size N=%d has a prime factor > 7; choose pad so that N is 2^a 3^b 5^c 7^d", n);
+    s.n = n;
+    s.fp64 = fp64;
+    s.uniform = uniform;
+    // aim for >= 512 row blocks and >= 256 column blocks over the batch
+    // LDS per block stays within 64 KiB (n * lines complex elements)
+    const size_t cbytes = fp64 ? 16 : 8;
+    auto fits = [&](int lines) { return (size_t)n * lines * cbytes <= 65536; };
+    if (!fits(1)) return fail(FI_E_UNSUPPORTED, "padded size N=%d too large for one LDS line", n);
+    int L = 1;
+    while (L < 8 && (n / 2) % (2 * L) == 0 && fits(2 * L) && (size_t)(n / (4 * L)) * nmem >= 512) L *= 2;
+    const int cmax = fp64 ? 8 : 16;
+    int C = 1;
+    while (C < cmax && (n / 2) % (2 * C) == 0 && fits(2 * C) && (size_t)(n / (4 * C)) * nmem >= 256) C *= 2;
+    s.L = L;
+    s.C = C;
+    int rc = fft_threads(s.plan, L, s.threads_row);
+    if (rc) return rc;
+    rc = fft_threads(s.plan, C, s.threads_col);
+    if (rc) return rc;
+    Twiddles t;
+    rc = get_twiddles(n, t);
+    if (rc) return rc;
+    s.tw = fp64 ? (const void*)t.d64 : (const void*)t.d32;
+    s.tw64 = t.d64;
+    return 0;
+}
+
+int make_conv_shape(int n, bool fp64, ConvShape& s) {
+    s.n = n;
+    s.m = 2 * n;
+    s.fp64 = fp64;
+    if (!make_plan(s.m, s.plan)) return fail(FI_E_UNSUPPORTED, "convolution size 2N=%d not supported", s.m);
+    const size_t cbytes = fp64 ? 16 : 8;
+    auto fits = [&](int lines) { return (size_t)s.m * lines * cbytes <= 65536; };
+    if (!fits(1)) return fail(FI_E_UNSUPPORTED, "convolution size 2N=%d too large for one LDS line", s.m);
+    int L = 1;
+    while (L < 4 && n % (4 * L) == 0 && fits(2 * L)) L *= 2;
+    int C = 1;
+    while (C < 8 && fits(2 * C)) C *= 2;
+    s.Lr = L;
+    s.Cc = C;
+    int rc = fft_threads(s.plan, L, s.threads_row);
+    if (rc) return rc;
+    rc = fft_threads(s.plan, C, s.threads_col);
+    if (rc) return rc;
+    Twiddles t;
+    rc = get_twiddles(s.m, t);
+    if (rc) return rc;
+    s.tw = fp64 ? (const void*)t.d64 : (const void*)t.d32;
+    return 0;
+}
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+}  // namespace
+
+// ------------------------------------------------------------------ handles
+struct fi_bsl {
+    BslDev* d = nullptr;
+    BslDev h{};
+};
+
+struct fi_isos {
+    fi_params par{};
+    int n = 0, nx_ice = 0, ny_ice = 0, icrop1 = 0, jcrop1 = 0;  // 0-based crop origin
+    double dx = 0;
+    bool fp64 = true, uniform = true;
+    bool ref_was_set = false, state_init = false;
+    double time_diag = 0, time_prog = 0;
+    char* arena = nullptr;
+    Member hm{};
+    Member* d_mem = nullptr;
+    double *eta_eff = nullptr, *kappa = nullptr, *R = nullptr, *He = nullptr, *Dd = nullptr;
+    RhsShape shape{};
+    ConvShape cshape{};
+    bool has_conv = false;
+    int part_blocks = 0;
+    fi_bsl* bsl_bound = nullptr;
+};
+
+struct fi_batch {
+    std::vector<fi_isos*> mem;
+    Member* d_mem = nullptr;
+    std::vector<Member> hm;
+    RhsShape shape{};
+};
+
+static const ConvShape& conv_shape_of(const fi_isos* h) { return h->cshape; }
+
+namespace {
+
+int domain_size(int nx_ice, int ny_ice, double dx, double dy, double min_pad, int& n, int& icrop1, int& jcrop1) {
+    // init_domain_size (src/isos_utils.f90:285-354); crop origin 0-based
+    if (min_pad < 0.0) return fail(FI_E_ARG, "Minimum padding must be positive.");
+    if (dx != dy) return fail(FI_E_ARG, "Currently, only square grid cells are supported.");
+    if (nx_ice < 3 || ny_ice < 3) return fail(FI_E_ARG, "ice grid too small");
+    int ic = 1, jc = 1, nxt = nx_ice, nyt = ny_ice;
+    if (nx_ice % 2) { ic++; nxt++; }
+    if (ny_ice % 2) { jc++; nyt++; }
+    const int npad = (int)std::lround(min_pad / dx);
+    int npx = npad, npy = npad;
+    if (nxt < nyt) npx += (nyt - nxt) / 2;
+    else if (nxt > nyt) npy += (nxt - nyt) / 2;
+    n = nxt + 2 * npx;
+    if (n != nyt + 2 * npy) return fail(FI_E_ARG, "Failed to correctly pad domain.");
+    icrop1 = ic + npx - 1;
+    jcrop1 = jc + npy - 1;
+    return 0;
+}
+
+// Upload an ice-grid array embedded in an N x N background.
+int upload_embedded(const fi_isos* h, double* dst, const double* src, double background) {
+    const int n = h->n;
+    std::vector<double> buf((size_t)n * n, background);
+    for (int j = 0; j < h->ny_ice; ++j)
+        for (int i = 0; i < h->nx_ice; ++i)
+            buf[(size_t)(j + h->jcrop1) * n + (i + h->icrop1)] = src[(size_t)j * h->nx_ice + i];
+    HIPCHK(hipMemcpyAsync(dst, buf.data(), sizeof(double) * n * n, hipMemcpyHostToDevice, g_stream));
+    HIPCHK(hipStreamSynchronize(g_stream));
+    return 0;
+}
+
+int read_ctl(const Member& m, Ctl& c) {
+    HIPCHK(hipMemcpyAsync(&c, m.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipStreamSynchronize(g_stream));
+    return 0;
+}
+
+int write_ctl(const Member& m, const Ctl& c) {
+    HIPCHK(hipMemcpyAsync(m.ctl, &c, sizeof(Ctl), hipMemcpyHostToDevice, g_stream));
+    HIPCHK(hipStreamSynchronize(g_stream));
+    return 0;
+}
+
+// One isos_update for a group of members that share N, precision and call time.
+int run_update(std::vector<fi_isos*>& hs, Member* d_mem, const RhsShape& shape, double time, bool stepping) {
+    const int nmem = (int)hs.size();
+    const fi_isos& h0 = *hs[0];
+    const int n = h0.n;
+    // diagnostics cadence (fastisostasy.f90:782-787) is a function of the call times only
+    bool diag = false;
+    for (fi_isos* h : hs) {
+        const bool d = (time - h->time_diag >= (double)1e-3f);
+        if (d) h->time_diag = h->time_diag + h->par.dt_diagnostics;
+        if (h != hs[0] && d != diag) return fail(FI_E_STATE, "batch members disagree on the diagnostics cadence");
+        diag = d;
+    }
+    launch_pre_haf(d_mem, nmem, n, true, g_stream);
+    launch_pre_bsl(d_mem, nmem, n, g_stream);
+    if (!diag) {
+        launch_pre_fields(d_mem, nmem, n, 0, false, h0.fp64, g_stream);
+    } else {
+        launch_pre_fields(d_mem, nmem, n, 0, true, h0.fp64, g_stream);
+        if (h0.par.include_elastic) launch_convolution(d_mem, nmem, conv_shape_of(&h0), 0, g_stream);
+        launch_pre_fields(d_mem, nmem, n, 1, true, h0.fp64, g_stream);
+        if (h0.par.heterogeneous_ssh) launch_convolution(d_mem, nmem, conv_shape_of(&h0), 1, g_stream);
+        launch_pre_fields(d_mem, nmem, n, 2, true, h0.fp64, g_stream);
+    }
+    launch_begin(d_mem, nmem, time, stepping ? 1 : 0, g_stream);
+    HIPCHK(hipGetLastError());
+    if (stepping) {
+        // attempts until every member reached `time`; members that are done
+        // skip the launches on the device
+        std::vector<Ctl> ctl(nmem);
+        for (int it = 0;; ++it) {
+            launch_attempt(d_mem, nmem, shape, g_stream);
+            HIPCHK(hipGetLastError());
+            bool running = false;
+            for (int m = 0; m < nmem; ++m) {
+                int rc = read_ctl(hs[m]->hm, ctl[m]);
+                if (rc) return rc;
+                running |= (ctl[m].status == ST_RUN);
+            }
+            if (!running) break;
+            if (it > 100000000) return fail(FI_E_STATE, "integrator did not reach the target time");
+        }
+    }
+    launch_finalize(d_mem, nmem, n, g_stream);
+    HIPCHK(hipGetLastError());
+    for (int m = 0; m < nmem; ++m) {
+        Ctl c;
+        int rc = read_ctl(hs[m]->hm, c);
+        if (rc) return rc;
+        hs[m]->time_prog = time;
+        // check_isos_instabilities + NaN check (fastisostasy.f90:860-869)
+        if (c.maxw > 1e5) return fail(FI_E_UNSTABLE, "large_check:: Error: Large values detected in array w");
+        if (c.maxwe > 1e5) return fail(FI_E_UNSTABLE, "large_check:: Error: Large values detected in array we");
+        if (c.maxdzss > 1e3) return fail(FI_E_UNSTABLE, "large_check:: Error: Large values detected in array dz_ss");
+        if (c.nonfinite) return fail(FI_E_NAN, "isos_update:: Error: NaNs detected.");
+    }
+    return 0;
+}
+
+int bind_bsl(fi_isos* h, fi_bsl* b) {
+    if (!b) return fail(FI_E_ARG, "bsl is NULL");
+    if (h->bsl_bound != b) {
+        h->hm.bsl = b->d;
+        h->bsl_bound = b;
+        HIPCHK(hipMemcpy(h->d_mem, &h->hm, sizeof(Member), hipMemcpyHostToDevice));
+    }
+    return 0;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ C ABI
+extern "C" {
+
+const char* fi_last_error(void) { return g_err.c_str(); }
+int fi_version(void) { return 1; }
+
+int fi_padded_size(int nx_ice, int ny_ice, double dx, double dy, double min_pad, int* n_out) {
+    int n, ic, jc;
+    int rc = domain_size(nx_ice, ny_ice, dx, dy, min_pad, n, ic, jc);
+    if (rc) return rc;
+    *n_out = n;
+    return 0;
+}
+
+int fi_sync(void) {
+    int rc = ensure_device();
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(g_stream));
+    return 0;
+}
+
+int fi_bsl_create(int method, double bsl_init, double A_ocean_pd, int nvec, const double* bsl_vec,
+                  const double* A_vec, double year_bp, fi_bsl** out) {
+    int rc = ensure_device();
+    if (rc) return rc;
+    if (method != 0 && method != 1) return fail(FI_E_UNSUPPORTED, "bsl method must be const (0) or fastiso (1)");
+    if (nvec < 0 || nvec > 64) return fail(FI_E_ARG, "A_ocean table must have 0..64 entries");
+    fi_bsl* b = new fi_bsl();
+    BslDev& d = b->h;
+    d.method = method;
+    d.time = year_bp;
+    d.A_ocean_pd = A_ocean_pd;
+    d.A_ocean_now = A_ocean_pd;
+    d.bsl_init = bsl_init;
+    if (method == 1) {
+        if (nvec == 0) {
+            d.constant_ocean_surface = 1;
+        } else {
+            // bsl_init_fastiso (barysealevel.f90:176-188): correct the pd bias of the table
+            d.nvec = nvec;
+            for (int i = 0; i < nvec; ++i) { d.bsl_vec[i] = bsl_vec[i]; d.A_vec[i] = A_vec[i]; }
+            auto interp = [&](double x) {
+                const int n = d.nvec;
+                if (x < d.bsl_vec[0]) return d.A_vec[0];
+                if (x > d.bsl_vec[n - 1]) return d.A_vec[n - 1];
+                int j;
+                for (j = 0; j < n; ++j) if (d.bsl_vec[j] >= x) break;
+                if (j == 0) return d.A_vec[0];
+                if (j == n) return d.A_vec[n - 1];
+                const double al = (x - d.bsl_vec[j - 1]) / (d.bsl_vec[j] - d.bsl_vec[j - 1]);
+                return d.A_vec[j - 1] + al * (d.A_vec[j] - d.A_vec[j - 1]);
+            };
+            const double biased = interp(d.bsl_init);
+            for (int i = 0; i < nvec; ++i) d.A_vec[i] = d.A_vec[i] * d.A_ocean_pd / biased;
+            d.A_ocean_now = interp(d.bsl_init);
+        }
+    }
+    d.bsl_now = d.bsl_init;
+    if (hipMalloc(&b->d, sizeof(BslDev)) != hipSuccess) { delete b; return fail(FI_E_HIP, "hipMalloc bsl"); }
+    HIPCHK(hipMemcpy(b->d, &d, sizeof(BslDev), hipMemcpyHostToDevice));
+    *out = b;
+    return 0;
+}
+
+int fi_bsl_update(fi_bsl* b, double year_bp) {
+    if (!b) return fail(FI_E_ARG, "bsl is NULL");
+    launch_bsl_update(b->d, year_bp, g_stream);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int fi_bsl_get(fi_bsl* b, double* bsl_now, double* A_ocean_now) {
+    if (!b) return fail(FI_E_ARG, "bsl is NULL");
+    HIPCHK(hipMemcpyAsync(&b->h, b->d, sizeof(BslDev), hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipStreamSynchronize(g_stream));
+    if (bsl_now) *bsl_now = b->h.bsl_now;
+    if (A_ocean_now) *A_ocean_now = b->h.A_ocean_now;
+    return 0;
+}
+
+int fi_bsl_destroy(fi_bsl* b) {
+    if (!b) return 0;
+    (void)hipStreamSynchronize(g_stream);
+    (void)hipFree(b->d);
+    delete b;
+    return 0;
+}
+
+int fi_isos_create(const fi_params* p, int nx_ice, int ny_ice, double dx, double dy, const double* K,
+                   const double* rho_ice, const double* he_field, const double* eta_field, fi_isos** out) {
+    int rc = ensure_device();
+    if (rc) return rc;
+    if (!p || !out) return fail(FI_E_ARG, "NULL argument");
+    if (p->method != 3) return fail(FI_E_UNSUPPORTED, "only method 3 (LV-ELVA) is supported");
+    if (p->nl < 1 || p->nl > FI_MAX_LAYERS) return fail(FI_E_ARG, "nl out of range");
+    if (p->dt_method < 0 || p->dt_method > 3) return fail(FI_E_ARG, "unknown dt_method");
+    if (p->lithosphere == 3 && !he_field) return fail(FI_E_ARG, "lithosphere=fields needs he_field");
+    if (p->mantle == 3 && !eta_field) return fail(FI_E_ARG, "mantle=fields needs eta_field");
+    if (p->layering != 0 && p->layering != 1) return fail(FI_E_UNSUPPORTED, "layering must be parallel or equalized");
+    fi_isos* h = new fi_isos();
+    h->par = *p;
+    if (K) h->par.correct_distortion = 1;
+    if (rho_ice) h->par.rho_ice = *rho_ice;
+    rc = domain_size(nx_ice, ny_ice, dx, dy, p->min_pad, h->n, h->icrop1, h->jcrop1);
+    if (rc) { delete h; return rc; }
+    if (p->interactive_sealevel && !(p->min_pad > (double)100.e3f)) {
+        delete h;
+        return fail(FI_E_ARG, "interactive_sealevel=true requires a mask file or at least 100km of padding.");
+    }
+    h->nx_ice = nx_ice;
+    h->ny_ice = ny_ice;
+    h->dx = dx;
+    h->fp64 = (p->precision == FI_FP64);
+    h->uniform = (K == nullptr);
+    const int n = h->n;
+    const size_t nn = (size_t)n * n;
+    rc = make_shape(n, 1, h->fp64, h->uniform, h->shape);
+    if (rc) { delete h; return rc; }
+    h->has_conv = p->include_elastic || p->heterogeneous_ssh;
+    if (h->has_conv) {
+        rc = make_conv_shape(n, h->fp64, h->cshape);
+        if (rc) { delete h; return rc; }
+    }
+    const size_t rp = h->fp64 ? 8 : 4;
+    h->part_blocks = (int)std::max<size_t>((nn + 1023) / 1024, (size_t)n);
+
+    // ---- arena layout
+    size_t off = 0;
+    auto take = [&](size_t bytes) { size_t o = off; off += align256(bytes); return o; };
+    const size_t f64 = nn * 8;
+    size_t oX0 = take(f64), oX1 = take(f64), oW = take(f64), odw = take(f64);
+    size_t oz = take(f64), ozss = take(f64), odz = take(f64), owe = take(f64), oH = take(f64), oHaf = take(f64);
+    size_t ocm = take(f64), ocf = take(f64), ocl = take(f64), oci = take(f64);
+    size_t ozr = take(f64), ozssr = take(f64), ohafr = take(f64), ohr = take(f64), owr = take(f64), ower = take(f64),
+           odzr = take(f64);
+    size_t oA = 0, odxm = 0, oK2 = 0;
+    if (!h->uniform) { oA = take(f64); odxm = take(f64); oK2 = take(f64); }
+    size_t omask = take(nn), omaskr = take(nn);
+    size_t oHin = take((size_t)nx_ice * ny_ice * 8);
+    size_t oD = take(nn * rp), oeta = take(nn * rp), okR = take(nn * rp), ocfr = take(nn * rp), og1 = take(nn * rp),
+           og2 = take(nn * rp);
+    size_t ocor = take((size_t)n * 4 * 8), opart = take((size_t)h->part_blocks * 4 * 8);
+    size_t oee = take(f64), okap = take(f64), oR = take(f64), oHe = take(f64), oDd = take(f64);
+    size_t oscr = take(f64 + 64);
+    const size_t hc = (size_t)(n + 1) * (h->fp64 ? 16 : 8);
+    size_t ocw = 0, oFGE = 0, oFGN = 0;
+    if (h->has_conv) { ocw = take(2 * (size_t)n * hc); oFGE = take(2 * (size_t)n * hc); oFGN = take(2 * (size_t)n * hc); }
+    size_t octl = take(sizeof(Ctl)), ophys = take(sizeof(PhysDev)), omem = take(sizeof(Member));
+    if (hipMalloc(&h->arena, off) != hipSuccess) { delete h; return fail(FI_E_HIP, "hipMalloc arena (%zu bytes)", off); }
+    HIPCHK(hipMemsetAsync(h->arena, 0, off, g_stream));
+    char* a = h->arena;
+    Member& M = h->hm;
+    M.n = n; M.icrop1 = h->icrop1; M.jcrop1 = h->jcrop1; M.nx_ice = nx_ice; M.ny_ice = ny_ice;
+    M.X[0] = (double*)(a + oX0); M.X[1] = (double*)(a + oX1); M.W = (double*)(a + oW); M.dwdt = (double*)(a + odw);
+    M.z_bed = (double*)(a + oz); M.z_ss = (double*)(a + ozss); M.dz_ss = (double*)(a + odz); M.we = (double*)(a + owe);
+    M.Hice = (double*)(a + oH); M.Haf = (double*)(a + oHaf); M.canom_mantle = (double*)(a + ocm);
+    M.canom_full = (double*)(a + ocf); M.canom_load = (double*)(a + ocl); M.conv_in = (double*)(a + oci);
+    M.z_bed_ref = (double*)(a + ozr); M.z_ss_ref = (double*)(a + ozssr); M.Haf_ref = (double*)(a + ohafr);
+    M.Hice_ref = (double*)(a + ohr); M.w_ref = (double*)(a + owr); M.we_ref = (double*)(a + ower);
+    M.dz_ss_ref = (double*)(a + odzr);
+    if (!h->uniform) { M.A = (double*)(a + oA); M.dxm = (double*)(a + odxm); M.K2 = (double*)(a + oK2); }
+    M.mask = (uint8_t*)(a + omask); M.mask_ref = (uint8_t*)(a + omaskr);
+    M.H_in = (double*)(a + oHin);
+    M.D = a + oD; M.eta = a + oeta; M.kR = a + okR; M.cfull = a + ocfr; M.g1 = a + og1; M.g2 = a + og2;
+    M.corners = (double*)(a + ocor); M.partials = (double*)(a + opart);
+    M.ctl = (Ctl*)(a + octl); M.phys = (PhysDev*)(a + ophys);
+    h->d_mem = (Member*)(a + omem);
+    h->eta_eff = (double*)(a + oee); h->kappa = (double*)(a + okap); h->R = (double*)(a + oR);
+    h->He = (double*)(a + oHe); h->Dd = (double*)(a + oDd);
+    double* scratch = (double*)(a + oscr);
+    if (h->has_conv) { M.conv_work = (double*)(a + ocw); M.FGE = a + oFGE; M.FGN = a + oFGN; }
+
+    // ---- physics scalars (fastisostasy.f90:152-164)
+    const fi_params& P = h->par;
+    PhysDev ph{};
+    ph.g = P.g; ph.nu = P.nu; ph.rho_ice = P.rho_ice; ph.rho_water = P.rho_water; ph.rho_seawater = P.rho_seawater;
+    ph.rho_uppermantle = P.rho_uppermantle; ph.rho_litho = P.rho_litho;
+    ph.sec_per_year = (double)(3600.0f * 24.0f * 365.25f);
+    ph.Vden_factor = P.rho_ice / P.rho_water - P.rho_ice / P.rho_seawater;
+    ph.dt_diagnostics = P.dt_diagnostics;
+    ph.atol = P.atol; ph.rtol = P.rtol; ph.dt_max = P.dt_max; ph.q = P.q;
+    ph.dx = dx;
+    ph.dt_method = P.dt_method;
+    ph.interactive_sealevel = P.interactive_sealevel; ph.include_elastic = P.include_elastic;
+    ph.heterogeneous_ssh = P.heterogeneous_ssh;
+    ph.uniform_K = h->uniform;
+    ph.method = P.method;
+    HIPCHK(hipMemcpyAsync(M.phys, &ph, sizeof ph, hipMemcpyHostToDevice, g_stream));
+
+    // ---- distortion fields (fastisostasy.f90:153-155)
+    if (!h->uniform) {
+        std::vector<double> A(nn), dxm(nn), K2(nn);
+        for (size_t c = 0; c < nn; ++c) {
+            dxm[c] = dx * K[c];
+            A[c] = dxm[c] * (dy * K[c]);
+            K2[c] = std::pow(K[c], 2.0);
+        }
+        HIPCHK(hipMemcpy(M.A, A.data(), nn * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(M.dxm, dxm.data(), nn * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(M.K2, K2.data(), nn * 8, hipMemcpyHostToDevice));
+    }
+
+    // ---- maskactive (fastisostasy.f90:185-198) in bit 3 of both masks
+    {
+        std::vector<uint8_t> mk(nn, 0);
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) {
+                bool act = true;
+                if (P.interactive_sealevel)
+                    act = (i >= h->icrop1 && i < h->icrop1 + nx_ice && j >= h->jcrop1 && j < h->jcrop1 + ny_ice);
+                mk[(size_t)j * n + i] = act ? 8 : 0;
+            }
+        HIPCHK(hipMemcpy(M.mask, mk.data(), nn, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(M.mask_ref, mk.data(), nn, hipMemcpyHostToDevice));
+    }
+
+    // ---- rheology (fastisostasy.f90:166-172, 240-442)
+    if (P.method == 3) {
+        InitArgs ia{};
+        ia.n = n; ia.dx = dx; ia.nl = P.nl; ia.lithosphere = P.lithosphere; ia.mantle = P.mantle;
+        ia.layering = P.layering; ia.lumping = P.lumping; ia.vscale_method = P.viscosity_scaling_method;
+        ia.vscale = P.viscosity_scaling;
+        for (int k = 0; k < FI_MAX_LAYERS; ++k) { ia.zl[k] = P.zl[k]; ia.visc[k] = P.viscosities[k]; }
+        ia.dl = P.dl; ia.eta_ref = P.eta_ref; ia.E = P.E; ia.nu = P.nu;
+        ia.comp_corr = P.correct_compression ? 1.5 / (1 + P.nu) : 1.0;
+        ia.r_earth = P.r_earth; ia.m_earth = P.m_earth;
+        double *d_he = nullptr, *d_eta = nullptr;
+        if (P.lithosphere == 3) {
+            HIPCHK(hipMalloc(&d_he, nn * 8));
+            HIPCHK(hipMemcpy(d_he, he_field, nn * 8, hipMemcpyHostToDevice));
+        }
+        if (P.mantle == 3) {
+            HIPCHK(hipMalloc(&d_eta, nn * 8 * P.nl));
+            HIPCHK(hipMemcpy(d_eta, eta_field, nn * 8 * P.nl, hipMemcpyHostToDevice));
+        }
+        launch_init_rheology(h->d_mem, ia, d_he, d_eta, h->eta_eff, h->kappa, h->R, h->He, h->Dd, scratch, h->fp64,
+                             g_stream);
+        launch_init_finalize(ia, scratch, h->eta_eff, h->kappa, h->R, h->Dd, M.D, M.eta, M.kR, h->fp64, g_stream);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(g_stream));
+        if (d_he) (void)hipFree(d_he);
+        if (d_eta) (void)hipFree(d_eta);
+    }
+
+    // ---- Green's functions and their spectra (fastisostasy.f90:174-183)
+    if (h->has_conv) {
+        HIPCHK(hipMemcpy(h->d_mem, &M, sizeof M, hipMemcpyHostToDevice));
+        for (int which = 0; which < 2; ++which) {
+            launch_green(which, n, dx, P.r_earth, P.m_earth, M.conv_in, g_stream);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipStreamSynchronize(g_stream));
+            double* kc = which == 0 ? M.kc_E : M.kc_N;
+            HIPCHK(hipMemcpy(&kc[0], M.conv_in, 8, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(&kc[1], M.conv_in + nn - 1, 8, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(&kc[2], M.conv_in + (size_t)(n - 1) * n, 8, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(&kc[3], M.conv_in + (n - 1), 8, hipMemcpyDeviceToHost));
+            launch_kernel_spectrum(h->d_mem, 1, h->cshape, which, M.conv_in, g_stream);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipStreamSynchronize(g_stream));
+        }
+        HIPCHK(hipMemsetAsync(M.conv_in, 0, nn * 8, g_stream));
+    }
+
+    // ---- zero_init_state (fastisostasy.f90:461-462, 506-548)
+    {
+        std::vector<double> bg(nn, 1e6);
+        HIPCHK(hipMemcpy(M.z_bed, bg.data(), nn * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(M.z_bed_ref, bg.data(), nn * 8, hipMemcpyHostToDevice));
+    }
+    Ctl c{};
+    c.time_diag = (double)1e10f;
+    c.time_prog = (double)1e10f;
+    h->time_diag = c.time_diag;
+    h->time_prog = c.time_prog;
+    HIPCHK(hipMemcpy(M.ctl, &c, sizeof c, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->d_mem, &M, sizeof M, hipMemcpyHostToDevice));
+    HIPCHK(hipStreamSynchronize(g_stream));
+    *out = h;
+    return 0;
+}
+
+int fi_isos_init_ref(fi_isos* h, const double* z_bed, const double* H_ice, const double* bsl, const double* dz_ss) {
+    if (!h || !z_bed || !H_ice) return fail(FI_E_ARG, "NULL argument");
+    const size_t nn = (size_t)h->n * h->n;
+    int rc = upload_embedded(h, h->hm.z_bed_ref, z_bed, 1e6);
+    if (rc) return rc;
+    rc = upload_embedded(h, h->hm.Hice_ref, H_ice, 0.0);
+    if (rc) return rc;
+    if (bsl) {
+        Ctl c;
+        rc = read_ctl(h->hm, c);
+        if (rc) return rc;
+        c.ref_bsl = *bsl;
+        rc = write_ctl(h->hm, c);
+        if (rc) return rc;
+    }
+    if (dz_ss) HIPCHK(hipMemcpy(h->hm.dz_ss_ref, dz_ss, nn * 8, hipMemcpyHostToDevice));
+    h->ref_was_set = true;
+    return 0;
+}
+
+int fi_isos_init_state(fi_isos* h, const double* z_bed, const double* H_ice, double time, fi_bsl* b) {
+    if (!h || !z_bed || !H_ice || !b) return fail(FI_E_ARG, "NULL argument");
+    int rc = bind_bsl(h, b);
+    if (rc) return rc;
+    const Member& M = h->hm;
+    const size_t nn = (size_t)h->n * h->n;
+    // now%bsl = bsl%bsl_now (fastisostasy.f90:606)
+    rc = fi_bsl_get(b, nullptr, nullptr);
+    if (rc) return rc;
+    Ctl c;
+    rc = read_ctl(M, c);
+    if (rc) return rc;
+    c.bsl = b->h.bsl_now;
+    // out2in of z_bed and Hice into the zero-initialised state (:618-619)
+    rc = upload_embedded(h, M.z_bed, z_bed, 1e6);
+    if (rc) return rc;
+    rc = upload_embedded(h, M.Hice, H_ice, 0.0);
+    if (rc) return rc;
+    if (!h->ref_was_set) {
+        // isos_init_ref(z_bed, H_ice, now%bsl, now%dz_ss) (:621-624)
+        rc = upload_embedded(h, M.z_bed_ref, z_bed, 1e6);
+        if (rc) return rc;
+        rc = upload_embedded(h, M.Hice_ref, H_ice, 0.0);
+        if (rc) return rc;
+        c.ref_bsl = c.bsl;
+        HIPCHK(hipMemcpy(M.dz_ss_ref, M.dz_ss, nn * 8, hipMemcpyDeviceToDevice));
+        h->ref_was_set = true;
+    }
+    rc = write_ctl(M, c);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(M.H_in, H_ice, sizeof(double) * h->nx_ice * h->ny_ice, hipMemcpyHostToDevice, g_stream));
+    launch_init_state_fields(h->d_mem, 1, h->n, g_stream);
+    h->time_prog = time;
+    h->time_diag = time;
+    std::vector<fi_isos*> hs{h};
+    // the update inside isos_init_state runs before the ODE state exists: no step (:644)
+    rc = run_update(hs, h->d_mem, h->shape, time, false);
+    if (rc) return rc;
+    // ODE initial state (:654-656): x = now%w, kept in X[0]
+    rc = read_ctl(M, c);
+    if (rc) return rc;
+    HIPCHK(hipMemcpy(M.X[0], M.W, nn * 8, hipMemcpyDeviceToDevice));
+    c.dt = h->par.dt_init;
+    c.t = time;
+    c.cur = 0;
+    c.w_is_x = 0;
+    c.k_valid = 0;
+    rc = write_ctl(M, c);
+    if (rc) return rc;
+    h->state_init = true;
+    return 0;
+}
+
+int fi_isos_update(fi_isos* h, const double* H_ice, double time, fi_bsl* b) {
+    if (!h || !H_ice || !b) return fail(FI_E_ARG, "NULL argument");
+    if (!h->state_init) return fail(FI_E_STATE, "isos_update called before isos_init_state");
+    int rc = bind_bsl(h, b);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(h->hm.H_in, H_ice, sizeof(double) * h->nx_ice * h->ny_ice, hipMemcpyHostToDevice, g_stream));
+    std::vector<fi_isos*> hs{h};
+    return run_update(hs, h->d_mem, h->shape, time, true);
+}
+
+static const double* field_ptr(fi_isos* h, int f, bool& is_mask, int& bit) {
+    const Member& M = h->hm;
+    is_mask = false;
+    bit = 0;
+    switch (f) {
+    case FI_Z_BED: return M.z_bed;
+    case FI_Z_SS: return M.z_ss;
+    case FI_DZ_SS: return M.dz_ss;
+    case FI_W: return M.W;
+    case FI_WE: return M.we;
+    case FI_DWDT: return M.dwdt;
+    case FI_CANOM_FULL: return M.canom_full;
+    case FI_HICE: return M.Hice;
+    case FI_MASKOCEAN: is_mask = true; bit = 1; return nullptr;
+    case FI_MASKGROUNDED: is_mask = true; bit = 2; return nullptr;
+    case FI_MASKCONTINENT: is_mask = true; bit = 4; return nullptr;
+    case FI_MASKACTIVE: is_mask = true; bit = 8; return nullptr;
+    case FI_CANOM_MANTLE: return M.canom_mantle;
+    case FI_HAF: return M.Haf;
+    case FI_ETA_EFF: return h->eta_eff;
+    case FI_D_LITH: return h->Dd;
+    case FI_KAPPA: return h->kappa;
+    case FI_R: return h->R;
+    case FI_HE_LITH: return h->He;
+    case FI_ODE_X: return nullptr;
+    case FI_Z_SS_REF: return M.z_ss_ref;
+    case FI_Z_BED_REF: return M.z_bed_ref;
+    default: return nullptr;
+    }
+}
+
+int fi_isos_get_full(fi_isos* h, int field, double* dst) {
+    if (!h || !dst) return fail(FI_E_ARG, "NULL argument");
+    const size_t nn = (size_t)h->n * h->n;
+    bool is_mask;
+    int bit;
+    const double* p = field_ptr(h, field, is_mask, bit);
+    if (field == FI_ODE_X) {
+        Ctl c;
+        int rc = read_ctl(h->hm, c);
+        if (rc) return rc;
+        p = h->hm.X[c.cur];
+    }
+    HIPCHK(hipStreamSynchronize(g_stream));
+    if (is_mask) {
+        std::vector<uint8_t> m(nn);
+        HIPCHK(hipMemcpy(m.data(), h->hm.mask, nn, hipMemcpyDeviceToHost));
+        for (size_t c = 0; c < nn; ++c) dst[c] = (m[c] & bit) ? 1.0 : 0.0;
+        return 0;
+    }
+    if (!p) return fail(FI_E_ARG, "unknown field %d", field);
+    HIPCHK(hipMemcpy(dst, p, nn * 8, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int fi_isos_get(fi_isos* h, int field, double* dst) {
+    if (!h || !dst) return fail(FI_E_ARG, "NULL argument");
+    if (field < 0 || field > FI_MASKCONTINENT) return fail(FI_E_ARG, "field %d is not an out%% component", field);
+    bool is_mask;
+    int bit;
+    const double* p = field_ptr(h, field, is_mask, bit);
+    const int n = h->n, ni = h->nx_ice, nj = h->ny_ice;
+    HIPCHK(hipStreamSynchronize(g_stream));
+    if (is_mask) {
+        std::vector<uint8_t> m((size_t)ni * nj);
+        HIPCHK(hipMemcpy2D(m.data(), ni, h->hm.mask + (size_t)h->jcrop1 * n + h->icrop1, n, ni, nj,
+                           hipMemcpyDeviceToHost));
+        for (size_t c = 0; c < m.size(); ++c) dst[c] = (m[c] & bit) ? 1.0 : 0.0;
+        return 0;
+    }
+    HIPCHK(hipMemcpy2D(dst, (size_t)ni * 8, p + (size_t)h->jcrop1 * n + h->icrop1, (size_t)n * 8, (size_t)ni * 8, nj,
+                       hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int fi_isos_get_mask(fi_isos* h, int field, int8_t* dst) {
+    if (field < FI_MASKOCEAN || field > FI_MASKCONTINENT) return fail(FI_E_ARG, "not a mask field");
+    std::vector<double> tmp((size_t)h->nx_ice * h->ny_ice);
+    int rc = fi_isos_get(h, field, tmp.data());
+    if (rc) return rc;
+    for (size_t c = 0; c < tmp.size(); ++c) dst[c] = (int8_t)(tmp[c] != 0.0);
+    return 0;
+}
+
+int fi_isos_get_scalar(fi_isos* h, int which, double* out) {
+    if (!h || !out) return fail(FI_E_ARG, "NULL argument");
+    Ctl c;
+    int rc = read_ctl(h->hm, c);
+    if (rc) return rc;
+    switch (which) {
+    case FI_S_N: *out = h->n; break;
+    case FI_S_T: *out = c.t; break;
+    case FI_S_DT: *out = c.dt; break;
+    case FI_S_BSL: *out = c.bsl; break;
+    case FI_S_DELTAV: *out = c.deltaV; break;
+    case FI_S_V_AF: *out = c.V_af; break;
+    case FI_S_V_DEN: *out = c.V_den; break;
+    case FI_S_ATTEMPTS: *out = (double)c.attempts; break;
+    case FI_S_ACCEPTS: *out = (double)c.accepts; break;
+    case FI_S_D_LITH11: HIPCHK(hipMemcpy(out, h->Dd, 8, hipMemcpyDeviceToHost)); break;
+    case FI_S_ERR: *out = c.err; break;
+    case FI_S_TOL: *out = c.tol; break;
+    default: return fail(FI_E_ARG, "unknown scalar %d", which);
+    }
+    return 0;
+}
+
+int fi_isos_destroy(fi_isos* h) {
+    if (!h) return 0;
+    (void)hipStreamSynchronize(g_stream);
+    (void)hipFree(h->arena);
+    delete h;
+    return 0;
+}
+
+int fi_batch_create(fi_isos* const* members, int nmem, fi_batch** out) {
+    if (!members || nmem < 1 || !out) return fail(FI_E_ARG, "empty batch");
+    fi_batch* b = new fi_batch();
+    for (int m = 0; m < nmem; ++m) {
+        fi_isos* h = members[m];
+        if (!h || h->n != members[0]->n || h->fp64 != members[0]->fp64 || h->uniform != members[0]->uniform ||
+            h->par.dt_method != members[0]->par.dt_method || h->par.include_elastic != members[0]->par.include_elastic ||
+            h->par.heterogeneous_ssh != members[0]->par.heterogeneous_ssh) {
+            delete b;
+            return fail(FI_E_ARG, "batch members must share N, precision, distortion and dt_method");
+        }
+        b->mem.push_back(h);
+        b->hm.push_back(h->hm);
+    }
+    int rc = make_shape(members[0]->n, nmem, members[0]->fp64, members[0]->uniform, b->shape);
+    if (rc) { delete b; return rc; }
+    if (hipMalloc(&b->d_mem, sizeof(Member) * nmem) != hipSuccess) { delete b; return fail(FI_E_HIP, "hipMalloc"); }
+    HIPCHK(hipMemcpy(b->d_mem, b->hm.data(), sizeof(Member) * nmem, hipMemcpyHostToDevice));
+    *out = b;
+    return 0;
+}
+
+int fi_batch_update(fi_batch* b, const double* const* H_ice, double time, fi_bsl* const* bsl) {
+    if (!b || !bsl) return fail(FI_E_ARG, "NULL argument");
+    const int nmem = (int)b->mem.size();
+    bool dirty = false;
+    for (int m = 0; m < nmem; ++m) {
+        fi_isos* h = b->mem[m];
+        if (!h->state_init) return fail(FI_E_STATE, "member %d: update before init_state", m);
+        if (!bsl[m]) return fail(FI_E_ARG, "member %d: bsl is NULL", m);
+        if (b->hm[m].bsl != bsl[m]->d) { b->hm[m].bsl = bsl[m]->d; dirty = true; }
+        if (H_ice && H_ice[m])
+            HIPCHK(hipMemcpyAsync(h->hm.H_in, H_ice[m], sizeof(double) * h->nx_ice * h->ny_ice,
+                                  hipMemcpyHostToDevice, g_stream));
+    }
+    if (dirty) HIPCHK(hipMemcpy(b->d_mem, b->hm.data(), sizeof(Member) * nmem, hipMemcpyHostToDevice));
+    return run_update(b->mem, b->d_mem, b->shape, time, true);
+}
+
+int fi_batch_destroy(fi_batch* b) {
+    if (!b) return 0;
+    (void)hipStreamSynchronize(g_stream);
+    (void)hipFree(b->d_mem);
+    delete b;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,71 @@
+// fi_kernels.h -- host-callable launchers of the device kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "fi_types.h"
+
+namespace fi {
+
+// Launch geometry of one attempted step for a batch of equal-size domains.
+struct RhsShape {
+    int n;
+    int L;            // complex lines (= 2L real rows) per row-pass block
+    int C;            // complex lines (= 2C real columns) per column-pass block
+    int threads_row, threads_col;
+    FftPlan plan;
+    const void* tw;   // twiddles in RHS precision
+    const double2* tw64;
+    bool fp64;
+    bool uniform;     // K == 1
+};
+
+// Geometry of the 2N-point linear convolution.
+struct ConvShape {
+    int n, m;         // m = 2n
+    int Lr, Cc;       // lines per row / column block
+    int threads_row, threads_col;
+    FftPlan plan;     // length m
+    const void* tw;   // length-m twiddles in conv precision
+    bool fp64;
+};
+
+void launch_attempt(const Member* d_mem, int nmem, const RhsShape& s, hipStream_t st);
+
+// bookkeeping of isos_update (sealevel.f90, fastisostasy.f90:750-833)
+void launch_pre_haf(const Member* d_mem, int nmem, int n, bool new_H, hipStream_t st);
+void launch_pre_bsl(const Member* d_mem, int nmem, int n, hipStream_t st);
+// phase 0: z_ss/masks/loads and, without diagnostics, everything the RHS needs
+// phase 1: (after the elastic convolution) litho/full/mass anomaly
+// phase 2: (after the sea-surface convolution) z_ss/masks/loads/full
+void launch_pre_fields(const Member* d_mem, int nmem, int n, int phase, bool diag, bool fp64, hipStream_t st);
+void launch_begin(const Member* d_mem, int nmem, double tf, int stepping, hipStream_t st);
+void launch_finalize(const Member* d_mem, int nmem, int n, hipStream_t st);
+void launch_bsl_update(BslDev* b, double year_bp, hipStream_t st);
+void launch_set_ref_zss(const Member* d_mem, int nmem, int n, hipStream_t st);
+void launch_init_state_fields(const Member* d_mem, int nmem, int n, hipStream_t st);
+
+// linear convolution with a precomputed kernel spectrum (convolutions.f90:59-91)
+// src_kind: 0 elastic input (canom_load*g*K^2 taken from conv_work staging),
+// which = 0 elastic (writes we), 1 sea surface (writes dz_ss)
+void launch_convolution(const Member* d_mem, int nmem, const ConvShape& s, int which, hipStream_t st);
+void launch_kernel_spectrum(const Member* d_mem, int nmem, const ConvShape& s, int which,
+                            const double* d_kernel, hipStream_t st);
+
+// init-time fields (lv_elva.f90:442-481, 207-306; green_functions.f90:43-181)
+struct InitArgs {
+    int n;
+    double dx;
+    int nl;
+    int lithosphere, mantle, layering, lumping, vscale_method;
+    double vscale, zl[8], visc[8], dl, eta_ref, E, nu, comp_corr;
+    double r_earth, m_earth;
+};
+void launch_init_rheology(const Member* d_mem, const InitArgs& a, const double* d_he, const double* d_eta,
+                          double* d_etaeff, double* d_kappa, double* d_R, double* d_He, double* d_Dd,
+                          double* d_scratch, bool fp64, hipStream_t st);
+void launch_init_finalize(const InitArgs& a, const double* d_scratch, double* d_etaeff, const double* d_kappa,
+                          const double* d_R, const double* d_Dd, void* D_t, void* eta_t, void* kR_t,
+                          bool fp64, hipStream_t st);
+void launch_green(int which, int n, double dx, double r_earth, double m_earth, double* d_out, hipStream_t st);
+
+}  // namespace fi

@@ -1,0 +1,423 @@
+// rhs.hip -- LV-ELVA right-hand side and adaptive step on the GPU.
+//
+// One attempted step of the degenerate Tsitouras loop (integrators.f90:163-200)
+// for a batch of domains of equal size N, one blockIdx.y per domain:
+//
+//   k_stencil      FD flexure + load -> f (lv_elva.f90:95-113)
+//   k_row_fwd      Hartley transform along x (first half of DHT2)
+//   k_col          along y: DHT, divide by kappa*R (lv_elva.f90:115), DHT;
+//                  also the partial sums that give the four corners of the
+//                  inverse transform (apply_zerobc_at_corners_dp)
+//   k_row_inv      along x: DHT, /N^2, corner BC, * sec_per_year
+//                  (lv_elva.f90:118-123), then the stage sums x1/x2 and the
+//                  per-block max|x1|, max|x2|, sum (x2-x1)^2
+//   k_epi          same epilogue from a stored k (after a rejected attempt the
+//                  RHS is unchanged because it reads now%w, fastisostasy.f90:720)
+//   k_decide       scalar tol, error norm, accept / reject, dt control
+//
+// Every kernel reads the domain's Ctl and returns at once if the domain is not
+// stepping, so the same launch sequence serves a batch whose members need
+// different numbers of attempts.
+#include "fi_device.hpp"
+#include "fi_kernels.h"
+
+namespace fi {
+
+// ------------------------------------------------------------------ helpers
+__device__ __forceinline__ double attempt_dt(const Ctl& c, const PhysDev& ph) {
+    // integrators.f90:63,100-101,165
+    double dt = c.dt;
+    const double rem = c.tf - c.t;
+    if (ph.dt_method == 1) return fmin(dt, rem);  // rk4: h, not persisted
+    dt = fmin(dt, rem);
+    if (ph.dt_method == 2) dt = fmin(dt, ph.dt_max);
+    return dt;
+}
+
+template <typename T> __device__ __forceinline__ T ld(const void* p, size_t i) { return ((const T*)p)[i]; }
+
+// ------------------------------------------------------------------ stencil
+constexpr int ST_TX = 64, ST_TY = 8;
+
+template <typename T, bool UNIFORM>
+__global__ void __launch_bounds__(256) k_stencil(const Member* __restrict__ mem) {
+    const Member& M = mem[blockIdx.z];
+    const Ctl& c = *M.ctl;
+    if (c.status != ST_RUN || c.k_valid) return;
+    const PhysDev& ph = *M.phys;
+    const int N = M.n;
+    const int i0 = blockIdx.x * ST_TX, j0 = blockIdx.y * ST_TY;
+    if (i0 >= N || j0 >= N) return;
+    const double* __restrict__ w = c.w_is_x ? M.X[c.cur] : M.W;
+    const double* __restrict__ dxm = M.dxm;
+    const double dxs = ph.dx;
+    auto DX = [&](int i, int j) -> double { return UNIFORM ? dxs : dxm[(size_t)j * N + i]; };
+
+    constexpr int W0 = ST_TX + 4, H0 = ST_TY + 4;   // w:      [i0-2, i0+TX+2) x [j0-2, j0+TY+2)
+    constexpr int W1 = ST_TX + 2, H1 = ST_TY + 4;   // w_x:    [i0-1, i0+TX+1) x [j0-2, j0+TY+2)
+    constexpr int W2 = ST_TX + 2, H2 = ST_TY + 2;   // M**:    [i0-1, i0+TX+1) x [j0-1, j0+TY+1)
+    constexpr int W3 = ST_TX, H3 = ST_TY + 2;       // Mxy_x:  [i0,   i0+TX)   x [j0-1, j0+TY+1)
+    __shared__ double sw[H0 * W0];
+    __shared__ double swx[H1 * W1];
+    __shared__ double sMxx[H2 * W2], sMyy[H2 * W2], sMxy[H2 * W2];
+    __shared__ double sMxyx[H3 * W3];
+    const int tid = threadIdx.x, nt = blockDim.x;
+#define SW(i, j) sw[((j) - (j0 - 2)) * W0 + ((i) - (i0 - 2))]
+#define SWX(i, j) swx[((j) - (j0 - 2)) * W1 + ((i) - (i0 - 1))]
+#define S2(a, i, j) a[((j) - (j0 - 1)) * W2 + ((i) - (i0 - 1))]
+#define S3(i, j) sMxyx[((j) - (j0 - 1)) * W3 + ((i) - i0)]
+
+    for (int q = tid; q < H0 * W0; q += nt) {
+        const int i = i0 - 2 + q % W0, j = j0 - 2 + q / W0;
+        if (i >= 0 && i < N && j >= 0 && j < N) SW(i, j) = w[(size_t)j * N + i];
+    }
+    __syncthreads();
+    // w_x = Dx(w)  (finite_differences.f90:17-41)
+    for (int q = tid; q < H1 * W1; q += nt) {
+        const int i = i0 - 1 + q % W1, j = j0 - 2 + q / W1;
+        if (i < 0 || i >= N || j < 0 || j >= N) continue;
+        double v;
+        if (i == 0) v = (SW(1, j) - SW(0, j)) / DX(0, j);
+        else if (i == N - 1) v = (SW(N - 1, j) - SW(N - 2, j)) / DX(N - 1, j);
+        else v = (SW(i + 1, j) - SW(i - 1, j)) / (2 * DX(i, j));
+        SWX(i, j) = v;
+    }
+    __syncthreads();
+    // w_xx, w_yy, w_xy = Dy(w_x) and the plate moments (lv_elva.f90:95-104)
+    const double nu = ph.nu;
+    for (int q = tid; q < H2 * W2; q += nt) {
+        const int i = i0 - 1 + q % W2, j = j0 - 1 + q / W2;
+        if (i < 0 || i >= N || j < 0 || j >= N) continue;
+        double wxx, wyy, wxy;
+        if (i == 0) wxx = (SW(2, j) - 2 * SW(1, j) + SW(0, j)) / (DX(0, j) * DX(0, j));
+        else if (i == N - 1) wxx = (SW(N - 1, j) - 2 * SW(N - 2, j) + SW(N - 3, j)) / (DX(0, j) * DX(0, j));
+        else wxx = (SW(i + 1, j) - 2 * SW(i, j) + SW(i - 1, j)) / (DX(i, j) * DX(i, j));
+        if (j == 0) wyy = (SW(i, 2) - 2 * SW(i, 1) + SW(i, 0)) / (DX(i, 0) * DX(i, 0));
+        else if (j == N - 1) wyy = (SW(i, N - 1) - 2 * SW(i, N - 2) + SW(i, N - 3)) / (DX(i, N - 1) * DX(i, N - 1));
+        else wyy = (SW(i, j + 1) - 2 * SW(i, j) + SW(i, j - 1)) / (DX(i, j) * DX(i, j));
+        if (j == 0) wxy = (SWX(i, 1) - SWX(i, 0)) / DX(i, 0);
+        else if (j == N - 1) wxy = (SWX(i, N - 1) - SWX(i, N - 2)) / DX(i, N - 1);
+        else wxy = (SWX(i, j + 1) - SWX(i, j - 1)) / (2 * DX(i, j));
+        const double D = (double)ld<T>(M.D, (size_t)j * N + i);
+        S2(sMxx, i, j) = -(D * (wxx + nu * wyy));
+        S2(sMyy, i, j) = -(D * (wyy + nu * wxx));
+        S2(sMxy, i, j) = -(D * (1.0 - nu) * wxy);
+    }
+    __syncthreads();
+    // Mxy_x = Dx(Mxy)
+    for (int q = tid; q < H3 * W3; q += nt) {
+        const int i = i0 + q % W3, j = j0 - 1 + q / W3;
+        if (i >= N || j < 0 || j >= N) continue;
+        double v;
+        if (i == 0) v = (S2(sMxy, 1, j) - S2(sMxy, 0, j)) / DX(0, j);
+        else if (i == N - 1) v = (S2(sMxy, N - 1, j) - S2(sMxy, N - 2, j)) / DX(N - 1, j);
+        else v = (S2(sMxy, i + 1, j) - S2(sMxy, i - 1, j)) / (2 * DX(i, j));
+        S3(i, j) = v;
+    }
+    __syncthreads();
+    // f = (p + Mx_xx + 2 Mxy_xy + My_yy) / (2 eta)   (lv_elva.f90:107-113)
+    const double g = ph.g;
+    for (int q = tid; q < ST_TX * ST_TY; q += nt) {
+        const int i = i0 + q % ST_TX, j = j0 + q / ST_TX;
+        if (i >= N || j >= N) continue;
+        double mxx, myy, mxyxy;
+        if (i == 0) mxx = (S2(sMxx, 2, j) - 2 * S2(sMxx, 1, j) + S2(sMxx, 0, j)) / (DX(0, j) * DX(0, j));
+        else if (i == N - 1) mxx = (S2(sMxx, N - 1, j) - 2 * S2(sMxx, N - 2, j) + S2(sMxx, N - 3, j)) / (DX(0, j) * DX(0, j));
+        else mxx = (S2(sMxx, i + 1, j) - 2 * S2(sMxx, i, j) + S2(sMxx, i - 1, j)) / (DX(i, j) * DX(i, j));
+        if (j == 0) myy = (S2(sMyy, i, 2) - 2 * S2(sMyy, i, 1) + S2(sMyy, i, 0)) / (DX(i, 0) * DX(i, 0));
+        else if (j == N - 1) myy = (S2(sMyy, i, N - 1) - 2 * S2(sMyy, i, N - 2) + S2(sMyy, i, N - 3)) / (DX(i, N - 1) * DX(i, N - 1));
+        else myy = (S2(sMyy, i, j + 1) - 2 * S2(sMyy, i, j) + S2(sMyy, i, j - 1)) / (DX(i, j) * DX(i, j));
+        if (j == 0) mxyxy = (S3(i, 1) - S3(i, 0)) / DX(i, 0);
+        else if (j == N - 1) mxyxy = (S3(i, N - 1) - S3(i, N - 2)) / DX(i, N - 1);
+        else mxyxy = (S3(i, j + 1) - S3(i, j - 1)) / (2 * DX(i, j));
+        const size_t cidx = (size_t)j * N + i;
+        const double p = -(g * (double)ld<T>(M.cfull, cidx));
+        const double num = p + mxx + 2.0 * mxyxy + myy;
+        T f;
+        if constexpr (sizeof(T) == 8) f = (T)(num / (2.0 * (double)ld<T>(M.eta, cidx)));
+        else f = (T)(num * (double)ld<T>(M.eta, cidx));   // mixed: eta holds 1/(2 eta)
+        ((T*)M.g1)[cidx] = f;
+    }
+#undef SW
+#undef SWX
+#undef S2
+#undef S3
+}
+
+// ------------------------------------------------------------------ row pass
+// Block: 2L rows [r0, r0 + 2L); rows r0+l (real part) and r0+L+l (imaginary
+// part) share complex line l.  INV selects the inverse pass + epilogue.
+template <typename T>
+__device__ __forceinline__ void load_rows(cplx<T>* buf, const T* __restrict__ src, int N, int L, int r0) {
+    const int tid = threadIdx.x, nt = blockDim.x;
+    T* bf = (T*)buf;
+    for (int q = tid; q < 2 * L * N; q += nt) {
+        const int mr = q / N, i = q % N;
+        const int l = mr % L, part = mr / L;
+        bf[2 * (i * L + l) + part] = src[(size_t)(r0 + mr) * N + i];
+    }
+    __syncthreads();
+}
+
+template <typename T>
+__global__ void __launch_bounds__(512) k_row_fwd(const Member* __restrict__ mem, FftPlan plan,
+                                                 const void* __restrict__ twv, int L) {
+    const Member& M = mem[blockIdx.y];
+    const Ctl& c = *M.ctl;
+    if (c.status != ST_RUN || c.k_valid) return;
+    extern __shared__ __align__(16) unsigned char smem[];
+    cplx<T>* buf = (cplx<T>*)smem;
+    const int N = M.n, r0 = blockIdx.x * 2 * L;
+    const cplx<T>* tw = (const cplx<T>*)twv;
+    load_rows<T>(buf, (const T*)M.g1, N, L, r0);
+    lds_fft<T>(buf, plan, L, L, tw);
+    T* out = (T*)M.g2;
+    for (int q = threadIdx.x; q < 2 * L * N; q += blockDim.x) {
+        const int mr = q / N, k = q % N;
+        const int l = mr % L, part = mr / L;
+        T ha, hb;
+        dht_pair(buf[k * L + l], buf[((N - k) % N) * L + l], ha, hb);
+        out[(size_t)(r0 + mr) * N + k] = part ? hb : ha;
+    }
+}
+
+// ------------------------------------------------------------------ column pass
+// Block: real columns [c0, c0 + 2C); columns c0+l and c0+C+l share line l.
+template <typename T>
+__global__ void __launch_bounds__(512) k_col(const Member* __restrict__ mem, FftPlan plan,
+                                             const void* __restrict__ twv, const double2* __restrict__ tw64,
+                                             int C) {
+    const Member& M = mem[blockIdx.y];
+    const Ctl& c = *M.ctl;
+    if (c.status != ST_RUN || c.k_valid) return;
+    extern __shared__ __align__(16) unsigned char smem[];
+    cplx<T>* buf = (cplx<T>*)smem;
+    const int N = M.n, c0 = blockIdx.x * 2 * C;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const cplx<T>* tw = (const cplx<T>*)twv;
+    const T* __restrict__ src = (const T*)M.g2;
+    const T* __restrict__ kR = (const T*)M.kR;
+    T* bf = (T*)buf;
+    for (int q = tid; q < 2 * C * N; q += nt) {
+        const int k = q / (2 * C), mc = q % (2 * C);
+        bf[2 * (k * C + (mc % C)) + mc / C] = src[(size_t)k * N + c0 + mc];
+    }
+    __syncthreads();
+    lds_fft<T>(buf, plan, C, C, tw);
+    // Hartley values, spectral divide, re-pack (lv_elva.f90:115)
+    for (int q = tid; q < (N / 2 + 1) * C; q += nt) {
+        const int l = q % C, k = q / C, km = (N - k) % N;
+        const cplx<T> zk = buf[k * C + l], zm = buf[km * C + l];
+        T ha, hb, ham, hbm;
+        dht_pair(zk, zm, ha, hb);
+        dht_pair(zm, zk, ham, hbm);
+        ha = ha / kR[(size_t)k * N + c0 + l];
+        hb = hb / kR[(size_t)k * N + c0 + C + l];
+        buf[k * C + l] = mk<T>(ha, hb);
+        if (km != k) {
+            ham = ham / kR[(size_t)km * N + c0 + l];
+            hbm = hbm / kR[(size_t)km * N + c0 + C + l];
+            buf[km * C + l] = mk<T>(ham, hbm);
+        }
+    }
+    __syncthreads();
+    lds_fft<T>(buf, plan, C, C, tw);
+    T* dst = (T*)M.g1;
+    for (int q = tid; q < 2 * C * N; q += nt) {
+        const int k = q / (2 * C), mc = q % (2 * C);
+        const int l = mc % C;
+        T ha, hb;
+        dht_pair(buf[k * C + l], buf[((N - k) % N) * C + l], ha, hb);
+        dst[(size_t)k * N + c0 + mc] = (mc / C) ? hb : ha;
+    }
+    // corner partials: rows 0 and N-1 summed with weights 1 and cas(2 pi i (N-1)/N)
+    if (tid < 4) {
+        const int k = (tid < 2) ? 0 : N - 1;
+        const bool cs = tid & 1;
+        double acc = 0.0;
+        for (int mc = 0; mc < 2 * C; ++mc) {
+            const int l = mc % C;
+            T ha, hb;
+            dht_pair(buf[k * C + l], buf[((N - k) % N) * C + l], ha, hb);
+            const double v = (double)((mc / C) ? hb : ha);
+            const int col = c0 + mc;
+            acc += cs ? v * (tw64[col].x + tw64[col].y) : v;
+        }
+        M.corners[(size_t)blockIdx.x * 4 + tid] = acc;
+    }
+}
+
+// ------------------------------------------------------------------ epilogue
+struct EpiOut { double x1max, x2max, d2; };
+
+// Stage sums of integrators.f90:12-203 for one cell, in the reference's
+// association order.  Returns x2 (the candidate / new state).
+__device__ __forceinline__ double stage_sums(int method, double x, double k, double dt,
+                                             double& x1) {
+    if (method == 0) { x1 = x + dt * k; return x1; }                       // euler
+    if (method == 1) { x1 = x + (dt / 6) * (((k + 2 * k) + 2 * k) + k); return x1; }  // rk4
+    if (method == 2) {                                                     // bs32
+        x1 = x + dt / 9 * ((2 * k + 3 * k) + 4 * k);
+        return x + dt * (((0.0 * k + 0.0 * k) + 0.0 * k) + 3 * k);
+    }
+    // tsit54: b = (0.0845, 0, 0, 0.5179, 0.1276, 0.27), bh(6) = 0.27 - 1/5
+    const double b1 = dt * 0.0845, b4 = dt * 0.5179, b5 = dt * 0.1276, b6 = dt * 0.2700;
+    const double z = dt * 0.0, h6 = dt * (0.2700 - 1 / 5.0);
+    x1 = x + b1 * k + z * k + z * k + b4 * k + b5 * k + h6 * k;
+    return x + b1 * k + z * k + z * k + b4 * k + b5 * k + b6 * k;
+}
+
+__device__ __forceinline__ void block_reduce3(double a, double b, double s, double* out) {
+    __shared__ double red[3][16];
+    a = wave_max(a); b = wave_max(b); s = wave_sum(s);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) { red[0][wv] = a; red[1][wv] = b; red[2][wv] = s; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int nw = (blockDim.x + 63) >> 6;
+        double A = red[0][0], B = red[1][0], S = red[2][0];
+        for (int q = 1; q < nw; ++q) { A = fmax(A, red[0][q]); B = fmax(B, red[1][q]); S += red[2][q]; }
+        out[0] = A; out[1] = B; out[2] = S;
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(512) k_row_inv(const Member* __restrict__ mem, FftPlan plan,
+                                                 const void* __restrict__ twv, int L, int ncolblk) {
+    const Member& M = mem[blockIdx.y];
+    const Ctl& c = *M.ctl;
+    if (c.status != ST_RUN || c.k_valid) return;
+    const PhysDev& ph = *M.phys;
+    extern __shared__ __align__(16) unsigned char smem[];
+    cplx<T>* buf = (cplx<T>*)smem;
+    __shared__ double corner_c;
+    const int N = M.n, r0 = blockIdx.x * 2 * L;
+    const cplx<T>* tw = (const cplx<T>*)twv;
+    if (threadIdx.x == 0) {
+        // four corners of the raw inverse transform, fixed summation order
+        double v[4] = {0, 0, 0, 0};
+        for (int b = 0; b < ncolblk; ++b)
+            for (int q = 0; q < 4; ++q) v[q] += M.corners[(size_t)b * 4 + q];
+        const double S = (double)(float)((double)N * (double)N);
+        // x(1,1) + x(nx,ny) + x(1,ny) + x(nx,1)  (isos_utils.f90:116)
+        corner_c = 0.25 * (v[0] / S + v[3] / S + v[2] / S + v[1] / S);
+    }
+    load_rows<T>(buf, (const T*)M.g1, N, L, r0);   // also synchronises corner_c
+    lds_fft<T>(buf, plan, L, L, tw);
+    const double S = (double)(float)((double)N * (double)N);
+    const double cc = corner_c, spy = ph.sec_per_year;
+    const double dt = attempt_dt(c, ph);
+    const double* __restrict__ X = M.X[c.cur];
+    double* __restrict__ Xn = M.X[c.cur ^ 1];
+    double a1 = 0.0, a2 = 0.0, s2 = 0.0;
+    for (int q = threadIdx.x; q < 2 * L * N; q += blockDim.x) {
+        const int mr = q / N, i = q % N;
+        const int l = mr % L, part = mr / L;
+        T ha, hb;
+        dht_pair(buf[i * L + l], buf[((N - i) % N) * L + l], ha, hb);
+        const double raw = (double)(part ? hb : ha);
+        const double k = (raw / S - cc) * spy;
+        const size_t cidx = (size_t)(r0 + mr) * N + i;
+        const double x = X[cidx];
+        double x1;
+        const double x2 = stage_sums(ph.dt_method, x, k, dt, x1);
+        M.dwdt[cidx] = k;
+        Xn[cidx] = x2;
+        a1 = fmax(a1, fabs(x1)); a2 = fmax(a2, fabs(x2));
+        const double d = x2 - x1;
+        s2 += d * d;
+    }
+    block_reduce3(a1, a2, s2, M.partials + (size_t)blockIdx.x * 4);
+}
+
+// Candidate from a stored k (after a rejection).  Same block decomposition
+// as k_row_inv so that the partial layout is identical.
+__global__ void __launch_bounds__(256) k_epi(const Member* __restrict__ mem, int L) {
+    const Member& M = mem[blockIdx.y];
+    const Ctl& c = *M.ctl;
+    if (c.status != ST_RUN || !c.k_valid) return;
+    const PhysDev& ph = *M.phys;
+    const int N = M.n, r0 = blockIdx.x * 2 * L;
+    const double dt = attempt_dt(c, ph);
+    const double* __restrict__ X = M.X[c.cur];
+    double* __restrict__ Xn = M.X[c.cur ^ 1];
+    double a1 = 0.0, a2 = 0.0, s2 = 0.0;
+    for (int q = threadIdx.x; q < 2 * L * N; q += blockDim.x) {
+        const size_t cidx = (size_t)r0 * N + q;
+        double x1;
+        const double x2 = stage_sums(ph.dt_method, X[cidx], M.dwdt[cidx], dt, x1);
+        Xn[cidx] = x2;
+        a1 = fmax(a1, fabs(x1)); a2 = fmax(a2, fabs(x2));
+        const double d = x2 - x1;
+        s2 += d * d;
+    }
+    block_reduce3(a1, a2, s2, M.partials + (size_t)blockIdx.x * 4);
+}
+
+// One wave per domain: error norm and step control (integrators.f90:186-199).
+__global__ void __launch_bounds__(64) k_decide(const Member* __restrict__ mem, int nblk) {
+    const Member& M = mem[blockIdx.x];
+    Ctl& c = *M.ctl;
+    if (c.status != ST_RUN) return;
+    const PhysDev& ph = *M.phys;
+    const int lane = threadIdx.x;
+    double a1 = 0.0, a2 = 0.0, s2 = 0.0;
+    for (int b = lane; b < nblk; b += 64) {
+        a1 = fmax(a1, M.partials[(size_t)b * 4 + 0]);
+        a2 = fmax(a2, M.partials[(size_t)b * 4 + 1]);
+        s2 += M.partials[(size_t)b * 4 + 2];
+    }
+    a1 = wave_max(a1); a2 = wave_max(a2); s2 = wave_sum(s2);
+    if (lane != 0) return;
+    const double dt = attempt_dt(c, ph);
+    c.attempts++;
+    const int m = ph.dt_method;
+    bool accept = true;
+    if (m == 2 || m == 3) {
+        const double tol = ph.atol + ph.rtol * fmax(a1, a2);
+        const double nn = (double)M.n * (double)M.n;
+        const double err = sqrt(s2 / (tol * tol) / nn);
+        c.tol = tol; c.err = err;
+        accept = err < 1;
+        if (accept) c.dt = fmin(dt * pow(1 / err, 1 / ph.q), ph.dt_max);
+        else c.dt = dt * 0.5;
+    } else if (m == 0) {
+        c.dt = dt;
+    }
+    if (accept) {
+        c.t = c.t + dt;
+        c.cur ^= 1;
+        c.w_is_x = 1;
+        c.k_valid = 0;
+        c.accepts++;
+    } else {
+        c.k_valid = 1;
+    }
+    if (!(c.t < c.tf)) c.status = ST_IDLE;
+}
+
+// ------------------------------------------------------------------ launchers
+template <typename T>
+static size_t lds_bytes(int n, int lines) { return (size_t)n * lines * sizeof(cplx<T>); }
+
+template <typename T>
+void launch_attempt_t(const Member* d_mem, int nmem, const RhsShape& s, hipStream_t st) {
+    dim3 gs((s.n + ST_TX - 1) / ST_TX, (s.n + ST_TY - 1) / ST_TY, nmem);
+    if (s.uniform) hipLaunchKernelGGL((k_stencil<T, true>), gs, dim3(256), 0, st, d_mem);
+    else hipLaunchKernelGGL((k_stencil<T, false>), gs, dim3(256), 0, st, d_mem);
+    hipLaunchKernelGGL(k_row_fwd<T>, dim3(s.n / (2 * s.L), nmem), dim3(s.threads_row),
+                       lds_bytes<T>(s.n, s.L), st, d_mem, s.plan, s.tw, s.L);
+    hipLaunchKernelGGL(k_col<T>, dim3(s.n / (2 * s.C), nmem), dim3(s.threads_col),
+                       lds_bytes<T>(s.n, s.C), st, d_mem, s.plan, s.tw, s.tw64, s.C);
+    hipLaunchKernelGGL(k_row_inv<T>, dim3(s.n / (2 * s.L), nmem), dim3(s.threads_row),
+                       lds_bytes<T>(s.n, s.L), st, d_mem, s.plan, s.tw, s.L, s.n / (2 * s.C));
+    hipLaunchKernelGGL(k_epi, dim3(s.n / (2 * s.L), nmem), dim3(256), 0, st, d_mem, s.L);
+    hipLaunchKernelGGL(k_decide, dim3(nmem), dim3(64), 0, st, d_mem, s.n / (2 * s.L));
+}
+
+void launch_attempt(const Member* d_mem, int nmem, const RhsShape& s, hipStream_t st) {
+    if (s.fp64) launch_attempt_t<double>(d_mem, nmem, s, st);
+    else launch_attempt_t<float>(d_mem, nmem, s, st);
+}
+
+}  // namespace fi

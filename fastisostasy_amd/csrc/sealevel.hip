@@ -1,0 +1,303 @@
+// sealevel.hip -- per-update bookkeeping of isos_update on the GPU.
+//
+//   k_pre_haf     Hice embedding (out2in), calc_Haf and the two domain sums of
+//                 calc_sl_contributions (sealevel.f90:132-145, 189-205)
+//   k_pre_bsl     V_af / V_den / deltaV and the barystatic sea-level update
+//                 (fastisostasy.f90:763-778).  One thread walks the domains in
+//                 batch order, so domains sharing a bsl object see the same
+//                 sequence of updates as consecutive isos_update calls.
+//   k_pre_fields  z_ss, masks and the column anomalies (sealevel.f90:25-187,
+//                 fastisostasy.f90:778-833) in three phases around the two
+//                 Green's-function convolutions
+//   k_finalize    corner BC of now%w, canom_mantle, z_bed and the instability
+//                 maxima (fastisostasy.f90:856-869)
+// All bookkeeping is FP64; masks are bit-packed in one byte per cell.
+#include "fi_device.hpp"
+#include "fi_kernels.h"
+
+namespace fi {
+
+constexpr int PRE_THREADS = 256, PRE_CELLS = 1024;   // cells per block
+
+enum { MB_OCEAN = 1, MB_GROUNDED = 2, MB_CONTINENT = 4, MB_ACTIVE = 8 };
+
+__device__ __forceinline__ double calc_haf(double z_ss, double z_bed, double Hice, const PhysDev& ph) {
+    double h = (z_ss - z_bed) * (ph.rho_seawater / ph.rho_ice);
+    if (h < 0.0) h = 0.0;
+    h = Hice - h;
+    if (h < 0.0) h = 0.0;
+    return h;
+}
+
+__device__ __forceinline__ uint8_t calc_mask(double z_bed, double z_ss, double Hice, double Haf, uint8_t keep) {
+    const bool cont = z_bed > z_ss;
+    const bool grd = (Hice > 0.0) && (Haf > 0.0);
+    const bool ocean = !(cont || grd);
+    return (uint8_t)((keep & MB_ACTIVE) | (ocean ? MB_OCEAN : 0) | (grd ? MB_GROUNDED : 0) | (cont ? MB_CONTINENT : 0));
+}
+
+__device__ __forceinline__ double block_sum(double v, double* red) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[wv] = v;
+    __syncthreads();
+    double s = red[0];
+    for (int q = 1; q < (int)(blockDim.x >> 6); ++q) s += red[q];
+    return s;
+}
+
+__global__ void __launch_bounds__(PRE_THREADS) k_pre_haf(const Member* __restrict__ mem, int new_H) {
+    const Member& M = mem[blockIdx.y];
+    const PhysDev& ph = *M.phys;
+    const int N = M.n;
+    const size_t nn = (size_t)N * N;
+    double s1 = 0.0, s2 = 0.0;
+    const double A_uni = ph.dx * ph.dx;
+    for (int q = threadIdx.x; q < PRE_CELLS; q += PRE_THREADS) {
+        const size_t c = (size_t)blockIdx.x * PRE_CELLS + q;
+        if (c >= nn) break;
+        double H;
+        if (new_H) {
+            const int i = (int)(c % N) - M.icrop1, j = (int)(c / N) - M.jcrop1;
+            H = (i >= 0 && i < M.nx_ice && j >= 0 && j < M.ny_ice) ? M.H_in[(size_t)j * M.nx_ice + i] : 0.0;
+            M.Hice[c] = H;
+        } else {
+            H = M.Hice[c];
+        }
+        const double haf = calc_haf(M.z_ss[c], M.z_bed[c], H, ph);
+        M.Haf[c] = haf;
+        const double A = ph.uniform_K ? A_uni : M.A[c];
+        s1 += (haf - M.Haf_ref[c]) * A;
+        s2 += (H - M.Hice_ref[c]) * ph.Vden_factor * A;
+    }
+    __shared__ double red[16];
+    s1 = block_sum(s1, red);
+    s2 = block_sum(s2, red);
+    if (threadIdx.x == 0) {
+        M.partials[(size_t)blockIdx.x * 4 + 0] = s1;
+        M.partials[(size_t)blockIdx.x * 4 + 1] = s2;
+    }
+}
+
+__global__ void __launch_bounds__(64) k_pre_bsl(const Member* __restrict__ mem, int nmem, int nblk) {
+    const int lane = threadIdx.x;
+    for (int m = 0; m < nmem; ++m) {
+        const Member& M = mem[m];
+        double s1 = 0.0, s2 = 0.0;
+        for (int b = lane; b < nblk; b += 64) { s1 += M.partials[(size_t)b * 4]; s2 += M.partials[(size_t)b * 4 + 1]; }
+        s1 = wave_sum(s1); s2 = wave_sum(s2);
+        if (lane == 0) {
+            const PhysDev& ph = *M.phys;
+            Ctl& c = *M.ctl;
+            const double V_old = c.V_af + c.V_den + c.V_pov;
+            c.V_af = s1 * (ph.rho_ice / ph.rho_seawater);
+            c.V_den = s2;
+            c.V_pov = 0.0;
+            c.deltaV = (c.V_af + c.V_den + c.V_pov) - V_old;
+            BslDev& b = *M.bsl;
+            if (b.method == 1) b.bsl_now = b.bsl_now - c.deltaV / b.A_ocean_now;
+            c.bsl = b.bsl_now;
+        }
+        __syncthreads();
+    }
+}
+
+// phase 0: first pass after the BSL update; phase 1: after the elastic
+// convolution; phase 2: after the sea-surface convolution.
+template <typename T>
+__global__ void __launch_bounds__(PRE_THREADS) k_pre_fields(const Member* __restrict__ mem, int phase, int diag) {
+    const Member& M = mem[blockIdx.y];
+    const PhysDev& ph = *M.phys;
+    const Ctl& ctl = *M.ctl;
+    const int N = M.n;
+    const size_t nn = (size_t)N * N;
+    const double bsl = ctl.bsl;
+    for (int q = threadIdx.x; q < PRE_CELLS; q += PRE_THREADS) {
+        const size_t c = (size_t)blockIdx.x * PRE_CELLS + q;
+        if (c >= nn) break;
+        const uint8_t mold = M.mask[c];
+        const bool active = mold & MB_ACTIVE;
+        if (phase == 1) {
+            // canom_litho, canom_full, mass_anom (fastisostasy.f90:811-813)
+            const double litho = 0.0 + ph.rho_litho * (M.we[c] - M.we_ref[c]);
+            const double full = active ? (M.canom_load[c] + litho + M.canom_mantle[c]) : 0.0;
+            const double A = ph.uniform_K ? ph.dx * ph.dx : M.A[c];
+            if (ph.heterogeneous_ssh) M.conv_in[c] = active ? A * full : 0.0;
+            else M.dz_ss[c] = 0.0;
+            continue;
+        }
+        // calc_z_ss, calc_masks, seawater and load anomalies
+        const double z_ss = bsl + M.z_ss_ref[c] + M.dz_ss[c];
+        const double z_bed = M.z_bed[c], H = M.Hice[c], haf = M.Haf[c];
+        const uint8_t mk = calc_mask(z_bed, z_ss, H, haf, mold);
+        M.z_ss[c] = z_ss;
+        M.mask[c] = mk;
+        const double ice = 0.0 + ph.rho_ice * (haf - M.Haf_ref[c]);
+        const double sw = 0.0 + ((mk & MB_OCEAN) ? ph.rho_water * (z_ss - M.z_ss_ref[c]) : 0.0);
+        double load = ice;
+        if (ph.interactive_sealevel) load = load + sw;
+        load = active ? load : 0.0;
+        if (phase == 0 && diag) {
+            M.canom_load[c] = load;
+            if (ph.include_elastic) {
+                const double K2 = ph.uniform_K ? 1.0 : M.K2[c];
+                M.conv_in[c] = load * ph.g * K2;
+            } else {
+                M.we[c] = 0.0;
+            }
+            continue;
+        }
+        const double litho = 0.0 + ph.rho_litho * (M.we[c] - M.we_ref[c]);
+        const double full = active ? (load + litho + M.canom_mantle[c]) : 0.0;
+        M.canom_full[c] = full;
+        ((T*)M.cfull)[c] = (T)full;
+    }
+}
+
+// Corner correction of now%w (isos_utils.f90:104-110), one thread per domain.
+__global__ void k_fin_corner(const Member* __restrict__ mem, int nmem) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= nmem) return;
+    const Member& M = mem[m];
+    Ctl& c = *M.ctl;
+    const int N = M.n;
+    const double* w = c.w_is_x ? M.X[c.cur] : M.W;
+    c.wcorner = 0.25 * (w[0] + w[(size_t)N * N - 1] + w[(size_t)(N - 1) * N] + w[N - 1]);
+}
+
+__global__ void __launch_bounds__(PRE_THREADS) k_finalize(const Member* __restrict__ mem) {
+    const Member& M = mem[blockIdx.y];
+    const PhysDev& ph = *M.phys;
+    const Ctl& ctl = *M.ctl;
+    const int N = M.n;
+    const size_t nn = (size_t)N * N;
+    const double* __restrict__ wsrc = ctl.w_is_x ? M.X[ctl.cur] : M.W;
+    const double cw = ctl.wcorner;
+    double mw = 0.0, mwe = 0.0, mdz = 0.0, nanf = 0.0;
+    for (int q = threadIdx.x; q < PRE_CELLS; q += PRE_THREADS) {
+        const size_t c = (size_t)blockIdx.x * PRE_CELLS + q;
+        if (c >= nn) break;
+        const double w = wsrc[c] - cw;
+        M.W[c] = w;
+        M.canom_mantle[c] = 0.0 + ph.rho_uppermantle * (w - M.w_ref[c]);
+        const double we = M.we[c];
+        const double zb = M.z_bed_ref[c] + w + we;
+        M.z_bed[c] = zb;
+        mw = fmax(mw, fabs(w));
+        mwe = fmax(mwe, fabs(we));
+        mdz = fmax(mdz, fabs(M.dz_ss[c]));
+        if (zb != zb || w != w) nanf = 1.0;
+    }
+    __shared__ double red[4][16];
+    mw = wave_max(mw); mwe = wave_max(mwe); mdz = wave_max(mdz); nanf = wave_max(nanf);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) { red[0][wv] = mw; red[1][wv] = mwe; red[2][wv] = mdz; red[3][wv] = nanf; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 1; q < PRE_THREADS / 64; ++q)
+            for (int r = 0; r < 4; ++r) red[r][0] = fmax(red[r][0], red[r][q]);
+        for (int r = 0; r < 4; ++r) M.partials[(size_t)blockIdx.x * 4 + r] = red[r][0];
+    }
+}
+
+__global__ void __launch_bounds__(64) k_fin_check(const Member* __restrict__ mem, int nblk) {
+    const Member& M = mem[blockIdx.x];
+    Ctl& c = *M.ctl;
+    double v[4] = {0, 0, 0, 0};
+    for (int b = threadIdx.x; b < nblk; b += 64)
+        for (int r = 0; r < 4; ++r) v[r] = fmax(v[r], M.partials[(size_t)b * 4 + r]);
+    for (int r = 0; r < 4; ++r) v[r] = wave_max(v[r]);
+    if (threadIdx.x == 0) {
+        c.maxw = v[0]; c.maxwe = v[1]; c.maxdzss = v[2]; c.nonfinite = v[3] > 0.0;
+        c.w_is_x = 0;
+        c.status = ST_IDLE;
+    }
+}
+
+__global__ void k_begin(const Member* __restrict__ mem, int nmem, double tf, int stepping) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= nmem) return;
+    Ctl& c = *mem[m].ctl;
+    c.tf = tf;
+    c.k_valid = 0;
+    c.status = (stepping && c.t < tf) ? ST_RUN : ST_IDLE;
+}
+
+// barysealevel.f90:195-252 (const / fastiso), one thread
+__global__ void k_bsl_update(BslDev* b, double year_bp) {
+    b->time = year_bp;
+    if (b->method == 0) { b->bsl_now = b->bsl_init; return; }
+    if (b->constant_ocean_surface) { b->A_ocean_now = b->A_ocean_pd; return; }
+    // interp_0d (isos_utils.f90:423-457)
+    const int n = b->nvec;
+    const double x = b->bsl_now;
+    double y;
+    if (x < b->bsl_vec[0]) y = b->A_vec[0];
+    else if (x > b->bsl_vec[n - 1]) y = b->A_vec[n - 1];
+    else {
+        int j;
+        for (j = 0; j < n; ++j) if (b->bsl_vec[j] >= x) break;
+        if (j == 0) y = b->A_vec[0];
+        else if (j == n) y = b->A_vec[n - 1];
+        else {
+            const double al = (x - b->bsl_vec[j - 1]) / (b->bsl_vec[j] - b->bsl_vec[j - 1]);
+            y = b->A_vec[j - 1] + al * (b->A_vec[j] - b->A_vec[j - 1]);
+        }
+    }
+    b->A_ocean_now = y;
+}
+
+// isos_init_state (fastisostasy.f90:628-637): ref%z_ss, now%z_ss, Haf and masks
+// of both states, after z_bed / Hice / dz_ss have been embedded.
+__global__ void __launch_bounds__(PRE_THREADS) k_init_state(const Member* __restrict__ mem) {
+    const Member& M = mem[blockIdx.y];
+    const PhysDev& ph = *M.phys;
+    const Ctl& ctl = *M.ctl;
+    const int N = M.n;
+    const size_t nn = (size_t)N * N;
+    for (int q = threadIdx.x; q < PRE_CELLS; q += PRE_THREADS) {
+        const size_t c = (size_t)blockIdx.x * PRE_CELLS + q;
+        if (c >= nn) break;
+        const double zr = ctl.ref_bsl + 0.0 + M.dz_ss_ref[c];
+        M.z_ss_ref[c] = zr;
+        const double zn = ctl.bsl + M.z_ss[c] + M.dz_ss[c];
+        M.z_ss[c] = zn;
+        const double har = calc_haf(zr, M.z_bed_ref[c], M.Hice_ref[c], ph);
+        M.Haf_ref[c] = har;
+        const double han = calc_haf(zn, M.z_bed[c], M.Hice[c], ph);
+        M.Haf[c] = han;
+        M.mask_ref[c] = calc_mask(M.z_bed_ref[c], zr, M.Hice_ref[c], har, M.mask_ref[c]);
+        M.mask[c] = calc_mask(M.z_bed[c], zn, M.Hice[c], han, M.mask[c]);
+    }
+}
+
+// ------------------------------------------------------------------ launchers
+static int pre_blocks(int n) { return (int)(((size_t)n * n + PRE_CELLS - 1) / PRE_CELLS); }
+
+void launch_pre_haf(const Member* d_mem, int nmem, int n, bool new_H, hipStream_t st) {
+    hipLaunchKernelGGL(k_pre_haf, dim3(pre_blocks(n), nmem), dim3(PRE_THREADS), 0, st, d_mem, (int)new_H);
+}
+void launch_pre_bsl(const Member* d_mem, int nmem, int n, hipStream_t st) {
+    hipLaunchKernelGGL(k_pre_bsl, dim3(1), dim3(64), 0, st, d_mem, nmem, pre_blocks(n));
+}
+void launch_pre_fields(const Member* d_mem, int nmem, int n, int phase, bool diag, bool fp64, hipStream_t st) {
+    if (fp64) hipLaunchKernelGGL(k_pre_fields<double>, dim3(pre_blocks(n), nmem), dim3(PRE_THREADS), 0, st, d_mem, phase, (int)diag);
+    else hipLaunchKernelGGL(k_pre_fields<float>, dim3(pre_blocks(n), nmem), dim3(PRE_THREADS), 0, st, d_mem, phase, (int)diag);
+}
+void launch_begin(const Member* d_mem, int nmem, double tf, int stepping, hipStream_t st) {
+    hipLaunchKernelGGL(k_begin, dim3((nmem + 63) / 64), dim3(64), 0, st, d_mem, nmem, tf, stepping);
+}
+void launch_finalize(const Member* d_mem, int nmem, int n, hipStream_t st) {
+    hipLaunchKernelGGL(k_fin_corner, dim3((nmem + 63) / 64), dim3(64), 0, st, d_mem, nmem);
+    hipLaunchKernelGGL(k_finalize, dim3(pre_blocks(n), nmem), dim3(PRE_THREADS), 0, st, d_mem);
+    hipLaunchKernelGGL(k_fin_check, dim3(nmem), dim3(64), 0, st, d_mem, pre_blocks(n));
+}
+void launch_bsl_update(BslDev* b, double year_bp, hipStream_t st) {
+    hipLaunchKernelGGL(k_bsl_update, dim3(1), dim3(1), 0, st, b, year_bp);
+}
+void launch_init_state_fields(const Member* d_mem, int nmem, int n, hipStream_t st) {
+    hipLaunchKernelGGL(k_init_state, dim3(pre_blocks(n), nmem), dim3(PRE_THREADS), 0, st, d_mem);
+}
+
+}  // namespace fi

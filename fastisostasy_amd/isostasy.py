@@ -1,0 +1,220 @@
+"""Python mirror of the reference's isos_class / bsl_class API.
+
+Names, argument meaning and call order follow the reference's public Fortran
+module (src/fastisostasy.f90:34-60 and src/barysealevel.f90:31-36):
+
+    bsl = bsl_init(method="const", bsl_init=0.0, year_bp=t0)
+    isos = isos_init(params, nx_ice, ny_ice, dx, dy)
+    isos_init_state(isos, z_bed, H_ice, t0, bsl)
+    for t in times:
+        bsl_update(bsl, t)
+        isos_update(isos, H_ice, t, bsl)
+        z_bed = isos.out("z_bed")
+    isos_end(isos)
+
+The compute path is the HIP library (libfastiso.so) through its C ABI; a
+reference `stop` becomes a raised FiError.  Arrays are float64 [ny][nx].
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import FiParams, check, load
+
+MANTLE = {"uniform": 0, "gaussian_plus": 1, "gaussian_minus": 2, "fields": 3}
+LITHOSPHERE = MANTLE
+LAYERING = {"parallel": 0, "equalized": 1}
+LUMPING = {"bottom": 0, "min": 1, "max": 2, "mean": 3, "logmean": 4, "freqdomain": 5}
+VSCALING = {"none": 0, "min": 1, "scale": 2, "max": 3, "stddev": 4}
+DT_METHOD = {"euler": 0, "rk4": 1, "bs32": 2, "tsit54": 3}
+PRECISION = {"fp64": 0, "mixed": 1}
+BSL_METHOD = {"const": 0, "fastiso": 1}
+
+_ENUMS = {"mantle": MANTLE, "lithosphere": LITHOSPHERE, "layering": LAYERING, "lumping": LUMPING,
+          "viscosity_scaling_method": VSCALING, "dt_method": DT_METHOD, "precision": PRECISION}
+
+
+def make_params(**kw):
+    """FiParams from keyword values; enumerated options may be given as the
+    reference's namelist strings.  Lengths are in metres."""
+    p = FiParams()
+    p.precision = 0
+    for k, v in kw.items():
+        if k in _ENUMS and isinstance(v, str):
+            v = _ENUMS[k][v]
+        if k in ("zl", "viscosities"):
+            arr = (ctypes.c_double * _lib.MAXL)()
+            for i, x in enumerate(v):
+                arr[i] = float(x)
+            setattr(p, k, arr)
+        elif isinstance(getattr(p, k), int):
+            setattr(p, k, int(v))
+        else:
+            setattr(p, k, float(v))
+    return p
+
+
+def _dp(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _c(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+
+
+def padded_size(nx_ice, ny_ice, dx, dy, min_pad):
+    n = ctypes.c_int()
+    check(load().fi_padded_size(nx_ice, ny_ice, dx, dy, min_pad, ctypes.byref(n)))
+    return n.value
+
+
+class BslClass:
+    """bsl_class (src/barysealevel.f90:16-29); methods const and fastiso."""
+
+    def __init__(self, method="const", bsl_init=0.0, A_ocean_pd=3.625e14, bsl_vec=None, A_vec=None,
+                 year_bp=0.0):
+        n = 0 if bsl_vec is None else len(bsl_vec)
+        self._bv, self._av = _c(bsl_vec), _c(A_vec)
+        h = ctypes.c_void_p()
+        m = BSL_METHOD[method] if isinstance(method, str) else int(method)
+        check(load().fi_bsl_create(m, bsl_init, A_ocean_pd, n, _dp(self._bv), _dp(self._av), year_bp,
+                                   ctypes.byref(h)))
+        self.h = h
+
+    @property
+    def bsl_now(self):
+        v = ctypes.c_double()
+        check(load().fi_bsl_get(self.h, ctypes.byref(v), None))
+        return v.value
+
+    @property
+    def A_ocean_now(self):
+        v = ctypes.c_double()
+        check(load().fi_bsl_get(self.h, None, ctypes.byref(v)))
+        return v.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().fi_bsl_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+class IsosClass:
+    """isos_class (src/isostasy_defs.f90:202-209) held on the GPU."""
+
+    def __init__(self, params, nx_ice, ny_ice, dx, dy=None, K=None, rho_ice=None, he_field=None,
+                 eta_field=None):
+        dy = dx if dy is None else dy
+        if isinstance(params, dict):
+            params = make_params(**params)
+        self.params = params
+        keep = [_c(K), _c(he_field), _c(eta_field)]
+        rho = None if rho_ice is None else np.array([rho_ice], dtype=np.float64)
+        h = ctypes.c_void_p()
+        check(load().fi_isos_create(ctypes.byref(params), nx_ice, ny_ice, dx, dy, _dp(keep[0]), _dp(rho),
+                                    _dp(keep[1]), _dp(keep[2]), ctypes.byref(h)))
+        self.h = h
+        self.nx_ice, self.ny_ice = nx_ice, ny_ice
+        self.n = int(self.scalar("n"))
+
+    def init_ref(self, z_bed, H_ice, bsl=None, dz_ss=None):
+        b = None if bsl is None else np.array([bsl], dtype=np.float64)
+        check(load().fi_isos_init_ref(self.h, _dp(_c(z_bed)), _dp(_c(H_ice)), _dp(b), _dp(_c(dz_ss))))
+
+    def init_state(self, z_bed, H_ice, time, bsl):
+        check(load().fi_isos_init_state(self.h, _dp(_c(z_bed)), _dp(_c(H_ice)), float(time), bsl.h))
+
+    def update(self, H_ice, time, bsl):
+        check(load().fi_isos_update(self.h, _dp(_c(H_ice)), float(time), bsl.h))
+
+    def out(self, name):
+        """out%<name> on the ice grid (masks as bool)."""
+        if name.startswith("mask"):
+            a = np.empty((self.ny_ice, self.nx_ice), dtype=np.int8)
+            check(load().fi_isos_get_mask(self.h, _lib.FIELD_ID[name],
+                                          a.ctypes.data_as(ctypes.POINTER(ctypes.c_int8))))
+            return a.astype(bool)
+        a = np.empty((self.ny_ice, self.nx_ice))
+        check(load().fi_isos_get(self.h, _lib.FIELD_ID[name], _dp(a)))
+        return a
+
+    def full(self, name):
+        """Padded-grid (N x N) state field."""
+        a = np.empty((self.n, self.n))
+        check(load().fi_isos_get_full(self.h, _lib.FIELD_ID[name], _dp(a)))
+        return a
+
+    def scalar(self, name):
+        v = ctypes.c_double()
+        check(load().fi_isos_get_scalar(self.h, _lib.SCALAR_ID[name], ctypes.byref(v)))
+        return v.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().fi_isos_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+class Batch:
+    """Many domains of one padded size advanced with one launch sequence."""
+
+    def __init__(self, members):
+        self.members = list(members)
+        arr = (ctypes.c_void_p * len(self.members))(*[m.h.value for m in self.members])
+        h = ctypes.c_void_p()
+        check(load().fi_batch_create(arr, len(self.members), ctypes.byref(h)))
+        self.h = h
+
+    def update(self, time, bsls, H_ice=None):
+        n = len(self.members)
+        keep = [None] * n if H_ice is None else [_c(a) for a in H_ice]
+        Harr = (ctypes.POINTER(ctypes.c_double) * n)(*[_dp(a) for a in keep])
+        barr = (ctypes.c_void_p * n)(*[b.h.value for b in bsls])
+        check(load().fi_batch_update(self.h, Harr, float(time), barr))
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().fi_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+# ---- reference-named procedures (src/fastisostasy.f90:40-50, barysealevel.f90:32-33)
+def bsl_init(method="const", bsl_init=0.0, A_ocean_pd=3.625e14, year_bp=0.0, bsl_vec=None, A_vec=None):
+    return BslClass(method, bsl_init, A_ocean_pd, bsl_vec, A_vec, year_bp)
+
+
+def bsl_update(bsl, year_bp):
+    check(load().fi_bsl_update(bsl.h, float(year_bp)))
+
+
+def isos_init(params, nx_ice, ny_ice, dx, dy=None, rho_ice=None, K=None, he_field=None, eta_field=None):
+    return IsosClass(params, nx_ice, ny_ice, dx, dy, K=K, rho_ice=rho_ice, he_field=he_field,
+                     eta_field=eta_field)
+
+
+def isos_init_ref(isos, z_bed, H_ice, bsl=None, dz_ss=None):
+    isos.init_ref(z_bed, H_ice, bsl, dz_ss)
+
+
+def isos_init_state(isos, z_bed, H_ice, time, bsl):
+    isos.init_state(z_bed, H_ice, time, bsl)
+
+
+def isos_update(isos, H_ice, time, bsl):
+    isos.update(H_ice, time, bsl)
+
+
+def isos_end(isos):
+    isos.close()

@@ -1,0 +1,151 @@
+/*
+ * fastiso.h -- C ABI of the MI355X-native LV-ELVA time step.
+ *
+ * Drop-in boundary for the reference's public Fortran API (module
+ * fastisostasy, src/fastisostasy.f90:34-60).  A Fortran ice-sheet model binds
+ * these entry points through iso_c_binding (fastisostasy_amd/fortran/) and keeps
+ * the isos_class / bsl_class init / update call sequence.
+ *
+ * Conventions
+ *  - Every 2-D array is [ny][nx] with x fastest, i.e. exactly the memory order
+ *    of a Fortran (nx, ny) array.  Ice-grid arrays are nx_ice x ny_ice; the
+ *    padded-grid arrays (K, rheology fields, *_full getters) are N x N with
+ *    N from fi_padded_size().
+ *  - Input pointers are borrowed for the duration of the call only.
+ *  - All functions return 0 on success and a negative FI_E* code on failure;
+ *    fi_last_error() returns a thread-local message.  Where the reference
+ *    calls `stop` the library returns an error instead.
+ *  - Handles own their device memory.  Calls on one handle must be
+ *    serialised by the caller; all handles of a process share one HIP stream.
+ */
+#ifndef FASTISO_H
+#define FASTISO_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FI_MAX_LAYERS 8
+
+enum fi_status {
+    FI_OK = 0,
+    FI_E_ARG = -1,        /* invalid argument / configuration */
+    FI_E_UNSUPPORTED = -2,/* option or size not supported by this build */
+    FI_E_HIP = -3,        /* HIP runtime error */
+    FI_E_UNSTABLE = -4,   /* large_check threshold exceeded (fastisostasy.f90:874-899) */
+    FI_E_NAN = -5,        /* NaN in z_bed (fastisostasy.f90:864-869) */
+    FI_E_STATE = -6       /* call sequence violated (e.g. update before init_state) */
+};
+
+/* Precision of the RHS (stencil output, Hartley passes, spectral divide,
+ * convolutions).  The integrator state, error norm, sea level, masks and the
+ * barystatic reduction are FP64 in both modes. */
+enum fi_precision { FI_FP64 = 0, FI_MIXED = 1 };
+
+/* isos_param_class (src/isostasy_defs.f90:34-100), after the km -> m
+ * conversions of isos_par_load (src/fastisostasy.f90:971-979).  Strings of
+ * the namelist are enumerations here. */
+typedef struct {
+    int heterogeneous_ssh, interactive_sealevel, include_elastic;
+    int correct_compression, correct_distortion;
+    int method;                   /* 3 = LV-ELVA (0 = no displacement) */
+    double dt_diagnostics;        /* [yr] */
+    double min_pad;               /* [m]  namelist "pad" */
+    int mantle;                   /* 0 uniform, 1 gaussian_plus, 2 gaussian_minus, 3 fields */
+    int lithosphere;              /* 0 uniform, 1 gaussian_plus, 2 gaussian_minus, 3 fields */
+    int layering;                 /* 0 parallel, 1 equalized */
+    int lumping;                  /* 0 bottom, 1 min, 2 max, 3 mean, 4 logmean, 5 freqdomain */
+    int viscosity_scaling_method; /* 0 as-is, 1 min, 2 scale, 3 max, 4 stddev (no-op) */
+    double viscosity_scaling;
+    int nl;
+    double zl[FI_MAX_LAYERS];     /* [m] */
+    double viscosities[FI_MAX_LAYERS]; /* [Pa s] */
+    double dl;                    /* [m] */
+    double eta_ref;               /* [Pa s] ref_viscosity */
+    double rho_water, rho_ice, rho_seawater, rho_uppermantle, rho_litho;
+    double E;                     /* [GPa] */
+    double nu, g, r_earth, m_earth; /* r_earth in m */
+    int dt_method;                /* 0 euler, 1 rk4, 2 bs32, 3 tsit54 */
+    double atol, rtol, dt_min, dt_max, dt_init, q;
+    int precision;                /* enum fi_precision */
+} fi_params;
+
+/* ice-grid output components of isos_out_class (isostasy_defs.f90:185-200) */
+enum fi_field {
+    FI_Z_BED = 0, FI_Z_SS, FI_DZ_SS, FI_W, FI_WE, FI_DWDT, FI_CANOM_FULL, FI_HICE,
+    FI_MASKOCEAN, FI_MASKGROUNDED, FI_MASKCONTINENT,
+    /* padded-grid extras (fi_isos_get_full) */
+    FI_CANOM_MANTLE, FI_HAF, FI_ETA_EFF, FI_D_LITH, FI_KAPPA, FI_R, FI_MASKACTIVE, FI_HE_LITH,
+    FI_ODE_X, FI_Z_SS_REF, FI_Z_BED_REF, FI_NFIELDS
+};
+
+enum fi_scalar {
+    FI_S_N = 0, FI_S_T, FI_S_DT, FI_S_BSL, FI_S_DELTAV, FI_S_V_AF, FI_S_V_DEN,
+    FI_S_ATTEMPTS, FI_S_ACCEPTS, FI_S_D_LITH11, FI_S_ERR, FI_S_TOL
+};
+
+typedef struct fi_isos fi_isos;
+typedef struct fi_bsl fi_bsl;
+typedef struct fi_batch fi_batch;
+
+const char* fi_last_error(void);
+int fi_version(void);
+
+/* Padded FFT edge N of init_domain_size (src/isos_utils.f90:285-354). */
+int fi_padded_size(int nx_ice, int ny_ice, double dx, double dy, double min_pad, int* n_out);
+
+/* isos_init (src/fastisostasy.f90:64-473).  K: distortion on the padded grid
+ * (N x N) or NULL; rho_ice: optional override; he_field (N x N, m) and
+ * eta_field (nl x N x N, Pa s) replace the rheology_file NetCDF path when
+ * lithosphere / mantle == 3. */
+int fi_isos_create(const fi_params* p, int nx_ice, int ny_ice, double dx, double dy,
+                   const double* K, const double* rho_ice, const double* he_field,
+                   const double* eta_field, fi_isos** out);
+/* isos_init_ref (src/fastisostasy.f90:550-575); bsl and dz_ss (N x N) may be NULL */
+int fi_isos_init_ref(fi_isos* h, const double* z_bed, const double* H_ice,
+                     const double* bsl, const double* dz_ss);
+/* isos_init_state (src/fastisostasy.f90:596-660) */
+int fi_isos_init_state(fi_isos* h, const double* z_bed, const double* H_ice, double time, fi_bsl* b);
+/* isos_update (src/fastisostasy.f90:731-872); dwdt_corr is accepted but, as in
+ * the reference (:755-759), unused */
+int fi_isos_update(fi_isos* h, const double* H_ice, double time, fi_bsl* b);
+/* out%<field> after the last update (cropstate2out, src/isos_utils.f90:386-405):
+ * double values, masks as 0.0 / 1.0 */
+int fi_isos_get(fi_isos* h, int field, double* dst);
+/* out%mask* as int8 0/1 */
+int fi_isos_get_mask(fi_isos* h, int field, int8_t* dst);
+/* padded-grid state (N x N) */
+int fi_isos_get_full(fi_isos* h, int field, double* dst);
+int fi_isos_get_scalar(fi_isos* h, int which, double* out);
+/* isos_end (src/fastisostasy.f90:911-923) */
+int fi_isos_destroy(fi_isos* h);
+
+/* bsl_init / bsl_update (src/barysealevel.f90:40-106, 195-252).
+ * method 0 = const, 1 = fastiso; nvec > 0 gives the A_ocean(bsl) table of
+ * A_ocean_path (otherwise the ocean surface is constant). */
+int fi_bsl_create(int method, double bsl_init, double A_ocean_pd, int nvec, const double* bsl_vec,
+                  const double* A_vec, double year_bp, fi_bsl** out);
+int fi_bsl_update(fi_bsl* b, double year_bp);
+int fi_bsl_get(fi_bsl* b, double* bsl_now, double* A_ocean_now);
+int fi_bsl_destroy(fi_bsl* b);
+
+/* Batched extension (no reference counterpart): advance many domains of the
+ * same padded size and precision with one launch sequence per attempted
+ * step.  Members keep their own handles (init_state, get, destroy); the
+ * batch only groups them.  Domains sharing one fi_bsl are updated in member
+ * order, as consecutive isos_update calls would be. */
+int fi_batch_create(fi_isos* const* members, int nmem, fi_batch** out);
+/* H_ice[m] may be NULL to keep member m's current ice thickness. */
+int fi_batch_update(fi_batch* b, const double* const* H_ice, double time, fi_bsl* const* bsl);
+int fi_batch_destroy(fi_batch* b);
+
+/* Synchronise the library stream and report the time of the last update
+ * spent on the device (ms); used by benchmarks. */
+int fi_sync(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

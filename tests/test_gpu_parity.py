@@ -1,0 +1,158 @@
+"""GPU parity: the HIP library (through its C ABI) against the FP64 oracle.
+
+Tolerances: FP64 path -- displacement agrees to 1e-6 m over the short runs
+here (only FFT rounding differs); mixed path -- max |dw| <= 1 mm.  Masks
+must be bit-exact in both modes.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from fastisostasy_amd import isostasy as G
+from fastisostasy_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+OUT = ["z_bed", "z_ss", "dz_ss", "w", "we", "dwdt", "canom_full", "Hice"]
+MASKS = ["maskocean", "maskgrounded", "maskcontinent"]
+
+
+def run_pair(wl, times, precision="fp64", bsl_method="const", H_of_t=None):
+    """Run the workload through oracle and GPU on the same inputs; yield after each update."""
+    p = dict(wl["params"])
+    op = O.make_params(**p)
+    gp = G.make_params(precision=precision, **p)
+    kw = dict(he_field=wl.get("he_field"), eta_field=wl.get("eta_field"))
+    o = O.OracleIsos(op, wl["nx"], wl["nx"], wl["dx"], **kw)
+    g = G.IsosClass(gp, wl["nx"], wl["nx"], wl["dx"], **kw)
+    ob = O.OracleBSL(bsl_method, 0.0, year_bp=times[0])
+    gb = G.bsl_init(bsl_method, 0.0, year_bp=times[0])
+    H = (lambda t: W.step_on_load(wl["H_ice"], t)) if H_of_t is None else H_of_t
+    o.init_state(wl["z_bed"], H(times[0]), times[0], ob)
+    g.init_state(wl["z_bed"], H(times[0]), times[0], gb)
+    yield o, g
+    for t in times[1:]:
+        ob.update(t)
+        G.bsl_update(gb, t)
+        o.update(H(t), t, ob)
+        g.update(H(t), t, gb)
+        yield o, g
+
+
+def compare(o, g, wtol, rel=None):
+    for f in OUT:
+        a, b = o.get(f), g.out(f)
+        scale = max(1.0, np.abs(a).max())
+        tol = wtol if f in ("w", "z_bed") else (rel or 1e-9) * scale
+        assert np.abs(a - b).max() <= tol, (f, np.abs(a - b).max(), tol)
+    for f in MASKS:
+        assert np.array_equal(o.get(f).astype(bool), g.out(f)), f
+
+
+def small_disc(n_ice=48, dx=50e3, pad=400e3):
+    wl = W.plain_disc()
+    wl["nx"], wl["dx"] = n_ice, dx
+    wl["params"] = dict(wl["params"], min_pad=pad)
+    wl["H_ice"] = W.disc_load(n_ice, dx)
+    wl["z_bed"] = np.full((n_ice, n_ice), 1e6)
+    return wl
+
+
+def test_init_fields_match_oracle():
+    wl = W.lv_rheology()
+    wl["params"] = dict(wl["params"], include_elastic=0)
+    p = wl["params"]
+    o = O.OracleIsos(O.make_params(**p), wl["nx"], wl["nx"], wl["dx"], he_field=wl["he_field"],
+                     eta_field=wl["eta_field"])
+    g = G.IsosClass(G.make_params(**p), wl["nx"], wl["nx"], wl["dx"], he_field=wl["he_field"],
+                    eta_field=wl["eta_field"])
+    assert g.n == o.n == 512
+    for f in ("kappa", "R", "eta_eff", "D_lith", "He_lith"):
+        a, b = o.get_full(f), g.full(f)
+        assert np.allclose(a, b, rtol=1e-13, atol=0), (f, np.abs(a - b).max())
+
+
+def test_plain_disc_fp64_short():
+    wl = small_disc()
+    times = [0.0] + [10.0 * k for k in range(1, 31)]
+    for o, g in run_pair(wl, times):
+        pass
+    compare(o, g, wtol=1e-6)
+    assert o.scalar("attempts") == g.scalar("attempts")
+    assert abs(o.scalar("dt") - g.scalar("dt")) < 1e-9
+
+
+def test_plain_disc_256_fp64():
+    wl = W.plain_disc()
+    times = [0.0] + [10.0 * k for k in range(1, 21)]
+    for o, g in run_pair(wl, times):
+        pass
+    compare(o, g, wtol=1e-6)
+
+
+def test_variable_rheology_fp64():
+    wl = W.lv_rheology()
+    wl["params"] = dict(wl["params"], include_elastic=0)
+    times = [0.0, 0.5, 1.0]
+    for o, g in run_pair(wl, times):
+        pass
+    compare(o, g, wtol=1e-6)
+
+
+def test_elastic_convolution_fp64():
+    wl = W.lv_rheology()
+    times = [0.0, 0.5, 1.0]
+    for o, g in run_pair(wl, times):
+        pass
+    assert np.abs(o.get("we")).max() > 0.1
+    compare(o, g, wtol=1e-6)
+
+
+def test_plain_disc_mixed_1mm():
+    wl = small_disc()
+    times = [0.0] + [10.0 * k for k in range(1, 31)]
+    for o, g in run_pair(wl, times, precision="mixed"):
+        pass
+    compare(o, g, wtol=1e-3, rel=1e-4)
+
+
+def test_batch_matches_single():
+    members = [W.ensemble_member(m, nx=64, dx=50e3, pad=0.0) for m in range(3)]
+    singles, gs, bsls = [], [], []
+    for wl in members:
+        p = G.make_params(**wl["params"])
+        a = G.IsosClass(p, wl["nx"], wl["nx"], wl["dx"], he_field=wl["he_field"], eta_field=wl["eta_field"])
+        b = G.IsosClass(p, wl["nx"], wl["nx"], wl["dx"], he_field=wl["he_field"], eta_field=wl["eta_field"])
+        ba, bb = G.bsl_init(), G.bsl_init()
+        a.init_state(wl["z_bed"], np.zeros_like(wl["H_ice"]), 0.0, ba)
+        b.init_state(wl["z_bed"], np.zeros_like(wl["H_ice"]), 0.0, bb)
+        singles.append((a, ba))
+        gs.append(b)
+        bsls.append(bb)
+    batch = G.Batch(gs)
+    for k in range(1, 11):
+        t = 10.0 * k
+        for (a, ba), wl in zip(singles, members):
+            a.update(wl["H_ice"], t, ba)
+        batch.update(t, bsls, H_ice=[wl["H_ice"] for wl in members])
+    for (a, _), b in zip(singles, gs):
+        for f in ("w", "z_bed", "dwdt"):
+            assert np.array_equal(a.out(f), b.out(f)), f
+
+
+def small_sealevel():
+    wl = W.sealevel_cycle(nx=48, dx=50e3, pad=400e3)
+    return wl
+
+
+def test_sealevel_masks_bsl_fp64():
+    wl = small_sealevel()
+    Hmax = wl["H_max"]
+    times = [0.0] + [10.0 * k for k in range(1, 21)]
+    H_of_t = lambda t: Hmax * min(1.0, t / 150.0)
+    for o, g in run_pair(wl, times, bsl_method="fastiso", H_of_t=H_of_t):
+        pass
+    assert o.get("maskocean").any() and o.get("maskcontinent").any() and o.get("maskgrounded").any()
+    assert np.abs(o.get("dz_ss")).max() > 1e-3
+    compare(o, g, wtol=1e-6)
+    assert abs(o.scalar("bsl") - g.scalar("bsl")) < 1e-9
