@@ -1,0 +1,243 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X LV-ELVA time step.
+
+Workload: the 64-member ensemble (256 x 256 ice grid at 16 km, N = 256, random
+log10-viscosity field, lithosphere thickness and dome per member, seeds
+1000 + m).  One bench step = one coupling step (isos_update to t + 10 yr) of the
+whole batch with the batched mixed-precision path (FP32 Hartley passes, FP64
+state, error norm and sea level); inputs are resident in HBM.
+
+Reported beside it:
+  plain_fp64    the single-domain FP64 drop-in path on a subset of members
+  max_err_m     max |w_mixed - w_fp64| over all members after the timed steps
+  roofline      the dominant kernel, timed with HIP events on the library
+                stream over the timed region
+  cpu_baseline  the FP64 oracle (a plain-C restatement of the reference path),
+                single thread, on one member for a bounded sample
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+KERNEL_NAMES = {0: "k_stencil", 1: "k_row_fwd", 2: "k_col", 3: "k_row_inv"}
+
+
+def kernel_bytes_per_cell(kernel, s):
+    """Algorithmic HBM bytes per cell per launch (s = bytes per RHS real)."""
+    return {0: 8 + 4 * s,          # read x (f64), D, 1/(2 eta), canom_full; write f
+            1: 2 * s,              # read f, write row-transformed
+            2: 3 * s,              # read, read kappa*R, write
+            3: s + 8 + 8 + 8}[kernel]  # read, read x, write x2, write dwdt (f64)
+
+
+def build_members(nmem, precision):
+    from fastisostasy_amd import isostasy as G
+    from fastisostasy_amd import workloads as W
+    hs, bsls, wls = [], [], []
+    for m in range(nmem):
+        wl = W.ensemble_member(m)
+        p = G.make_params(precision=precision, **wl["params"])
+        h = G.IsosClass(p, wl["nx"], wl["nx"], wl["dx"], he_field=wl["he_field"], eta_field=wl["eta_field"])
+        b = G.bsl_init("const", 0.0)
+        h.init_state(wl["z_bed"], np.zeros_like(wl["H_ice"]), 0.0, b)
+        hs.append(h)
+        bsls.append(b)
+        wls.append(wl)
+    return hs, bsls, wls
+
+
+def percentile(xs, q):
+    return float(np.percentile(np.asarray(xs), q)) if xs else None
+
+
+def run_batched(nmem, steps, warmup, precision, prof_kernel=None):
+    from fastisostasy_amd import isostasy as G
+    from fastisostasy_amd._lib import check, load
+    import ctypes
+    hs, bsls, wls = build_members(nmem, precision)
+    batch = G.Batch(hs)
+    t = 0.0
+    # first step uploads the loads (step-on at t = 1e-9 as in the reference driver)
+    t += 10.0
+    batch.update(t, bsls, H_ice=[wl["H_ice"] for wl in wls])
+    for _ in range(max(0, warmup - 1)):
+        t += 10.0
+        batch.update(t, bsls)
+    check(load().fi_sync())
+    if prof_kernel is not None:
+        check(load().fi_prof_select(prof_kernel))
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        t += 10.0
+        s0 = time.perf_counter()
+        batch.update(t, bsls)
+        lat.append(time.perf_counter() - s0)
+    check(load().fi_sync())
+    wall = time.perf_counter() - t0
+    prof = None
+    if prof_kernel is not None:
+        ms, cnt = ctypes.c_double(), ctypes.c_longlong()
+        check(load().fi_prof_collect(ctypes.byref(ms), ctypes.byref(cnt)))
+        check(load().fi_prof_select(-1))
+        prof = (ms.value, cnt.value)
+    attempts = sum(h.scalar("attempts") for h in hs)
+    w = [h.full("w") for h in hs]
+    n = hs[0].n
+    batch.close()
+    for h in hs:
+        h.close()
+    return dict(wall=wall, lat=lat, prof=prof, attempts=attempts, w=w, n=n, t_end=t)
+
+
+def run_plain(members, steps, warmup):
+    """Single-domain FP64 drop-in path, one member after the other."""
+    from fastisostasy_amd import isostasy as G
+    from fastisostasy_amd import workloads as W
+    total = 0.0
+    n = None
+    for m in range(members):
+        wl = W.ensemble_member(m)
+        p = G.make_params(precision="fp64", **wl["params"])
+        h = G.IsosClass(p, wl["nx"], wl["nx"], wl["dx"], he_field=wl["he_field"], eta_field=wl["eta_field"])
+        b = G.bsl_init("const", 0.0)
+        h.init_state(wl["z_bed"], np.zeros_like(wl["H_ice"]), 0.0, b)
+        t = 0.0
+        for _ in range(warmup):
+            t += 10.0
+            G.bsl_update(b, t)
+            h.update(wl["H_ice"], t, b)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            t += 10.0
+            G.bsl_update(b, t)
+            h.update(wl["H_ice"], t, b)
+        total += time.perf_counter() - t0
+        n = h.n
+        h.close()
+    return dict(wall=total, n=n)
+
+
+def run_cpu_baseline(seconds):
+    """FP64 oracle, one thread, one ensemble member, as many coupling steps as fit."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from fastisostasy_amd import workloads as W
+    wl = W.ensemble_member(0)
+    o = O.OracleIsos(O.make_params(**wl["params"]), wl["nx"], wl["nx"], wl["dx"], he_field=wl["he_field"],
+                     eta_field=wl["eta_field"])
+    b = O.OracleBSL("const", 0.0)
+    o.init_state(wl["z_bed"], np.zeros_like(wl["H_ice"]), 0.0, b)
+    t = 0.0
+    t0 = time.perf_counter()
+    steps = 0
+    while time.perf_counter() - t0 < seconds:
+        t += 10.0
+        b.update(t)
+        o.update(wl["H_ice"], t, b)
+        steps += 1
+    wall = time.perf_counter() - t0
+    n = o.n
+    return dict(value=n * n * 10.0 * steps / wall, steps=steps, wall=wall, n=n)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--members", type=int, default=64)
+    ap.add_argument("--plain-members", type=int, default=2)
+    ap.add_argument("--plain-steps", type=int, default=30)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--roofline-kernel", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-check", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); the ensemble "
+              "would shard members across GPUs without communication, which is not built here.",
+              file=sys.stderr)
+        sys.exit(2)
+    from fastisostasy_amd import _lib
+    _lib.load()
+
+    res = run_batched(args.members, args.steps, args.warmup, "mixed", prof_kernel=args.roofline_kernel)
+    n = res["n"]
+    cells = args.members * n * n
+    years = 10.0 * args.steps
+    value = cells * years / res["wall"]
+    print(f"# batched mixed: {args.members} x {n}^2, {args.steps} steps in {res['wall']:.3f} s, "
+          f"{res['attempts']:.0f} attempts total", file=sys.stderr, flush=True)
+
+    max_err = None
+    if not args.no_check:
+        ref = run_batched(args.members, args.steps, args.warmup, "fp64")
+        max_err = max(float(np.abs(a - b).max()) for a, b in zip(res["w"], ref["w"]))
+        print(f"# fp64 batch for the error check: {ref['wall']:.3f} s, max |dw| = {max_err:.3e} m",
+              file=sys.stderr, flush=True)
+
+    plain = run_plain(args.plain_members, args.plain_steps, 2)
+    plain_value = args.plain_members * plain["n"] ** 2 * 10.0 * args.plain_steps / plain["wall"]
+    print(f"# plain fp64: {args.plain_members} members x {args.plain_steps} steps in {plain['wall']:.3f} s",
+          file=sys.stderr, flush=True)
+
+    roof = None
+    if res["prof"] and res["prof"][1] > 0:
+        ms, cnt = res["prof"]
+        avg_s = ms / cnt * 1e-3
+        bytes_per_launch = kernel_bytes_per_cell(args.roofline_kernel, 4) * cells
+        achieved = bytes_per_launch / avg_s / 1e9
+        roof = {"kernel": KERNEL_NAMES[args.roofline_kernel], "bound": "hbm", "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": None, "avg_launch_us": round(avg_s * 1e6, 2), "launches": cnt,
+                "bytes_per_launch": bytes_per_launch}
+
+    cpu = None
+    if not args.no_cpu:
+        c = run_cpu_baseline(args.cpu_seconds)
+        cpu = {"value": round(c["value"], 1), "unit": "grid-cell-years/s", "cores": 1, "kind": "port",
+               "sample": f"oracle (plain-C FP64 restatement), 1 ensemble member N={c['n']}, "
+                         f"{c['steps']} coupling steps of 10 yr in {c['wall']:.1f} s"}
+
+    out = {
+        "metric": "grid-cell-years/s batched mixed vs plain FP64 at <=1 mm err; p99 step latency",
+        "value": round(value, 1),
+        "unit": "grid-cell-years/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * res["wall"] / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic (seeded ensemble members 1000+m)",
+        "config": {"workload": f"ensemble {args.members} x 256^2 ice grid, dx 16 km, no pad (N={n}), "
+                               "coupling step 10 yr, dt_max 0.2 yr",
+                   "members": args.members, "n": n, "precision": "mixed"},
+        "plain_fp64": {"value": round(plain_value, 1), "members": args.plain_members,
+                       "steps": args.plain_steps},
+        "speedup_vs_plain_fp64": round(value / plain_value, 2),
+        "max_err_vs_fp64_m": max_err,
+        "step_latency_ms": {"p50": round(1e3 * percentile(res["lat"], 50), 4),
+                            "p99": round(1e3 * percentile(res["lat"], 99), 4)},
+        "attempts_per_member_step": round(res["attempts"] / args.members / (args.steps + args.warmup), 2),
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -26,7 +26,8 @@ SCALAR_ID = {n: i for i, n in enumerate(SCALARS)}
 EXPORTS = ["fi_last_error", "fi_version", "fi_padded_size", "fi_isos_create", "fi_isos_init_ref",
            "fi_isos_init_state", "fi_isos_update", "fi_isos_get", "fi_isos_get_mask", "fi_isos_get_full",
            "fi_isos_get_scalar", "fi_isos_destroy", "fi_bsl_create", "fi_bsl_update", "fi_bsl_get",
-           "fi_bsl_destroy", "fi_batch_create", "fi_batch_update", "fi_batch_destroy", "fi_sync"]
+           "fi_bsl_destroy", "fi_batch_create", "fi_batch_update", "fi_batch_destroy", "fi_sync",
+           "fi_prof_select", "fi_prof_collect"]
 
 
 class FiParams(ctypes.Structure):
@@ -85,6 +86,8 @@ def load():
     L.fi_batch_create.argtypes = [ctypes.POINTER(vp), i, ctypes.POINTER(vp)]
     L.fi_batch_update.argtypes = [vp, ctypes.POINTER(dp), d, ctypes.POINTER(vp)]
     L.fi_batch_destroy.argtypes = [vp]
+    L.fi_prof_select.argtypes = [i]
+    L.fi_prof_collect.argtypes = [dp, ctypes.POINTER(ctypes.c_longlong)]
     for name in EXPORTS:
         if name not in ("fi_last_error", "fi_version"):
             getattr(L, name).restype = i

@@ -162,6 +162,13 @@ size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 }  // namespace
 
+// Summary buffers of a launch group (device + pinned host).
+struct SumBuf {
+    CtlSummary* d = nullptr;
+    CtlSummary* h = nullptr;
+    int n = 0;
+};
+
 // ------------------------------------------------------------------ handles
 struct fi_bsl {
     BslDev* d = nullptr;
@@ -184,6 +191,7 @@ struct fi_isos {
     bool has_conv = false;
     int part_blocks = 0;
     fi_bsl* bsl_bound = nullptr;
+    SumBuf sb;
 };
 
 struct fi_batch {
@@ -191,6 +199,7 @@ struct fi_batch {
     Member* d_mem = nullptr;
     std::vector<Member> hm;
     RhsShape shape{};
+    SumBuf sb;
 };
 
 static const ConvShape& conv_shape_of(const fi_isos* h) { return h->cshape; }
@@ -240,8 +249,30 @@ int write_ctl(const Member& m, const Ctl& c) {
     return 0;
 }
 
+int sum_alloc(SumBuf& b, int n) {
+    HIPCHK(hipMalloc(&b.d, sizeof(CtlSummary) * n));
+    HIPCHK(hipHostMalloc(&b.h, sizeof(CtlSummary) * n, hipHostMallocDefault));
+    b.n = n;
+    return 0;
+}
+void sum_free(SumBuf& b) {
+    if (b.d) (void)hipFree(b.d);
+    if (b.h) (void)hipHostFree(b.h);
+    b.d = nullptr;
+    b.h = nullptr;
+}
+
+int gather(Member* d_mem, SumBuf& sb) {
+    launch_gather(d_mem, sb.n, sb.d, g_stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(sb.h, sb.d, sizeof(CtlSummary) * sb.n, hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipStreamSynchronize(g_stream));
+    return 0;
+}
+
 // One isos_update for a group of members that share N, precision and call time.
-int run_update(std::vector<fi_isos*>& hs, Member* d_mem, const RhsShape& shape, double time, bool stepping) {
+int run_update(std::vector<fi_isos*>& hs, Member* d_mem, const RhsShape& shape, SumBuf& sb, double time,
+               bool stepping) {
     const int nmem = (int)hs.size();
     const fi_isos& h0 = *hs[0];
     const int n = h0.n;
@@ -267,28 +298,37 @@ int run_update(std::vector<fi_isos*>& hs, Member* d_mem, const RhsShape& shape, 
     launch_begin(d_mem, nmem, time, stepping ? 1 : 0, g_stream);
     HIPCHK(hipGetLastError());
     if (stepping) {
-        // attempts until every member reached `time`; members that are done
-        // skip the launches on the device
-        std::vector<Ctl> ctl(nmem);
-        for (int it = 0;; ++it) {
-            launch_attempt(d_mem, nmem, shape, g_stream);
+        // Attempts until every member reached `time`.  Members that are done
+        // skip the launches on the device, so the host may launch ahead: after
+        // each readback it launches as many attempts as the slowest member
+        // still needs at its current dt (at least one).
+        int ahead = 1;
+        long long total = 0;
+        for (;;) {
+            for (int a = 0; a < ahead; ++a) launch_attempt(d_mem, nmem, shape, g_stream);
             HIPCHK(hipGetLastError());
+            total += ahead;
+            int rc = gather(d_mem, sb);
+            if (rc) return rc;
+            double need = 0.0;
             bool running = false;
             for (int m = 0; m < nmem; ++m) {
-                int rc = read_ctl(hs[m]->hm, ctl[m]);
-                if (rc) return rc;
-                running |= (ctl[m].status == ST_RUN);
+                const CtlSummary& c = sb.h[m];
+                if (c.status != ST_RUN) continue;
+                running = true;
+                need = std::max(need, std::ceil((c.tf - c.t) / std::max(c.dt, 1e-300)));
             }
             if (!running) break;
-            if (it > 100000000) return fail(FI_E_STATE, "integrator did not reach the target time");
+            ahead = (int)std::min(256.0, std::max(1.0, need));
+            if (total > 100000000LL) return fail(FI_E_STATE, "integrator did not reach the target time");
         }
     }
     launch_finalize(d_mem, nmem, n, g_stream);
     HIPCHK(hipGetLastError());
+    int rcg = gather(d_mem, sb);
+    if (rcg) return rcg;
     for (int m = 0; m < nmem; ++m) {
-        Ctl c;
-        int rc = read_ctl(hs[m]->hm, c);
-        if (rc) return rc;
+        const CtlSummary& c = sb.h[m];
         hs[m]->time_prog = time;
         // check_isos_instabilities + NaN check (fastisostasy.f90:860-869)
         if (c.maxw > 1e5) return fail(FI_E_UNSTABLE, "large_check:: Error: Large values detected in array w");
@@ -461,6 +501,8 @@ int fi_isos_create(const fi_params* p, int nx_ice, int ny_ice, double dx, double
     size_t octl = take(sizeof(Ctl)), ophys = take(sizeof(PhysDev)), omem = take(sizeof(Member));
     if (hipMalloc(&h->arena, off) != hipSuccess) { delete h; return fail(FI_E_HIP, "hipMalloc arena (%zu bytes)", off); }
     HIPCHK(hipMemsetAsync(h->arena, 0, off, g_stream));
+    rc = sum_alloc(h->sb, 1);
+    if (rc) return rc;
     char* a = h->arena;
     Member& M = h->hm;
     M.n = n; M.icrop1 = h->icrop1; M.jcrop1 = h->jcrop1; M.nx_ice = nx_ice; M.ny_ice = ny_ice;
@@ -648,7 +690,7 @@ int fi_isos_init_state(fi_isos* h, const double* z_bed, const double* H_ice, dou
     h->time_diag = time;
     std::vector<fi_isos*> hs{h};
     // the update inside isos_init_state runs before the ODE state exists: no step (:644)
-    rc = run_update(hs, h->d_mem, h->shape, time, false);
+    rc = run_update(hs, h->d_mem, h->shape, h->sb, time, false);
     if (rc) return rc;
     // ODE initial state (:654-656): x = now%w, kept in X[0]
     rc = read_ctl(M, c);
@@ -672,7 +714,7 @@ int fi_isos_update(fi_isos* h, const double* H_ice, double time, fi_bsl* b) {
     if (rc) return rc;
     HIPCHK(hipMemcpyAsync(h->hm.H_in, H_ice, sizeof(double) * h->nx_ice * h->ny_ice, hipMemcpyHostToDevice, g_stream));
     std::vector<fi_isos*> hs{h};
-    return run_update(hs, h->d_mem, h->shape, time, true);
+    return run_update(hs, h->d_mem, h->shape, h->sb, time, true);
 }
 
 static const double* field_ptr(fi_isos* h, int f, bool& is_mask, int& bit) {
@@ -786,6 +828,7 @@ int fi_isos_destroy(fi_isos* h) {
     if (!h) return 0;
     (void)hipStreamSynchronize(g_stream);
     (void)hipFree(h->arena);
+    sum_free(h->sb);
     delete h;
     return 0;
 }
@@ -807,6 +850,8 @@ int fi_batch_create(fi_isos* const* members, int nmem, fi_batch** out) {
     int rc = make_shape(members[0]->n, nmem, members[0]->fp64, members[0]->uniform, b->shape);
     if (rc) { delete b; return rc; }
     if (hipMalloc(&b->d_mem, sizeof(Member) * nmem) != hipSuccess) { delete b; return fail(FI_E_HIP, "hipMalloc"); }
+    rc = sum_alloc(b->sb, nmem);
+    if (rc) return rc;
     HIPCHK(hipMemcpy(b->d_mem, b->hm.data(), sizeof(Member) * nmem, hipMemcpyHostToDevice));
     *out = b;
     return 0;
@@ -826,13 +871,28 @@ int fi_batch_update(fi_batch* b, const double* const* H_ice, double time, fi_bsl
                                   hipMemcpyHostToDevice, g_stream));
     }
     if (dirty) HIPCHK(hipMemcpy(b->d_mem, b->hm.data(), sizeof(Member) * nmem, hipMemcpyHostToDevice));
-    return run_update(b->mem, b->d_mem, b->shape, time, true);
+    return run_update(b->mem, b->d_mem, b->shape, b->sb, time, true);
+}
+
+int fi_prof_select(int kernel) {
+    int rc = ensure_device();
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(g_stream));
+    prof_select(kernel);
+    return 0;
+}
+
+int fi_prof_collect(double* total_ms, long long* launches) {
+    if (!total_ms || !launches) return fail(FI_E_ARG, "NULL argument");
+    if (prof_collect(total_ms, launches)) return fail(FI_E_HIP, "event timing failed");
+    return 0;
 }
 
 int fi_batch_destroy(fi_batch* b) {
     if (!b) return 0;
     (void)hipStreamSynchronize(g_stream);
     (void)hipFree(b->d_mem);
+    sum_free(b->sb);
     delete b;
     return 0;
 }

@@ -30,6 +30,9 @@ struct ConvShape {
 };
 
 void launch_attempt(const Member* d_mem, int nmem, const RhsShape& s, hipStream_t st);
+// event timing of one attempt kernel: 0 stencil, 1 row DHT, 2 column DHT+divide, 3 inverse row DHT+update
+void prof_select(int kernel);
+int prof_collect(double* total_ms, long long* launches);
 
 // bookkeeping of isos_update (sealevel.f90, fastisostasy.f90:750-833)
 void launch_pre_haf(const Member* d_mem, int nmem, int n, bool new_H, hipStream_t st);
@@ -40,6 +43,7 @@ void launch_pre_bsl(const Member* d_mem, int nmem, int n, hipStream_t st);
 void launch_pre_fields(const Member* d_mem, int nmem, int n, int phase, bool diag, bool fp64, hipStream_t st);
 void launch_begin(const Member* d_mem, int nmem, double tf, int stepping, hipStream_t st);
 void launch_finalize(const Member* d_mem, int nmem, int n, hipStream_t st);
+void launch_gather(const Member* d_mem, int nmem, CtlSummary* d_out, hipStream_t st);
 void launch_bsl_update(BslDev* b, double year_bp, hipStream_t st);
 void launch_set_ref_zss(const Member* d_mem, int nmem, int n, hipStream_t st);
 void launch_init_state_fields(const Member* d_mem, int nmem, int n, hipStream_t st);

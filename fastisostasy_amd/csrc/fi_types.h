@@ -38,6 +38,12 @@ struct Ctl {
 
 enum { ST_IDLE = 0, ST_RUN = 1 };
 
+// Host-visible summary of one domain, gathered after a group of launches.
+struct CtlSummary {
+    double t, dt, tf, maxw, maxwe, maxdzss;
+    int status, nonfinite;
+};
+
 // Barystatic sea level, resident in device memory (bsl_class,
 // barysealevel.f90:16-29; methods const and fastiso).
 struct BslDev {

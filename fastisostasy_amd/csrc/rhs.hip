@@ -18,6 +18,8 @@
 // Every kernel reads the domain's Ctl and returns at once if the domain is not
 // stepping, so the same launch sequence serves a batch whose members need
 // different numbers of attempts.
+#include <vector>
+
 #include "fi_device.hpp"
 #include "fi_kernels.h"
 
@@ -400,17 +402,67 @@ __global__ void __launch_bounds__(64) k_decide(const Member* __restrict__ mem, i
 template <typename T>
 static size_t lds_bytes(int n, int lines) { return (size_t)n * lines * sizeof(cplx<T>); }
 
+// Optional HIP-event timing of one kernel of the attempt sequence (for
+// benchmarks): events are recorded on the launch stream around every launch
+// of the selected kernel and summed on demand.
+namespace {
+struct Prof {
+    int kernel = -1;
+    std::vector<hipEvent_t> ev;   // start/stop pairs
+    size_t used = 0;
+};
+Prof g_prof;
+
+inline void prof_mark(int kernel, hipStream_t st) {
+    if (g_prof.kernel != kernel) return;
+    if (g_prof.used == g_prof.ev.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return;
+        g_prof.ev.push_back(e);
+    }
+    (void)hipEventRecord(g_prof.ev[g_prof.used++], st);
+}
+}  // namespace
+
+void prof_select(int kernel) {
+    g_prof.kernel = kernel;
+    g_prof.used = 0;
+}
+
+int prof_collect(double* total_ms, long long* launches) {
+    double tot = 0.0;
+    long long n = 0;
+    for (size_t i = 0; i + 1 < g_prof.used; i += 2) {
+        float ms = 0.f;
+        if (hipEventSynchronize(g_prof.ev[i + 1]) != hipSuccess) return -1;
+        if (hipEventElapsedTime(&ms, g_prof.ev[i], g_prof.ev[i + 1]) != hipSuccess) return -1;
+        tot += ms;
+        ++n;
+    }
+    *total_ms = tot;
+    *launches = n;
+    return 0;
+}
+
 template <typename T>
 void launch_attempt_t(const Member* d_mem, int nmem, const RhsShape& s, hipStream_t st) {
     dim3 gs((s.n + ST_TX - 1) / ST_TX, (s.n + ST_TY - 1) / ST_TY, nmem);
+    prof_mark(0, st);
     if (s.uniform) hipLaunchKernelGGL((k_stencil<T, true>), gs, dim3(256), 0, st, d_mem);
     else hipLaunchKernelGGL((k_stencil<T, false>), gs, dim3(256), 0, st, d_mem);
+    prof_mark(0, st);
+    prof_mark(1, st);
     hipLaunchKernelGGL(k_row_fwd<T>, dim3(s.n / (2 * s.L), nmem), dim3(s.threads_row),
                        lds_bytes<T>(s.n, s.L), st, d_mem, s.plan, s.tw, s.L);
+    prof_mark(1, st);
+    prof_mark(2, st);
     hipLaunchKernelGGL(k_col<T>, dim3(s.n / (2 * s.C), nmem), dim3(s.threads_col),
                        lds_bytes<T>(s.n, s.C), st, d_mem, s.plan, s.tw, s.tw64, s.C);
+    prof_mark(2, st);
+    prof_mark(3, st);
     hipLaunchKernelGGL(k_row_inv<T>, dim3(s.n / (2 * s.L), nmem), dim3(s.threads_row),
                        lds_bytes<T>(s.n, s.L), st, d_mem, s.plan, s.tw, s.L, s.n / (2 * s.C));
+    prof_mark(3, st);
     hipLaunchKernelGGL(k_epi, dim3(s.n / (2 * s.L), nmem), dim3(256), 0, st, d_mem, s.L);
     hipLaunchKernelGGL(k_decide, dim3(nmem), dim3(64), 0, st, d_mem, s.n / (2 * s.L));
 }

@@ -224,6 +224,20 @@ __global__ void k_begin(const Member* __restrict__ mem, int nmem, double tf, int
     c.status = (stepping && c.t < tf) ? ST_RUN : ST_IDLE;
 }
 
+__global__ void k_gather(const Member* __restrict__ mem, int nmem, CtlSummary* __restrict__ out) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= nmem) return;
+    const Ctl& c = *mem[m].ctl;
+    CtlSummary s;
+    s.t = c.t; s.dt = c.dt; s.tf = c.tf; s.maxw = c.maxw; s.maxwe = c.maxwe; s.maxdzss = c.maxdzss;
+    s.status = c.status; s.nonfinite = c.nonfinite;
+    out[m] = s;
+}
+
+void launch_gather(const Member* d_mem, int nmem, CtlSummary* d_out, hipStream_t st) {
+    hipLaunchKernelGGL(k_gather, dim3((nmem + 63) / 64), dim3(64), 0, st, d_mem, nmem, d_out);
+}
+
 // barysealevel.f90:195-252 (const / fastiso), one thread
 __global__ void k_bsl_update(BslDev* b, double year_bp) {
     b->time = year_bp;

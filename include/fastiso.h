@@ -141,9 +141,17 @@ int fi_batch_create(fi_isos* const* members, int nmem, fi_batch** out);
 int fi_batch_update(fi_batch* b, const double* const* H_ice, double time, fi_bsl* const* bsl);
 int fi_batch_destroy(fi_batch* b);
 
-/* Synchronise the library stream and report the time of the last update
- * spent on the device (ms); used by benchmarks. */
+/* Synchronise the library stream. */
 int fi_sync(void);
+
+/* Benchmark instrumentation (no reference counterpart): time every launch of
+ * one attempt kernel with HIP events on the library stream.  kernel: 0 FD
+ * stencil, 1 forward row Hartley pass, 2 column Hartley pass with spectral
+ * divide, 3 inverse row Hartley pass with the step update; -1 disables.
+ * fi_prof_collect returns the summed device time and the launch count since
+ * the last fi_prof_select. */
+int fi_prof_select(int kernel);
+int fi_prof_collect(double* total_ms, long long* launches);
 
 #ifdef __cplusplus
 }
