@@ -109,6 +109,40 @@ int make_shape(int n, int nmem, bool fp64, bool uniform, RhsShape& s) {
     s.n = n;
     s.fp64 = fp64;
     s.uniform = uniform;
+    s.fast = (n & (n - 1)) == 0 && n >= 64 && n <= 1024 && !std::getenv("FI_GENERIC_FFT");
+    if (s.fast) {
+        // register-FFT kernels: TPL = n/8 threads per line, LDS <= 64 KiB
+        const int tpl = n / 8;
+        const size_t cb = fp64 ? 16 : 8;
+        const size_t pitch = n + n / 8 + 1;
+        int best_t = 0;
+        for (int th : {256, 128, 64}) {
+            if (th < tpl) continue;
+            const int L = th / tpl;
+            if ((n + L * pitch) * cb > 65536 || n % (2 * L)) continue;
+            best_t = th;
+            if ((size_t)(n / (2 * L)) * nmem >= 512) break;
+        }
+        int best_c = 0;
+        for (int C : {16, 8, 4, 2, 1}) {
+            if (C * tpl > 1024 || ((size_t)n + (size_t)n * (C + 1)) * cb > 65536 || n % (2 * C)) continue;
+            best_c = C;
+            if ((size_t)(n / (2 * C)) * nmem >= 256) break;
+        }
+        if (best_t && best_c) {
+            s.threads_row = best_t;
+            s.L = best_t / tpl;
+            s.C = best_c;
+            s.threads_col = best_c * tpl;
+            Twiddles t;
+            int rc = get_twiddles(n, t);
+            if (rc) return rc;
+            s.tw = fp64 ? (const void*)t.d64 : (const void*)t.d32;
+            s.tw64 = t.d64;
+            return 0;
+        }
+        s.fast = false;
+    }
     // aim for >= 512 row blocks and >= 256 column blocks over the batch
     // LDS per block stays within 64 KiB (n * lines complex elements)
     const size_t cbytes = fp64 ? 16 : 8;

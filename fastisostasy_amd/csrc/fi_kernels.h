@@ -17,6 +17,7 @@ struct RhsShape {
     const double2* tw64;
     bool fp64;
     bool uniform;     // K == 1
+    bool fast;        // power-of-two N: register-FFT kernels (threads_row = block size)
 };
 
 // Geometry of the 2N-point linear convolution.
@@ -30,6 +31,8 @@ struct ConvShape {
 };
 
 void launch_attempt(const Member* d_mem, int nmem, const RhsShape& s, hipStream_t st);
+// power-of-two passes: which = 1 row forward, 2 column, 3 row inverse
+bool launch_fast_pass(const Member* d_mem, int nmem, const RhsShape& s, hipStream_t st, int which);
 // event timing of one attempt kernel: 0 stencil, 1 row DHT, 2 column DHT+divide, 3 inverse row DHT+update
 void prof_select(int kernel);
 int prof_collect(double* total_ms, long long* launches);

@@ -11,9 +11,10 @@
 //   k_row_inv      along x: DHT, /N^2, corner BC, * sec_per_year
 //                  (lv_elva.f90:118-123), then the stage sums x1/x2 and the
 //                  per-block max|x1|, max|x2|, sum (x2-x1)^2
-//   k_epi          same epilogue from a stored k (after a rejected attempt the
-//                  RHS is unchanged because it reads now%w, fastisostasy.f90:720)
-//   k_decide       scalar tol, error norm, accept / reject, dt control
+//                  (after a rejected attempt the RHS is unchanged because it
+//                  reads now%w, fastisostasy.f90:720: only this epilogue runs)
+//                  and, in the last block to finish, the scalar tol, error
+//                  norm, accept / reject and dt control
 //
 // Every kernel reads the domain's Ctl and returns at once if the domain is not
 // stepping, so the same launch sequence serves a batch whose members need
@@ -22,21 +23,9 @@
 
 #include "fi_device.hpp"
 #include "fi_kernels.h"
+#include "rhs_common.hpp"
 
 namespace fi {
-
-// ------------------------------------------------------------------ helpers
-__device__ __forceinline__ double attempt_dt(const Ctl& c, const PhysDev& ph) {
-    // integrators.f90:63,100-101,165
-    double dt = c.dt;
-    const double rem = c.tf - c.t;
-    if (ph.dt_method == 1) return fmin(dt, rem);  // rk4: h, not persisted
-    dt = fmin(dt, rem);
-    if (ph.dt_method == 2) dt = fmin(dt, ph.dt_max);
-    return dt;
-}
-
-template <typename T> __device__ __forceinline__ T ld(const void* p, size_t i) { return ((const T*)p)[i]; }
 
 // ------------------------------------------------------------------ stencil
 constexpr int ST_TX = 64, ST_TY = 8;
@@ -49,89 +38,77 @@ __global__ void __launch_bounds__(256) k_stencil(const Member* __restrict__ mem)
     const PhysDev& ph = *M.phys;
     const int N = M.n;
     const int i0 = blockIdx.x * ST_TX, j0 = blockIdx.y * ST_TY;
-    if (i0 >= N || j0 >= N) return;
     const double* __restrict__ w = c.w_is_x ? M.X[c.cur] : M.W;
     const double* __restrict__ dxm = M.dxm;
     const double dxs = ph.dx;
-    auto DX = [&](int i, int j) -> double { return UNIFORM ? dxs : dxm[(size_t)j * N + i]; };
+    // uniform spacing: divisions by dx, 2 dx and dx^2 become multiplications
+    const double inv1 = 1.0 / dxs, inv2 = 1.0 / (2 * dxs), invq = 1.0 / (dxs * dxs);
+    auto DX = [&](int i, int j) -> double { return dxm[(size_t)j * N + i]; };
+    auto Q1 = [&](double a, int i, int j) -> double { return UNIFORM ? a * inv1 : a / DX(i, j); };
+    auto Q2 = [&](double a, int i, int j) -> double { return UNIFORM ? a * inv2 : a / (2 * DX(i, j)); };
+    auto QQ = [&](double a, int i, int j) -> double { return UNIFORM ? a * invq : a / (DX(i, j) * DX(i, j)); };
 
-    constexpr int W0 = ST_TX + 4, H0 = ST_TY + 4;   // w:      [i0-2, i0+TX+2) x [j0-2, j0+TY+2)
-    constexpr int W1 = ST_TX + 2, H1 = ST_TY + 4;   // w_x:    [i0-1, i0+TX+1) x [j0-2, j0+TY+2)
-    constexpr int W2 = ST_TX + 2, H2 = ST_TY + 2;   // M**:    [i0-1, i0+TX+1) x [j0-1, j0+TY+1)
-    constexpr int W3 = ST_TX, H3 = ST_TY + 2;       // Mxy_x:  [i0,   i0+TX)   x [j0-1, j0+TY+1)
+    constexpr int W0 = ST_TX + 4, H0 = ST_TY + 4;   // w:   [i0-2, i0+TX+2) x [j0-2, j0+TY+2)
+    constexpr int W2 = ST_TX + 2, H2 = ST_TY + 2;   // M**: [i0-1, i0+TX+1) x [j0-1, j0+TY+1)
     __shared__ double sw[H0 * W0];
-    __shared__ double swx[H1 * W1];
     __shared__ double sMxx[H2 * W2], sMyy[H2 * W2], sMxy[H2 * W2];
-    __shared__ double sMxyx[H3 * W3];
     const int tid = threadIdx.x, nt = blockDim.x;
 #define SW(i, j) sw[((j) - (j0 - 2)) * W0 + ((i) - (i0 - 2))]
-#define SWX(i, j) swx[((j) - (j0 - 2)) * W1 + ((i) - (i0 - 1))]
 #define S2(a, i, j) a[((j) - (j0 - 1)) * W2 + ((i) - (i0 - 1))]
-#define S3(i, j) sMxyx[((j) - (j0 - 1)) * W3 + ((i) - i0)]
 
     for (int q = tid; q < H0 * W0; q += nt) {
         const int i = i0 - 2 + q % W0, j = j0 - 2 + q / W0;
         if (i >= 0 && i < N && j >= 0 && j < N) SW(i, j) = w[(size_t)j * N + i];
     }
     __syncthreads();
-    // w_x = Dx(w)  (finite_differences.f90:17-41)
-    for (int q = tid; q < H1 * W1; q += nt) {
-        const int i = i0 - 1 + q % W1, j = j0 - 2 + q / W1;
-        if (i < 0 || i >= N || j < 0 || j >= N) continue;
-        double v;
-        if (i == 0) v = (SW(1, j) - SW(0, j)) / DX(0, j);
-        else if (i == N - 1) v = (SW(N - 1, j) - SW(N - 2, j)) / DX(N - 1, j);
-        else v = (SW(i + 1, j) - SW(i - 1, j)) / (2 * DX(i, j));
-        SWX(i, j) = v;
-    }
-    __syncthreads();
+    // w_x (finite_differences.f90:17-41) at (ii, jj)
+    auto WX = [&](int ii, int jj) -> double {
+        if (ii == 0) return Q1(SW(1, jj) - SW(0, jj), 0, jj);
+        if (ii == N - 1) return Q1(SW(N - 1, jj) - SW(N - 2, jj), N - 1, jj);
+        return Q2(SW(ii + 1, jj) - SW(ii - 1, jj), ii, jj);
+    };
     // w_xx, w_yy, w_xy = Dy(w_x) and the plate moments (lv_elva.f90:95-104)
     const double nu = ph.nu;
     for (int q = tid; q < H2 * W2; q += nt) {
         const int i = i0 - 1 + q % W2, j = j0 - 1 + q / W2;
         if (i < 0 || i >= N || j < 0 || j >= N) continue;
         double wxx, wyy, wxy;
-        if (i == 0) wxx = (SW(2, j) - 2 * SW(1, j) + SW(0, j)) / (DX(0, j) * DX(0, j));
-        else if (i == N - 1) wxx = (SW(N - 1, j) - 2 * SW(N - 2, j) + SW(N - 3, j)) / (DX(0, j) * DX(0, j));
-        else wxx = (SW(i + 1, j) - 2 * SW(i, j) + SW(i - 1, j)) / (DX(i, j) * DX(i, j));
-        if (j == 0) wyy = (SW(i, 2) - 2 * SW(i, 1) + SW(i, 0)) / (DX(i, 0) * DX(i, 0));
-        else if (j == N - 1) wyy = (SW(i, N - 1) - 2 * SW(i, N - 2) + SW(i, N - 3)) / (DX(i, N - 1) * DX(i, N - 1));
-        else wyy = (SW(i, j + 1) - 2 * SW(i, j) + SW(i, j - 1)) / (DX(i, j) * DX(i, j));
-        if (j == 0) wxy = (SWX(i, 1) - SWX(i, 0)) / DX(i, 0);
-        else if (j == N - 1) wxy = (SWX(i, N - 1) - SWX(i, N - 2)) / DX(i, N - 1);
-        else wxy = (SWX(i, j + 1) - SWX(i, j - 1)) / (2 * DX(i, j));
+        if (i == 0) wxx = QQ(SW(2, j) - 2 * SW(1, j) + SW(0, j), 0, j);
+        else if (i == N - 1) wxx = QQ(SW(N - 1, j) - 2 * SW(N - 2, j) + SW(N - 3, j), 0, j);
+        else wxx = QQ(SW(i + 1, j) - 2 * SW(i, j) + SW(i - 1, j), i, j);
+        if (j == 0) wyy = QQ(SW(i, 2) - 2 * SW(i, 1) + SW(i, 0), i, 0);
+        else if (j == N - 1) wyy = QQ(SW(i, N - 1) - 2 * SW(i, N - 2) + SW(i, N - 3), i, N - 1);
+        else wyy = QQ(SW(i, j + 1) - 2 * SW(i, j) + SW(i, j - 1), i, j);
+        if (j == 0) wxy = Q1(WX(i, 1) - WX(i, 0), i, 0);
+        else if (j == N - 1) wxy = Q1(WX(i, N - 1) - WX(i, N - 2), i, N - 1);
+        else wxy = Q2(WX(i, j + 1) - WX(i, j - 1), i, j);
         const double D = (double)ld<T>(M.D, (size_t)j * N + i);
         S2(sMxx, i, j) = -(D * (wxx + nu * wyy));
         S2(sMyy, i, j) = -(D * (wyy + nu * wxx));
         S2(sMxy, i, j) = -(D * (1.0 - nu) * wxy);
     }
     __syncthreads();
-    // Mxy_x = Dx(Mxy)
-    for (int q = tid; q < H3 * W3; q += nt) {
-        const int i = i0 + q % W3, j = j0 - 1 + q / W3;
-        if (i >= N || j < 0 || j >= N) continue;
-        double v;
-        if (i == 0) v = (S2(sMxy, 1, j) - S2(sMxy, 0, j)) / DX(0, j);
-        else if (i == N - 1) v = (S2(sMxy, N - 1, j) - S2(sMxy, N - 2, j)) / DX(N - 1, j);
-        else v = (S2(sMxy, i + 1, j) - S2(sMxy, i - 1, j)) / (2 * DX(i, j));
-        S3(i, j) = v;
-    }
-    __syncthreads();
+    // Mxy_x = Dx(Mxy) at (ii, jj)
+    auto MXYX = [&](int ii, int jj) -> double {
+        if (ii == 0) return Q1(S2(sMxy, 1, jj) - S2(sMxy, 0, jj), 0, jj);
+        if (ii == N - 1) return Q1(S2(sMxy, N - 1, jj) - S2(sMxy, N - 2, jj), N - 1, jj);
+        return Q2(S2(sMxy, ii + 1, jj) - S2(sMxy, ii - 1, jj), ii, jj);
+    };
     // f = (p + Mx_xx + 2 Mxy_xy + My_yy) / (2 eta)   (lv_elva.f90:107-113)
     const double g = ph.g;
     for (int q = tid; q < ST_TX * ST_TY; q += nt) {
         const int i = i0 + q % ST_TX, j = j0 + q / ST_TX;
         if (i >= N || j >= N) continue;
         double mxx, myy, mxyxy;
-        if (i == 0) mxx = (S2(sMxx, 2, j) - 2 * S2(sMxx, 1, j) + S2(sMxx, 0, j)) / (DX(0, j) * DX(0, j));
-        else if (i == N - 1) mxx = (S2(sMxx, N - 1, j) - 2 * S2(sMxx, N - 2, j) + S2(sMxx, N - 3, j)) / (DX(0, j) * DX(0, j));
-        else mxx = (S2(sMxx, i + 1, j) - 2 * S2(sMxx, i, j) + S2(sMxx, i - 1, j)) / (DX(i, j) * DX(i, j));
-        if (j == 0) myy = (S2(sMyy, i, 2) - 2 * S2(sMyy, i, 1) + S2(sMyy, i, 0)) / (DX(i, 0) * DX(i, 0));
-        else if (j == N - 1) myy = (S2(sMyy, i, N - 1) - 2 * S2(sMyy, i, N - 2) + S2(sMyy, i, N - 3)) / (DX(i, N - 1) * DX(i, N - 1));
-        else myy = (S2(sMyy, i, j + 1) - 2 * S2(sMyy, i, j) + S2(sMyy, i, j - 1)) / (DX(i, j) * DX(i, j));
-        if (j == 0) mxyxy = (S3(i, 1) - S3(i, 0)) / DX(i, 0);
-        else if (j == N - 1) mxyxy = (S3(i, N - 1) - S3(i, N - 2)) / DX(i, N - 1);
-        else mxyxy = (S3(i, j + 1) - S3(i, j - 1)) / (2 * DX(i, j));
+        if (i == 0) mxx = QQ(S2(sMxx, 2, j) - 2 * S2(sMxx, 1, j) + S2(sMxx, 0, j), 0, j);
+        else if (i == N - 1) mxx = QQ(S2(sMxx, N - 1, j) - 2 * S2(sMxx, N - 2, j) + S2(sMxx, N - 3, j), 0, j);
+        else mxx = QQ(S2(sMxx, i + 1, j) - 2 * S2(sMxx, i, j) + S2(sMxx, i - 1, j), i, j);
+        if (j == 0) myy = QQ(S2(sMyy, i, 2) - 2 * S2(sMyy, i, 1) + S2(sMyy, i, 0), i, 0);
+        else if (j == N - 1) myy = QQ(S2(sMyy, i, N - 1) - 2 * S2(sMyy, i, N - 2) + S2(sMyy, i, N - 3), i, N - 1);
+        else myy = QQ(S2(sMyy, i, j + 1) - 2 * S2(sMyy, i, j) + S2(sMyy, i, j - 1), i, j);
+        if (j == 0) mxyxy = Q1(MXYX(i, 1) - MXYX(i, 0), i, 0);
+        else if (j == N - 1) mxyxy = Q1(MXYX(i, N - 1) - MXYX(i, N - 2), i, N - 1);
+        else mxyxy = Q2(MXYX(i, j + 1) - MXYX(i, j - 1), i, j);
         const size_t cidx = (size_t)j * N + i;
         const double p = -(g * (double)ld<T>(M.cfull, cidx));
         const double num = p + mxx + 2.0 * mxyxy + myy;
@@ -141,9 +118,81 @@ __global__ void __launch_bounds__(256) k_stencil(const Member* __restrict__ mem)
         ((T*)M.g1)[cidx] = f;
     }
 #undef SW
-#undef SWX
 #undef S2
-#undef S3
+}
+
+// Uniform spacing (K == 1): every one-sided formula of finite_differences.f90
+// is the interior formula at a clamped centre (second derivatives: centre
+// clamped to [1, N-2]; first derivatives: (u(hi) - u(lo)) / ((hi - lo) dx) with
+// lo = max(i-1, 0), hi = min(i+1, N-1)), so the tile runs without divergent
+// boundary branches.  Halo cells outside the domain compute finite values
+// from clamped loads that no in-domain output reads.
+template <typename T>
+__global__ void __launch_bounds__(256) k_stencil_u(const Member* __restrict__ mem) {
+    const Member& M = mem[blockIdx.z];
+    const Ctl& c = *M.ctl;
+    if (c.status != ST_RUN || c.k_valid) return;
+    const PhysDev& ph = *M.phys;
+    const int N = M.n;
+    const int i0 = blockIdx.x * ST_TX, j0 = blockIdx.y * ST_TY;
+    const double* __restrict__ w = c.w_is_x ? M.X[c.cur] : M.W;
+    const double dxs = ph.dx;
+    const double inv1 = 1.0 / dxs, inv2 = 1.0 / (2 * dxs), invq = 1.0 / (dxs * dxs);
+    constexpr int W0 = ST_TX + 4, H0 = ST_TY + 4;   // w:   [i0-2, i0+TX+2) x [j0-2, j0+TY+2)
+    constexpr int W2 = ST_TX + 2, H2 = ST_TY + 2;   // M**: [i0-1, i0+TX+1) x [j0-1, j0+TY+1)
+    __shared__ double sw[H0 * W0];
+    __shared__ double sMxx[H2 * W2], sMyy[H2 * W2], sMxy[H2 * W2];
+    const int tid = threadIdx.x, nt = blockDim.x;
+    auto cl = [&](int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); };
+#define SW(i, j) sw[((j) - (j0 - 2)) * W0 + ((i) - (i0 - 2))]
+#define S2(a, i, j) a[((j) - (j0 - 1)) * W2 + ((i) - (i0 - 1))]
+    for (int q = tid; q < H0 * W0; q += nt) {
+        const int i = cl(i0 - 2 + q % W0, 0, N - 1), j = cl(j0 - 2 + q / W0, 0, N - 1);
+        sw[q] = w[(size_t)j * N + i];
+    }
+    __syncthreads();
+    const double nu = ph.nu;
+    for (int q = tid; q < H2 * W2; q += nt) {
+        const int i = i0 - 1 + q % W2, j = j0 - 1 + q / W2;
+        const int jc = cl(j, 0, N - 1), ic = cl(i, 0, N - 1);
+        const int ii = cl(i, 1, N - 2), jj = cl(j, 1, N - 2);
+        const double wxx = (SW(ii + 1, jc) - 2 * SW(ii, jc) + SW(ii - 1, jc)) * invq;
+        const double wyy = (SW(ic, jj + 1) - 2 * SW(ic, jj) + SW(ic, jj - 1)) * invq;
+        const int ilo = i - 1 < 0 ? 0 : i - 1, ihi = i + 1 > N - 1 ? N - 1 : i + 1;
+        const int jlo = j - 1 < 0 ? 0 : j - 1, jhi = j + 1 > N - 1 ? N - 1 : j + 1;
+        const double sx = (ihi - ilo == 2) ? inv2 : inv1, sy = (jhi - jlo == 2) ? inv2 : inv1;
+        const double wx_hi = (SW(cl(ihi, i0 - 2, i0 + ST_TX + 1), jhi) - SW(cl(ilo, i0 - 2, i0 + ST_TX + 1), jhi)) * sx;
+        const double wx_lo = (SW(cl(ihi, i0 - 2, i0 + ST_TX + 1), jlo) - SW(cl(ilo, i0 - 2, i0 + ST_TX + 1), jlo)) * sx;
+        const double wxy = (wx_hi - wx_lo) * sy;
+        const double D = (double)ld<T>(M.D, (size_t)jc * N + ic);
+        sMxx[q] = -(D * (wxx + nu * wyy));
+        sMyy[q] = -(D * (wyy + nu * wxx));
+        sMxy[q] = -(D * (1.0 - nu) * wxy);
+    }
+    __syncthreads();
+    const double g = ph.g;
+    for (int q = tid; q < ST_TX * ST_TY; q += nt) {
+        const int i = i0 + q % ST_TX, j = j0 + q / ST_TX;
+        if (i >= N || j >= N) continue;
+        const int ii = cl(i, 1, N - 2), jj = cl(j, 1, N - 2);
+        const double mxx = (S2(sMxx, ii + 1, j) - 2 * S2(sMxx, ii, j) + S2(sMxx, ii - 1, j)) * invq;
+        const double myy = (S2(sMyy, i, jj + 1) - 2 * S2(sMyy, i, jj) + S2(sMyy, i, jj - 1)) * invq;
+        const int ilo = i - 1 < 0 ? 0 : i - 1, ihi = i + 1 > N - 1 ? N - 1 : i + 1;
+        const int jlo = j - 1 < 0 ? 0 : j - 1, jhi = j + 1 > N - 1 ? N - 1 : j + 1;
+        const double sx = (ihi - ilo == 2) ? inv2 : inv1, sy = (jhi - jlo == 2) ? inv2 : inv1;
+        const double mxyx_hi = (S2(sMxy, ihi, jhi) - S2(sMxy, ilo, jhi)) * sx;
+        const double mxyx_lo = (S2(sMxy, ihi, jlo) - S2(sMxy, ilo, jlo)) * sx;
+        const double mxyxy = (mxyx_hi - mxyx_lo) * sy;
+        const size_t cidx = (size_t)j * N + i;
+        const double p = -(g * (double)ld<T>(M.cfull, cidx));
+        const double num = p + mxx + 2.0 * mxyxy + myy;
+        T f;
+        if constexpr (sizeof(T) == 8) f = (T)(num / (2.0 * (double)ld<T>(M.eta, cidx)));
+        else f = (T)(num * (double)ld<T>(M.eta, cidx));   // mixed: eta holds 1/(2 eta)
+        ((T*)M.g1)[cidx] = f;
+    }
+#undef SW
+#undef S2
 }
 
 // ------------------------------------------------------------------ row pass
@@ -249,153 +298,70 @@ __global__ void __launch_bounds__(512) k_col(const Member* __restrict__ mem, Fft
     }
 }
 
-// ------------------------------------------------------------------ epilogue
-struct EpiOut { double x1max, x2max, d2; };
-
-// Stage sums of integrators.f90:12-203 for one cell, in the reference's
-// association order.  Returns x2 (the candidate / new state).
-__device__ __forceinline__ double stage_sums(int method, double x, double k, double dt,
-                                             double& x1) {
-    if (method == 0) { x1 = x + dt * k; return x1; }                       // euler
-    if (method == 1) { x1 = x + (dt / 6) * (((k + 2 * k) + 2 * k) + k); return x1; }  // rk4
-    if (method == 2) {                                                     // bs32
-        x1 = x + dt / 9 * ((2 * k + 3 * k) + 4 * k);
-        return x + dt * (((0.0 * k + 0.0 * k) + 0.0 * k) + 3 * k);
-    }
-    // tsit54: b = (0.0845, 0, 0, 0.5179, 0.1276, 0.27), bh(6) = 0.27 - 1/5
-    const double b1 = dt * 0.0845, b4 = dt * 0.5179, b5 = dt * 0.1276, b6 = dt * 0.2700;
-    const double z = dt * 0.0, h6 = dt * (0.2700 - 1 / 5.0);
-    x1 = x + b1 * k + z * k + z * k + b4 * k + b5 * k + h6 * k;
-    return x + b1 * k + z * k + z * k + b4 * k + b5 * k + b6 * k;
-}
-
-__device__ __forceinline__ void block_reduce3(double a, double b, double s, double* out) {
-    __shared__ double red[3][16];
-    a = wave_max(a); b = wave_max(b); s = wave_sum(s);
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    if (lane == 0) { red[0][wv] = a; red[1][wv] = b; red[2][wv] = s; }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const int nw = (blockDim.x + 63) >> 6;
-        double A = red[0][0], B = red[1][0], S = red[2][0];
-        for (int q = 1; q < nw; ++q) { A = fmax(A, red[0][q]); B = fmax(B, red[1][q]); S += red[2][q]; }
-        out[0] = A; out[1] = B; out[2] = S;
-    }
-}
-
 template <typename T>
 __global__ void __launch_bounds__(512) k_row_inv(const Member* __restrict__ mem, FftPlan plan,
                                                  const void* __restrict__ twv, int L, int ncolblk) {
     const Member& M = mem[blockIdx.y];
     const Ctl& c = *M.ctl;
-    if (c.status != ST_RUN || c.k_valid) return;
+    if (c.status != ST_RUN) return;
     const PhysDev& ph = *M.phys;
     extern __shared__ __align__(16) unsigned char smem[];
     cplx<T>* buf = (cplx<T>*)smem;
     __shared__ double corner_c;
     const int N = M.n, r0 = blockIdx.x * 2 * L;
     const cplx<T>* tw = (const cplx<T>*)twv;
-    if (threadIdx.x == 0) {
-        // four corners of the raw inverse transform, fixed summation order
-        double v[4] = {0, 0, 0, 0};
-        for (int b = 0; b < ncolblk; ++b)
-            for (int q = 0; q < 4; ++q) v[q] += M.corners[(size_t)b * 4 + q];
+    const double dt = attempt_dt(c, ph);
+    const double* __restrict__ X = M.X[c.cur];
+    double* __restrict__ Xn = M.X[c.cur ^ 1];
+    double a1 = 0.0, a2 = 0.0, s2 = 0.0;
+    auto update = [&](size_t cidx, double k) {
+        double x1;
+        const double x2 = stage_sums(ph.dt_method, X[cidx], k, dt, x1);
+        Xn[cidx] = x2;
+        a1 = fmax(a1, fabs(x1)); a2 = fmax(a2, fabs(x2));
+        const double d = x2 - x1;
+        s2 += d * d;
+    };
+    if (c.k_valid) {
+        // rejected attempt before: the RHS is unchanged, only dt differs
+        for (int q = threadIdx.x; q < 2 * L * N; q += blockDim.x) {
+            const size_t cidx = (size_t)r0 * N + q;
+            update(cidx, M.dwdt[cidx]);
+        }
+    } else {
+        if (threadIdx.x == 0) {
+            // four corners of the raw inverse transform, fixed summation order
+            double v[4] = {0, 0, 0, 0};
+            for (int b = 0; b < ncolblk; ++b)
+                for (int q = 0; q < 4; ++q) v[q] += M.corners[(size_t)b * 4 + q];
+            const double S = (double)(float)((double)N * (double)N);
+            // x(1,1) + x(nx,ny) + x(1,ny) + x(nx,1)  (isos_utils.f90:116)
+            corner_c = 0.25 * (v[0] / S + v[3] / S + v[2] / S + v[1] / S);
+        }
+        load_rows<T>(buf, (const T*)M.g1, N, L, r0);   // also synchronises corner_c
+        lds_fft<T>(buf, plan, L, L, tw);
         const double S = (double)(float)((double)N * (double)N);
-        // x(1,1) + x(nx,ny) + x(1,ny) + x(nx,1)  (isos_utils.f90:116)
-        corner_c = 0.25 * (v[0] / S + v[3] / S + v[2] / S + v[1] / S);
-    }
-    load_rows<T>(buf, (const T*)M.g1, N, L, r0);   // also synchronises corner_c
-    lds_fft<T>(buf, plan, L, L, tw);
-    const double S = (double)(float)((double)N * (double)N);
-    const double cc = corner_c, spy = ph.sec_per_year;
-    const double dt = attempt_dt(c, ph);
-    const double* __restrict__ X = M.X[c.cur];
-    double* __restrict__ Xn = M.X[c.cur ^ 1];
-    double a1 = 0.0, a2 = 0.0, s2 = 0.0;
-    for (int q = threadIdx.x; q < 2 * L * N; q += blockDim.x) {
-        const int mr = q / N, i = q % N;
-        const int l = mr % L, part = mr / L;
-        T ha, hb;
-        dht_pair(buf[i * L + l], buf[((N - i) % N) * L + l], ha, hb);
-        const double raw = (double)(part ? hb : ha);
-        const double k = (raw / S - cc) * spy;
-        const size_t cidx = (size_t)(r0 + mr) * N + i;
-        const double x = X[cidx];
-        double x1;
-        const double x2 = stage_sums(ph.dt_method, x, k, dt, x1);
-        M.dwdt[cidx] = k;
-        Xn[cidx] = x2;
-        a1 = fmax(a1, fabs(x1)); a2 = fmax(a2, fabs(x2));
-        const double d = x2 - x1;
-        s2 += d * d;
+        const double cc = corner_c, spy = ph.sec_per_year;
+        const bool store_k = !(c.t + dt < c.tf);
+        for (int q = threadIdx.x; q < 2 * L * N; q += blockDim.x) {
+            const int mr = q / N, i = q % N;
+            const int l = mr % L, part = mr / L;
+            T ha, hb;
+            dht_pair(buf[i * L + l], buf[((N - i) % N) * L + l], ha, hb);
+            const double raw = (double)(part ? hb : ha);
+            const double k = (raw / S - cc) * spy;
+            const size_t cidx = (size_t)(r0 + mr) * N + i;
+            if (store_k) M.dwdt[cidx] = k;
+            update(cidx, k);
+        }
     }
     block_reduce3(a1, a2, s2, M.partials + (size_t)blockIdx.x * 4);
 }
 
-// Candidate from a stored k (after a rejection).  Same block decomposition
-// as k_row_inv so that the partial layout is identical.
-__global__ void __launch_bounds__(256) k_epi(const Member* __restrict__ mem, int L) {
-    const Member& M = mem[blockIdx.y];
-    const Ctl& c = *M.ctl;
-    if (c.status != ST_RUN || !c.k_valid) return;
-    const PhysDev& ph = *M.phys;
-    const int N = M.n, r0 = blockIdx.x * 2 * L;
-    const double dt = attempt_dt(c, ph);
-    const double* __restrict__ X = M.X[c.cur];
-    double* __restrict__ Xn = M.X[c.cur ^ 1];
-    double a1 = 0.0, a2 = 0.0, s2 = 0.0;
-    for (int q = threadIdx.x; q < 2 * L * N; q += blockDim.x) {
-        const size_t cidx = (size_t)r0 * N + q;
-        double x1;
-        const double x2 = stage_sums(ph.dt_method, X[cidx], M.dwdt[cidx], dt, x1);
-        Xn[cidx] = x2;
-        a1 = fmax(a1, fabs(x1)); a2 = fmax(a2, fabs(x2));
-        const double d = x2 - x1;
-        s2 += d * d;
-    }
-    block_reduce3(a1, a2, s2, M.partials + (size_t)blockIdx.x * 4);
-}
-
-// One wave per domain: error norm and step control (integrators.f90:186-199).
 __global__ void __launch_bounds__(64) k_decide(const Member* __restrict__ mem, int nblk) {
     const Member& M = mem[blockIdx.x];
-    Ctl& c = *M.ctl;
-    if (c.status != ST_RUN) return;
-    const PhysDev& ph = *M.phys;
-    const int lane = threadIdx.x;
-    double a1 = 0.0, a2 = 0.0, s2 = 0.0;
-    for (int b = lane; b < nblk; b += 64) {
-        a1 = fmax(a1, M.partials[(size_t)b * 4 + 0]);
-        a2 = fmax(a2, M.partials[(size_t)b * 4 + 1]);
-        s2 += M.partials[(size_t)b * 4 + 2];
-    }
-    a1 = wave_max(a1); a2 = wave_max(a2); s2 = wave_sum(s2);
-    if (lane != 0) return;
-    const double dt = attempt_dt(c, ph);
-    c.attempts++;
-    const int m = ph.dt_method;
-    bool accept = true;
-    if (m == 2 || m == 3) {
-        const double tol = ph.atol + ph.rtol * fmax(a1, a2);
-        const double nn = (double)M.n * (double)M.n;
-        const double err = sqrt(s2 / (tol * tol) / nn);
-        c.tol = tol; c.err = err;
-        accept = err < 1;
-        if (accept) c.dt = fmin(dt * pow(1 / err, 1 / ph.q), ph.dt_max);
-        else c.dt = dt * 0.5;
-    } else if (m == 0) {
-        c.dt = dt;
-    }
-    if (accept) {
-        c.t = c.t + dt;
-        c.cur ^= 1;
-        c.w_is_x = 1;
-        c.k_valid = 0;
-        c.accepts++;
-    } else {
-        c.k_valid = 1;
-    }
-    if (!(c.t < c.tf)) c.status = ST_IDLE;
+    if (M.ctl->status != ST_RUN) return;
+    decide_wave(M, nblk);
 }
 
 // ------------------------------------------------------------------ launchers
@@ -448,22 +414,24 @@ template <typename T>
 void launch_attempt_t(const Member* d_mem, int nmem, const RhsShape& s, hipStream_t st) {
     dim3 gs((s.n + ST_TX - 1) / ST_TX, (s.n + ST_TY - 1) / ST_TY, nmem);
     prof_mark(0, st);
-    if (s.uniform) hipLaunchKernelGGL((k_stencil<T, true>), gs, dim3(256), 0, st, d_mem);
+    if (s.uniform) hipLaunchKernelGGL(k_stencil_u<T>, gs, dim3(256), 0, st, d_mem);
     else hipLaunchKernelGGL((k_stencil<T, false>), gs, dim3(256), 0, st, d_mem);
     prof_mark(0, st);
     prof_mark(1, st);
-    hipLaunchKernelGGL(k_row_fwd<T>, dim3(s.n / (2 * s.L), nmem), dim3(s.threads_row),
-                       lds_bytes<T>(s.n, s.L), st, d_mem, s.plan, s.tw, s.L);
+    if (!s.fast || !launch_fast_pass(d_mem, nmem, s, st, 1))
+        hipLaunchKernelGGL(k_row_fwd<T>, dim3(s.n / (2 * s.L), nmem), dim3(s.threads_row),
+                           lds_bytes<T>(s.n, s.L), st, d_mem, s.plan, s.tw, s.L);
     prof_mark(1, st);
     prof_mark(2, st);
-    hipLaunchKernelGGL(k_col<T>, dim3(s.n / (2 * s.C), nmem), dim3(s.threads_col),
-                       lds_bytes<T>(s.n, s.C), st, d_mem, s.plan, s.tw, s.tw64, s.C);
+    if (!s.fast || !launch_fast_pass(d_mem, nmem, s, st, 2))
+        hipLaunchKernelGGL(k_col<T>, dim3(s.n / (2 * s.C), nmem), dim3(s.threads_col),
+                           lds_bytes<T>(s.n, s.C), st, d_mem, s.plan, s.tw, s.tw64, s.C);
     prof_mark(2, st);
     prof_mark(3, st);
-    hipLaunchKernelGGL(k_row_inv<T>, dim3(s.n / (2 * s.L), nmem), dim3(s.threads_row),
-                       lds_bytes<T>(s.n, s.L), st, d_mem, s.plan, s.tw, s.L, s.n / (2 * s.C));
+    if (!s.fast || !launch_fast_pass(d_mem, nmem, s, st, 3))
+        hipLaunchKernelGGL(k_row_inv<T>, dim3(s.n / (2 * s.L), nmem), dim3(s.threads_row),
+                           lds_bytes<T>(s.n, s.L), st, d_mem, s.plan, s.tw, s.L, s.n / (2 * s.C));
     prof_mark(3, st);
-    hipLaunchKernelGGL(k_epi, dim3(s.n / (2 * s.L), nmem), dim3(256), 0, st, d_mem, s.L);
     hipLaunchKernelGGL(k_decide, dim3(nmem), dim3(64), 0, st, d_mem, s.n / (2 * s.L));
 }
 
