@@ -39,5 +39,30 @@ def build(verbose=False):
     return LIB
 
 
+FORTRAN_DIR = os.path.join(HERE, "fortran")
+FLANG = os.environ.get("FLANG", "/opt/rocm/bin/amdflang")
+
+
+def build_fortran(verbose=False):
+    """Compile the iso_c_binding module (fortran/fastisostasy.f90) and the
+    Fortran caller drive_disc against libfastiso.so.  Returns the driver path
+    or None when amdflang is not installed."""
+    if not os.path.exists(FLANG):
+        return None
+    mod = os.path.join(HERE, "_build", "fortran")
+    os.makedirs(mod, exist_ok=True)
+    obj = os.path.join(mod, "fastisostasy.o")
+    exe = os.path.join(FORTRAN_DIR, "drive_disc")
+    cmds = [[FLANG, "-O2", "-c", os.path.join(FORTRAN_DIR, "fastisostasy.f90"), "-J", mod, "-o", obj],
+            [FLANG, "-O2", os.path.join(FORTRAN_DIR, "drive_disc.f90"), "-J", mod, obj, "-L" + HERE,
+             "-lfastiso", "-Wl,-rpath,$ORIGIN/..", "-o", exe]]
+    for cmd in cmds:
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+    return exe
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+    print(build_fortran(verbose="-v" in sys.argv))

@@ -1,0 +1,55 @@
+"""The iso_c_binding drop-in module: a Fortran caller shaped like the
+reference driver (tests/test_isostasy.f90) reads an &isostasy namelist and
+runs through libfastiso.so."""
+import os
+import shutil
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from fastisostasy_amd import build as fb
+from fastisostasy_amd import workloads as W
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+NML = os.path.join(HERE, "golden", "disc.nml")
+EXE = os.path.join(fb.FORTRAN_DIR, "drive_disc")
+
+needs_flang = pytest.mark.skipif(not os.path.exists(fb.FLANG), reason="amdflang not installed")
+
+
+@needs_flang
+def test_fortran_module_builds_and_stops_without_gpu(tmp_path):
+    exe = fb.build_fortran()
+    assert os.path.exists(exe)
+    from fastisostasy_amd import _lib
+    if _lib.load().fi_sync() == 0:
+        pytest.skip("a HIP device is present")
+    r = subprocess.run([exe, NML, "32", "50e3", "20", str(tmp_path / "o.bin")], capture_output=True, text=True)
+    assert r.returncode != 0
+    assert "no HIP device" in r.stdout + r.stderr
+
+
+@pytest.mark.gpu
+def test_fortran_driver_matches_oracle(tmp_path):
+    if not os.path.exists(EXE):
+        pytest.skip("drive_disc not built")
+    years = 500.0
+    out = tmp_path / "w.bin"
+    r = subprocess.run([EXE, NML, "128", "25e3", str(years), str(out)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    raw = np.fromfile(out, dtype=np.float64)
+    w_f, zb_f = raw[:128 * 128].reshape(128, 128), raw[128 * 128:].reshape(128, 128)
+    wl = W.plain_disc()
+    o = O.OracleIsos(O.make_params(**wl["params"]), wl["nx"], wl["nx"], wl["dx"])
+    b = O.OracleBSL("const", 0.0)
+    o.init_state(wl["z_bed"], np.zeros_like(wl["H_ice"]), 0.0, b)
+    for k in range(1, int(years / 10) + 1):
+        t = 10.0 * k
+        b.update(t)
+        o.update(wl["H_ice"], t, b)
+    assert np.abs(o.get("w") - w_f).max() < 1e-6
+    assert np.abs(o.get("z_bed") - zb_f).max() < 1e-6
