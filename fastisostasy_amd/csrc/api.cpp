@@ -175,10 +175,14 @@ int make_conv_shape(int n, bool fp64, ConvShape& s) {
     const size_t cbytes = fp64 ? 16 : 8;
     auto fits = [&](int lines) { return (size_t)s.m * lines * cbytes <= 65536; };
     if (!fits(1)) return fail(FI_E_UNSUPPORTED, "convolution size 2N=%d too large for one LDS line", s.m);
+    // lines per block: LDS-bounded and at most FFT_MAXIT * 512 butterflies per stage
+    int nbmax = 0;
+    for (int q = 0; q < s.plan.nf; ++q) nbmax = std::max(nbmax, s.m / s.plan.radix[q]);
+    auto ok = [&](int lines) { return fits(lines) && nbmax * lines <= FFT_MAXIT * 512; };
     int L = 1;
-    while (L < 4 && n % (4 * L) == 0 && fits(2 * L)) L *= 2;
+    while (L < 4 && n % (4 * L) == 0 && ok(2 * L)) L *= 2;
     int C = 1;
-    while (C < 8 && fits(2 * C)) C *= 2;
+    while (C < 8 && ok(2 * C)) C *= 2;
     s.Lr = L;
     s.Cc = C;
     int rc = fft_threads(s.plan, L, s.threads_row);
