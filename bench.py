@@ -30,14 +30,20 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 KERNEL_NAMES = {0: "k_stencil", 1: "k_row_fwd", 2: "k_col", 3: "k_row_inv"}
+FUSED_NAME = "kf_srow_fwd (stencil + forward row pass)"
 
 
-def kernel_bytes_per_cell(kernel, s):
-    """Algorithmic HBM bytes per cell per launch (s = bytes per RHS real)."""
+def kernel_bytes_per_cell(kernel, s, fused):
+    """Algorithmic HBM bytes per cell per launch (s = bytes per RHS real).
+
+    dwdt is written only by the attempt that reaches the coupling time, so the
+    inverse row pass is priced without it."""
+    if kernel == 0 and fused:
+        return 8 + 3 * s + s       # read x (f64), D, 1/(2 eta), canom_full; write row-transformed
     return {0: 8 + 4 * s,          # read x (f64), D, 1/(2 eta), canom_full; write f
             1: 2 * s,              # read f, write row-transformed
             2: 3 * s,              # read, read kappa*R, write
-            3: s + 8 + 8 + 8}[kernel]  # read, read x, write x2, write dwdt (f64)
+            3: s + 8 + 8}[kernel]  # read, read x (f64), write x2 (f64)
 
 
 def build_members(nmem, precision):
@@ -90,7 +96,7 @@ def run_batched(nmem, steps, warmup, precision, prof_kernel=None):
         ms, cnt = ctypes.c_double(), ctypes.c_longlong()
         check(load().fi_prof_collect(ctypes.byref(ms), ctypes.byref(cnt)))
         check(load().fi_prof_select(-1))
-        prof = (ms.value, cnt.value)
+        prof = (ms.value, cnt.value, bool(load().fi_prof_fused()))
     attempts = sum(h.scalar("attempts") for h in hs)
     w = [h.full("w") for h in hs]
     n = hs[0].n
@@ -151,7 +157,23 @@ def run_cpu_baseline(seconds):
     return dict(value=n * n * 10.0 * steps / wall, steps=steps, wall=wall, n=n)
 
 
+def calibrate_copy(mib):
+    """A device-to-device copy of known size, for scaling the FETCH_SIZE /
+    WRITE_SIZE counters of rocprofv3 (not part of the benchmark)."""
+    import torch
+    n = mib * (1 << 20) // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": 5}))
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "--calibrate-copy":
+        calibrate_copy(int(sys.argv[2]) if len(sys.argv) > 2 else 256)
+        return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=300)
@@ -160,7 +182,7 @@ def main():
     ap.add_argument("--plain-members", type=int, default=2)
     ap.add_argument("--plain-steps", type=int, default=30)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--roofline-kernel", type=int, default=3)
+    ap.add_argument("--roofline-kernel", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     args = ap.parse_args()
@@ -194,11 +216,12 @@ def main():
 
     roof = None
     if res["prof"] and res["prof"][1] > 0:
-        ms, cnt = res["prof"]
+        ms, cnt, fused = res["prof"]
         avg_s = ms / cnt * 1e-3
-        bytes_per_launch = kernel_bytes_per_cell(args.roofline_kernel, 4) * cells
+        bytes_per_launch = kernel_bytes_per_cell(args.roofline_kernel, 4, fused) * cells
         achieved = bytes_per_launch / avg_s / 1e9
-        roof = {"kernel": KERNEL_NAMES[args.roofline_kernel], "bound": "hbm", "achieved": round(achieved, 1),
+        name = FUSED_NAME if (fused and args.roofline_kernel == 0) else KERNEL_NAMES[args.roofline_kernel]
+        roof = {"kernel": name, "bound": "hbm", "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": None, "avg_launch_us": round(avg_s * 1e6, 2), "launches": cnt,
                 "bytes_per_launch": bytes_per_launch}

@@ -27,7 +27,7 @@ EXPORTS = ["fi_last_error", "fi_version", "fi_padded_size", "fi_isos_create", "f
            "fi_isos_init_state", "fi_isos_update", "fi_isos_get", "fi_isos_get_mask", "fi_isos_get_full",
            "fi_isos_get_scalar", "fi_isos_destroy", "fi_bsl_create", "fi_bsl_update", "fi_bsl_get",
            "fi_bsl_destroy", "fi_batch_create", "fi_batch_update", "fi_batch_destroy", "fi_sync",
-           "fi_prof_select", "fi_prof_collect"]
+           "fi_prof_select", "fi_prof_collect", "fi_prof_fused"]
 
 
 class FiParams(ctypes.Structure):
@@ -88,6 +88,7 @@ def load():
     L.fi_batch_destroy.argtypes = [vp]
     L.fi_prof_select.argtypes = [i]
     L.fi_prof_collect.argtypes = [dp, ctypes.POINTER(ctypes.c_longlong)]
+    L.fi_prof_fused.argtypes = []
     for name in EXPORTS:
         if name not in ("fi_last_error", "fi_version"):
             getattr(L, name).restype = i

@@ -116,12 +116,20 @@ int make_shape(int n, int nmem, bool fp64, bool uniform, RhsShape& s) {
         const size_t cb = fp64 ? 16 : 8;
         const size_t pitch = n + n / 8 + 1;
         int best_t = 0;
-        for (int th : {256, 128, 64}) {
-            if (th < tpl) continue;
+        auto row_ok = [&](int th) {
+            if (th < tpl) return false;
             const int L = th / tpl;
-            if ((n + L * pitch) * cb > 65536 || n % (2 * L)) continue;
+            return (n + L * pitch) * cb <= 65536 && n % (2 * L) == 0;
+        };
+        for (int th : {256, 128, 64}) {
+            if (!row_ok(th)) continue;
             best_t = th;
-            if ((size_t)(n / (2 * L)) * nmem >= 512) break;
+            if ((size_t)(n / (2 * th / tpl)) * nmem >= 512) break;
+        }
+        // FI_ROW_THREADS = 64 | 128 | 256 overrides the row block size (tuning, tests)
+        if (const char* e = std::getenv("FI_ROW_THREADS")) {
+            const int th = std::atoi(e);
+            if ((th == 64 || th == 128 || th == 256) && row_ok(th)) best_t = th;
         }
         int best_c = 0;
         for (int C : {16, 8, 4, 2, 1}) {
@@ -130,6 +138,7 @@ int make_shape(int n, int nmem, bool fp64, bool uniform, RhsShape& s) {
             if ((size_t)(n / (2 * C)) * nmem >= 256) break;
         }
         if (best_t && best_c) {
+            s.fused = uniform && !std::getenv("FI_NO_FUSED_STENCIL");
             s.threads_row = best_t;
             s.L = best_t / tpl;
             s.C = best_c;
@@ -573,6 +582,9 @@ int fi_isos_create(const fi_params* p, int nx_ice, int ny_ice, double dx, double
     ph.dt_diagnostics = P.dt_diagnostics;
     ph.atol = P.atol; ph.rtol = P.rtol; ph.dt_max = P.dt_max; ph.q = P.q;
     ph.dx = dx;
+    ph.inv_dx = 1.0 / dx;
+    ph.inv_2dx = 1.0 / (2 * dx);
+    ph.inv_dx2 = 1.0 / (dx * dx);
     ph.dt_method = P.dt_method;
     ph.interactive_sealevel = P.interactive_sealevel; ph.include_elastic = P.include_elastic;
     ph.heterogeneous_ssh = P.heterogeneous_ssh;
@@ -925,6 +937,8 @@ int fi_prof_collect(double* total_ms, long long* launches) {
     if (prof_collect(total_ms, launches)) return fail(FI_E_HIP, "event timing failed");
     return 0;
 }
+
+int fi_prof_fused(void) { return prof_fused(); }
 
 int fi_batch_destroy(fi_batch* b) {
     if (!b) return 0;

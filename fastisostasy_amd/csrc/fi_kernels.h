@@ -18,6 +18,7 @@ struct RhsShape {
     bool fp64;
     bool uniform;     // K == 1
     bool fast;        // power-of-two N: register-FFT kernels (threads_row = block size)
+    bool fused;       // fast && uniform: stencil fused into the forward row pass
 };
 
 // Geometry of the 2N-point linear convolution.
@@ -33,9 +34,10 @@ struct ConvShape {
 void launch_attempt(const Member* d_mem, int nmem, const RhsShape& s, hipStream_t st);
 // power-of-two passes: which = 1 row forward, 2 column, 3 row inverse
 bool launch_fast_pass(const Member* d_mem, int nmem, const RhsShape& s, hipStream_t st, int which);
-// event timing of one attempt kernel: 0 stencil, 1 row DHT, 2 column DHT+divide, 3 inverse row DHT+update
+// event timing of one attempt kernel: 0 stencil (fused: stencil + row DHT), 1 row DHT, 2 column DHT+divide, 3 inverse row DHT+update
 void prof_select(int kernel);
 int prof_collect(double* total_ms, long long* launches);
+int prof_fused();
 
 // bookkeeping of isos_update (sealevel.f90, fastisostasy.f90:750-833)
 void launch_pre_haf(const Member* d_mem, int nmem, int n, bool new_H, hipStream_t st);

@@ -62,7 +62,8 @@ struct PhysDev {
     double sec_per_year, Vden_factor, dt_diagnostics;
     double atol, rtol, dt_max, q;
     double dx;
-    int dt_method;               // 0 euler, 1 rk4, 2 bs32, 3 tsit54
+    double inv_dx, inv_2dx, inv_dx2;   // 1/dx, 1/(2 dx), 1/(dx*dx), rounded once on the host
+    int dt_method;              // 0 euler, 1 rk4, 2 bs32, 3 tsit54
     int interactive_sealevel, include_elastic, heterogeneous_ssh;
     int uniform_K;               // K == 1 everywhere (dx_matrix scalar)
     int method;                  // 0 none, 3 LV-ELVA

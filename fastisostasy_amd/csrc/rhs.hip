@@ -136,9 +136,8 @@ __global__ void __launch_bounds__(256) k_stencil_u(const Member* __restrict__ me
     const int N = M.n;
     const int i0 = blockIdx.x * ST_TX, j0 = blockIdx.y * ST_TY;
     const double* __restrict__ w = c.w_is_x ? M.X[c.cur] : M.W;
-    const double dxs = ph.dx;
-    const double inv1 = 1.0 / dxs, inv2 = 1.0 / (2 * dxs), invq = 1.0 / (dxs * dxs);
-    constexpr int W0 = ST_TX + 4, H0 = ST_TY + 4;   // w:   [i0-2, i0+TX+2) x [j0-2, j0+TY+2)
+    const double inv1 = ph.inv_dx, inv2 = ph.inv_2dx, invq = ph.inv_dx2;
+    constexpr int W0 = ST_TX + 4, H0 = ST_TY + 4;  // w:   [i0-2, i0+TX+2) x [j0-2, j0+TY+2)
     constexpr int W2 = ST_TX + 2, H2 = ST_TY + 2;   // M**: [i0-1, i0+TX+1) x [j0-1, j0+TY+1)
     __shared__ double sw[H0 * W0];
     __shared__ double sMxx[H2 * W2], sMyy[H2 * W2], sMxy[H2 * W2];
@@ -376,6 +375,7 @@ struct Prof {
     int kernel = -1;
     std::vector<hipEvent_t> ev;   // start/stop pairs
     size_t used = 0;
+    bool fused = false;           // last attempt ran the fused stencil + row kernel
 };
 Prof g_prof;
 
@@ -394,6 +394,8 @@ void prof_select(int kernel) {
     g_prof.kernel = kernel;
     g_prof.used = 0;
 }
+
+int prof_fused() { return g_prof.fused ? 1 : 0; }
 
 int prof_collect(double* total_ms, long long* launches) {
     double tot = 0.0;
@@ -414,11 +416,16 @@ template <typename T>
 void launch_attempt_t(const Member* d_mem, int nmem, const RhsShape& s, hipStream_t st) {
     dim3 gs((s.n + ST_TX - 1) / ST_TX, (s.n + ST_TY - 1) / ST_TY, nmem);
     prof_mark(0, st);
-    if (s.uniform) hipLaunchKernelGGL(k_stencil_u<T>, gs, dim3(256), 0, st, d_mem);
-    else hipLaunchKernelGGL((k_stencil<T, false>), gs, dim3(256), 0, st, d_mem);
+    // uniform spacing, power-of-two N: stencil and forward row pass in one kernel
+    const bool fused = s.fused && s.fast && s.uniform && launch_fast_pass(d_mem, nmem, s, st, 0);
+    g_prof.fused = fused;
+    if (!fused) {
+        if (s.uniform) hipLaunchKernelGGL(k_stencil_u<T>, gs, dim3(256), 0, st, d_mem);
+        else hipLaunchKernelGGL((k_stencil<T, false>), gs, dim3(256), 0, st, d_mem);
+    }
     prof_mark(0, st);
     prof_mark(1, st);
-    if (!s.fast || !launch_fast_pass(d_mem, nmem, s, st, 1))
+    if (!fused && (!s.fast || !launch_fast_pass(d_mem, nmem, s, st, 1)))
         hipLaunchKernelGGL(k_row_fwd<T>, dim3(s.n / (2 * s.L), nmem), dim3(s.threads_row),
                            lds_bytes<T>(s.n, s.L), st, d_mem, s.plan, s.tw, s.L);
     prof_mark(1, st);

@@ -55,6 +55,194 @@ __global__ void __launch_bounds__(THREADS) kf_row_fwd(const Member* __restrict__
     }
 }
 
+// ------------------------------------------------------------------ fused stencil + row pass
+// Flexure stencil for uniform spacing (lv_elva.f90:60-108 with the one-sided
+// differences of isos_utils.f90 at the edges) fused with the forward row
+// transform, so f never leaves the CU.  The block owns 2L rows [R0, R0+2L);
+// thread `tid` owns columns tid + THREADS q.  All global inputs of the block
+// are loaded up front (w rows R0-2 .. R0+2L+1, D rows R0-1 .. R0+2L, canom_full
+// and eta rows R0 .. R0+2L-1), then the block marches down the rows:
+// iteration m (local row, -1 .. 2L+1) forms the moments of row m and f of row
+// m - 2.  x-neighbours go through LDS rings (w 3 rows, Mxx 3, Mxy 4); the
+// y-direction (w and Myy of the owned column) stays in registers.  Edge rows
+// pick the shifted windows the clamped centre rows of the reference select.
+//
+// Block order is XCD-aware: consecutive workgroup ids go to different XCDs,
+// so the row blocks of one member are mapped to ids that share an XCD and
+// its L2 holds the halo rows of the neighbouring blocks.
+template <typename T, int N, int THREADS> struct SRow {
+    static constexpr int TPL = N / 8, L = THREADS / TPL, ROWS = 2 * L;
+    static constexpr int CPT = N / THREADS;
+    static constexpr size_t ring_bytes = (size_t)(3 + 3 + 4) * N * sizeof(double);
+    static constexpr size_t stencil_bytes = ring_bytes + (size_t)ROWS * N * sizeof(T);
+    static constexpr size_t xch_bytes = (size_t)L * RowXch<T, N>::PITCH * sizeof(cplx<T>);
+    static constexpr size_t lds = (size_t)N * sizeof(cplx<T>) + (stencil_bytes > xch_bytes ? stencil_bytes : xch_bytes);
+};
+
+template <int B, int E, class F> __device__ __forceinline__ void static_for(F&& f) {
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(f);
+    }
+}
+
+// (member, block-within-member) for a grid (blocks per member, members)
+__device__ __forceinline__ void xcd_order(int& member, int& blk) {
+    const int per = gridDim.x, total = gridDim.x * gridDim.y;
+    int b = blockIdx.x + per * blockIdx.y;
+    if ((total & 7) == 0) b = (b & 7) * (total >> 3) + (b >> 3);
+    member = b / per;
+    blk = b % per;
+}
+
+template <typename T, int N, int THREADS>
+__global__ void __launch_bounds__(THREADS) kf_srow_fwd(const Member* __restrict__ mem, const void* __restrict__ twv) {
+    using SR = SRow<T, N, THREADS>;
+    constexpr int TPL = SR::TPL, E = P2Plan<N>::E, L = SR::L, ROWS = SR::ROWS, CPT = SR::CPT;
+    static_assert(CPT >= 1 && CPT * THREADS == N && ROWS >= 4, "fused stencil shape");
+    int mi, rblk;
+    xcd_order(mi, rblk);
+    const Member& M = mem[mi];
+    const Ctl& c = *M.ctl;
+    if (c.status != ST_RUN || c.k_valid) return;
+    const PhysDev& ph = *M.phys;
+    extern __shared__ __align__(16) unsigned char smem[];
+    cplx<T>* tw_l = (cplx<T>*)smem;
+    double* rw = (double*)(tw_l + N);     // w    rows, slot (r + 3) % 3
+    double* rxx = rw + 3 * N;             // Mxx  rows, slot (r + 3) % 3
+    double* rxy = rxx + 3 * N;            // Mxy  rows, slot (r + 4) % 4
+    T* sf = (T*)(rxy + 4 * N);            // f    [ROWS][N]
+    const int tid = threadIdx.x;
+    const int R0 = rblk * ROWS;
+    const bool top = R0 == 0, bottom = R0 + ROWS == N;
+    const double* __restrict__ w = c.w_is_x ? M.X[c.cur] : M.W;
+    const T* __restrict__ Dg = (const T*)M.D;
+    const T* __restrict__ Cg = (const T*)M.cfull;
+    const T* __restrict__ Eg = (const T*)M.eta;
+    auto crow = [](int r) { return r < 0 ? 0 : (r > N - 1 ? N - 1 : r); };
+    double wv[ROWS + 4][CPT];   // local rows -2 .. ROWS+1
+    T dv[ROWS + 2][CPT];        // -1 .. ROWS
+    T cv[ROWS][CPT], ev[ROWS][CPT];
+#pragma unroll
+    for (int r = 0; r < ROWS + 4; ++r)
+#pragma unroll
+        for (int q = 0; q < CPT; ++q) wv[r][q] = w[(size_t)crow(R0 - 2 + r) * N + tid + THREADS * q];
+#pragma unroll
+    for (int r = 0; r < ROWS + 2; ++r)
+#pragma unroll
+        for (int q = 0; q < CPT; ++q) dv[r][q] = Dg[(size_t)crow(R0 - 1 + r) * N + tid + THREADS * q];
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r)
+#pragma unroll
+        for (int q = 0; q < CPT; ++q) {
+            cv[r][q] = Cg[(size_t)(R0 + r) * N + tid + THREADS * q];
+            ev[r][q] = Eg[(size_t)(R0 + r) * N + tid + THREADS * q];
+        }
+    load_tw<T, N>(tw_l, (const cplx<T>*)twv);
+    const double invq = ph.inv_dx2, inv1 = ph.inv_dx, inv2 = ph.inv_2dx, nu = ph.nu, g = ph.g;
+    auto s3 = [](int r) { return (r + 3) % 3; };
+    auto s4 = [](int r) { return (r + 4) % 4; };
+#pragma unroll
+    for (int q = 0; q < CPT; ++q) {
+        rw[s3(-2) * N + tid + THREADS * q] = wv[0][q];
+        rw[s3(-1) * N + tid + THREADS * q] = wv[1][q];
+    }
+    double my[ROWS + 2][CPT];   // Myy of local rows -1 .. ROWS
+    static_for<-1, ROWS + 2>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        if constexpr (m <= ROWS) {
+            // moments of local row m (global gm); w rows m-1 .. m+1 in the ring
+            const int gm = R0 + m;
+#pragma unroll
+            for (int q = 0; q < CPT; ++q) rw[s3(m + 1) * N + tid + THREADS * q] = wv[m + 3][q];
+            __syncthreads();
+            const int jlo = (gm - 1 < 0 ? 0 : gm - 1) - R0, jhi = (gm + 1 > N - 1 ? N - 1 : gm + 1) - R0;
+            const double sy = (jhi - jlo == 2) ? inv2 : inv1;
+#pragma unroll
+            for (int q = 0; q < CPT; ++q) {
+                const int i = tid + THREADS * q;
+                const int ii = i < 1 ? 1 : (i > N - 2 ? N - 2 : i);
+                const int ilo = i - 1 < 0 ? 0 : i - 1, ihi = i + 1 > N - 1 ? N - 1 : i + 1;
+                const double sx = (ihi - ilo == 2) ? inv2 : inv1;
+                const double* wr = rw + s3(m) * N;
+                const double wxx = (wr[ii + 1] - 2 * wr[ii] + wr[ii - 1]) * invq;
+                // centre row of the y-difference: clamp(gm, 1, N-2)
+                double wa = wv[m + 3][q], wb = wv[m + 2][q], wc = wv[m + 1][q];
+                if constexpr (m <= 0) {
+                    if (top) { wa = wv[2 + 2][q]; wb = wv[1 + 2][q]; wc = wv[0 + 2][q]; }
+                }
+                if constexpr (m >= ROWS - 1) {
+                    if (bottom) { wa = wv[ROWS - 1 + 2][q]; wb = wv[ROWS - 2 + 2][q]; wc = wv[ROWS - 3 + 2][q]; }
+                }
+                const double wyy = (wa - 2 * wb + wc) * invq;
+                const double* wh = rw + s3(jhi) * N;
+                const double* wl = rw + s3(jlo) * N;
+                const double wx_hi = (wh[ihi] - wh[ilo]) * sx;
+                const double wx_lo = (wl[ihi] - wl[ilo]) * sx;
+                const double wxy = (wx_hi - wx_lo) * sy;
+                const double D = (double)dv[m + 1][q];
+                rxx[s3(m) * N + i] = -(D * (wxx + nu * wyy));
+                my[m + 1][q] = -(D * (wyy + nu * wxx));
+                rxy[s4(m) * N + i] = -(D * (1.0 - nu) * wxy);
+            }
+            __syncthreads();
+        }
+        if constexpr (m - 2 >= 0) {
+            // f of local row j = m - 2: moments of rows j-1 .. j+1 are complete
+            constexpr int j = m - 2;
+            const int gj = R0 + j;
+            const int jlo = (gj - 1 < 0 ? 0 : gj - 1) - R0, jhi = (gj + 1 > N - 1 ? N - 1 : gj + 1) - R0;
+            const double sy = (jhi - jlo == 2) ? inv2 : inv1;
+#pragma unroll
+            for (int q = 0; q < CPT; ++q) {
+                const int i = tid + THREADS * q;
+                const int ii = i < 1 ? 1 : (i > N - 2 ? N - 2 : i);
+                const int ilo = i - 1 < 0 ? 0 : i - 1, ihi = i + 1 > N - 1 ? N - 1 : i + 1;
+                const double sx = (ihi - ilo == 2) ? inv2 : inv1;
+                const double* xr = rxx + s3(j) * N;
+                const double mxx = (xr[ii + 1] - 2 * xr[ii] + xr[ii - 1]) * invq;
+                double ya = my[j + 2][q], yb = my[j + 1][q], yc = my[j][q];
+                if constexpr (j == 0) {
+                    if (top) { ya = my[2 + 1][q]; yb = my[1 + 1][q]; yc = my[0 + 1][q]; }
+                }
+                if constexpr (j == ROWS - 1) {
+                    if (bottom) { ya = my[ROWS - 1 + 1][q]; yb = my[ROWS - 2 + 1][q]; yc = my[ROWS - 3 + 1][q]; }
+                }
+                const double myy = (ya - 2 * yb + yc) * invq;
+                const double* hr = rxy + s4(jhi) * N;
+                const double* lr = rxy + s4(jlo) * N;
+                const double mxyx_hi = (hr[ihi] - hr[ilo]) * sx;
+                const double mxyx_lo = (lr[ihi] - lr[ilo]) * sx;
+                const double mxyxy = (mxyx_hi - mxyx_lo) * sy;
+                const double p = -(g * (double)cv[j][q]);
+                const double num = p + mxx + 2.0 * mxyxy + myy;
+                T f;
+                if constexpr (sizeof(T) == 8) f = (T)(num / (2.0 * (double)ev[j][q]));
+                else f = (T)(num * (double)ev[j][q]);   // mixed: eta holds 1/(2 eta)
+                sf[j * N + i] = f;
+            }
+        }
+    });
+    __syncthreads();
+    const int line = tid / TPL, t = tid % TPL;
+    cplx<T> v[E], mr[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = mk<T>(sf[line * N + t + TPL * e], sf[(L + line) * N + t + TPL * e]);
+    __syncthreads();
+    RowXch<T, N> xc((cplx<T>*)(tw_l + N), line);
+    reg_fft<T, N>(v, t, tw_l, xc);
+    reg_mirror<T, N>(v, mr, t, xc);
+    T* __restrict__ dst = (T*)M.g2;
+    const int ra = R0 + line, rb = ra + L;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        T ha, hb;
+        dht_pair(v[e], mr[e], ha, hb);
+        dst[(size_t)ra * N + t + TPL * e] = ha;
+        dst[(size_t)rb * N + t + TPL * e] = hb;
+    }
+}
+
 template <typename T, int N, int C>
 __global__ void __launch_bounds__(C * (N / 8)) kf_col(const Member* __restrict__ mem, const void* __restrict__ twv,
                                                      const double2* __restrict__ tw64) {
@@ -212,6 +400,15 @@ static void rows_fwd(const Member* d, int nmem, hipStream_t st, const void* tw) 
     kf_row_fwd<T, N, THREADS><<<dim3(N / (2 * L), nmem), dim3(THREADS), lds, st>>>(d, tw);
 }
 template <typename T, int N, int THREADS>
+static bool srow_fwd(const Member* d, int nmem, hipStream_t st, const void* tw) {
+    using SR = SRow<T, N, THREADS>;
+    if constexpr (SR::CPT >= 1 && SR::CPT * THREADS == N && SR::ROWS >= 4 && SR::lds <= 65536) {
+        kf_srow_fwd<T, N, THREADS><<<dim3(N / SR::ROWS, nmem), dim3(THREADS), SR::lds, st>>>(d, tw);
+        return true;
+    }
+    return false;
+}
+template <typename T, int N, int THREADS>
 static void rows_inv(const Member* d, int nmem, hipStream_t st, const void* tw, int ncolblk) {
     constexpr int L = THREADS / (N / 8);
     const size_t lds = row_lds<T, N>(THREADS);
@@ -225,7 +422,7 @@ static void cols(const Member* d, int nmem, hipStream_t st, const void* tw, cons
 
 template <typename T, int N>
 static bool fast_n(const Member* d, int nmem, const RhsShape& s, hipStream_t st, int which) {
-    // which: 1 row forward, 2 column, 3 row inverse
+    // which: 0 fused stencil + row forward, 1 row forward, 2 column, 3 row inverse
     if (which == 2) {
         switch (s.C) {
         case 1: cols<T, N, 1>(d, nmem, st, s.tw, s.tw64); return true;
@@ -237,6 +434,14 @@ static bool fast_n(const Member* d, int nmem, const RhsShape& s, hipStream_t st,
         return false;
     }
     const int ncolblk = s.n / (2 * s.C);
+    if (which == 0) {
+        switch (s.threads_row) {
+        case 64: if constexpr (N / 8 <= 64) return srow_fwd<T, N, 64>(d, nmem, st, s.tw); break;
+        case 128: if constexpr (N / 8 <= 128) return srow_fwd<T, N, 128>(d, nmem, st, s.tw); break;
+        case 256: if constexpr (N / 8 <= 256) return srow_fwd<T, N, 256>(d, nmem, st, s.tw); break;
+        }
+        return false;
+    }
     switch (s.threads_row) {
     case 64: if constexpr (N / 8 <= 64) { which == 1 ? rows_fwd<T, N, 64>(d, nmem, st, s.tw) : rows_inv<T, N, 64>(d, nmem, st, s.tw, ncolblk); return true; } break;
     case 128: if constexpr (N / 8 <= 128) { which == 1 ? rows_fwd<T, N, 128>(d, nmem, st, s.tw) : rows_inv<T, N, 128>(d, nmem, st, s.tw, ncolblk); return true; } break;

@@ -152,6 +152,9 @@ int fi_sync(void);
  * the last fi_prof_select. */
 int fi_prof_select(int kernel);
 int fi_prof_collect(double* total_ms, long long* launches);
+/* 1 when the last attempt ran the stencil fused into the forward row pass
+ * (uniform spacing, power-of-two N); kernel 0 then times the fused kernel. */
+int fi_prof_fused(void);
 
 #ifdef __cplusplus
 }

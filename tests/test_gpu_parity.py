@@ -156,3 +156,51 @@ def test_sealevel_masks_bsl_fp64():
     assert np.abs(o.get("dz_ss")).max() > 1e-3
     compare(o, g, wtol=1e-6)
     assert abs(o.scalar("bsl") - g.scalar("bsl")) < 1e-9
+
+
+
+def _run_gpu(wl, precision, times, fused, threads, monkeypatch):
+    from fastisostasy_amd._lib import load
+    monkeypatch.setenv("FI_ROW_THREADS", str(threads))
+    if fused:
+        monkeypatch.delenv("FI_NO_FUSED_STENCIL", raising=False)
+    else:
+        monkeypatch.setenv("FI_NO_FUSED_STENCIL", "1")
+    g = G.IsosClass(G.make_params(precision=precision, **wl["params"]), wl["nx"], wl["nx"], wl["dx"],
+                    he_field=wl.get("he_field"), eta_field=wl.get("eta_field"))
+    b = G.bsl_init("const", 0.0)
+    g.init_state(wl["z_bed"], W.step_on_load(wl["H_ice"], times[0]), times[0], b)
+    for t in times[1:]:
+        G.bsl_update(b, t)
+        g.update(W.step_on_load(wl["H_ice"], t), t, b)
+    out = {f: g.full(f) for f in ("w", "dwdt")}
+    out["attempts"] = g.scalar("attempts")
+    out["fused"] = bool(load().fi_prof_fused())
+    g.close()
+    return out
+
+
+@pytest.mark.parametrize("case,prec,threads", [
+    ("disc64", "fp64", 64),
+    ("ens256", "fp64", 64), ("ens256", "fp64", 128), ("ens256", "fp64", 256),
+    ("ens256", "mixed", 64), ("ens256", "mixed", 128), ("ens256", "mixed", 256),
+    ("lv512", "mixed", 128), ("lv512", "mixed", 256)])
+def test_fused_stencil_matches_separate_kernels(case, prec, threads, monkeypatch):
+    """The stencil fused into the forward row pass (uniform spacing,
+    power-of-two N) reproduces the separate stencil and row kernels bit for
+    bit -- same arithmetic, same rounding -- on every row block, the top and
+    bottom edge blocks included, for 1, 2 and 4 columns per thread."""
+    if case == "disc64":
+        wl, times = small_disc(), [0.0, 10.0, 20.0, 30.0]
+    elif case == "ens256":
+        wl, times = W.ensemble_member(3), [0.0, 10.0, 20.0]
+    else:
+        wl, times = W.lv_rheology(), [0.0, 0.5, 1.0]
+    print(f"fused vs separate: {case} {prec} threads={threads}", flush=True)
+    a = _run_gpu(wl, prec, times, True, threads, monkeypatch)
+    b = _run_gpu(wl, prec, times, False, threads, monkeypatch)
+    assert a["fused"] and not b["fused"]
+    assert a["attempts"] == b["attempts"]
+    for f in ("w", "dwdt"):
+        assert np.array_equal(a[f], b[f]), (case, f, np.abs(a[f] - b[f]).max())
+    assert np.abs(a["w"]).max() > 0
