@@ -566,6 +566,8 @@ int fi_isos_create(const fi_params* p, int nx_ice, int ny_ice, double dx, double
     M.D = a + oD; M.eta = a + oeta; M.kR = a + okR; M.cfull = a + ocfr; M.g1 = a + og1; M.g2 = a + og2;
     M.corners = (double*)(a + ocor); M.partials = (double*)(a + opart);
     M.ctl = (Ctl*)(a + octl); M.phys = (PhysDev*)(a + ophys);
+    M.bsl_lead = 1;
+    M.bsl_next = -1;
     h->d_mem = (Member*)(a + omem);
     h->eta_eff = (double*)(a + oee); h->kappa = (double*)(a + okap); h->R = (double*)(a + oR);
     h->He = (double*)(a + oHe); h->Dd = (double*)(a + oDd);
@@ -919,6 +921,20 @@ int fi_batch_update(fi_batch* b, const double* const* H_ice, double time, fi_bsl
         if (H_ice && H_ice[m])
             HIPCHK(hipMemcpyAsync(h->hm.H_in, H_ice[m], sizeof(double) * h->nx_ice * h->ny_ice,
                                   hipMemcpyHostToDevice, g_stream));
+    }
+    // groups of members sharing a bsl object, in batch order
+    for (int m = 0; m < nmem; ++m) {
+        int prev = -1, next = -1;
+        for (int q = m - 1; q >= 0 && prev < 0; --q)
+            if (b->hm[q].bsl == b->hm[m].bsl) prev = q;
+        for (int q = m + 1; q < nmem && next < 0; ++q)
+            if (b->hm[q].bsl == b->hm[m].bsl) next = q;
+        const int lead = prev < 0 ? 1 : 0;
+        if (b->hm[m].bsl_lead != lead || b->hm[m].bsl_next != next) {
+            b->hm[m].bsl_lead = lead;
+            b->hm[m].bsl_next = next;
+            dirty = true;
+        }
     }
     if (dirty) HIPCHK(hipMemcpy(b->d_mem, b->hm.data(), sizeof(Member) * nmem, hipMemcpyHostToDevice));
     return run_update(b->mem, b->d_mem, b->shape, b->sb, time, true);

@@ -1,5 +1,6 @@
 // fi_kernels.h -- host-callable launchers of the device kernels.
 #pragma once
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include "fi_types.h"
@@ -38,6 +39,21 @@ bool launch_fast_pass(const Member* d_mem, int nmem, const RhsShape& s, hipStrea
 void prof_select(int kernel);
 int prof_collect(double* total_ms, long long* launches);
 int prof_fused();
+// a start/stop event pair when launches of `kernel` are being timed
+bool prof_pair(int kernel, hipEvent_t* start, hipEvent_t* stop);
+
+// Launch of an attempt kernel.  When its launches are being timed, the events
+// ride on the dispatch itself (hipExtLaunchKernel), so the timing adds no
+// packets to the stream and measures the kernel alone.
+template <typename... P, typename... A>
+inline void launch_k(int prof_id, void (*kern)(P...), dim3 grid, dim3 block, size_t lds, hipStream_t st,
+                     A... args) {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (prof_pair(prof_id, &e0, &e1))
+        hipExtLaunchKernelGGL(kern, grid, block, (std::uint32_t)lds, st, e0, e1, 0u, args...);
+    else
+        kern<<<grid, block, lds, st>>>(args...);
+}
 
 // bookkeeping of isos_update (sealevel.f90, fastisostasy.f90:750-833)
 void launch_pre_haf(const Member* d_mem, int nmem, int n, bool new_H, hipStream_t st);

@@ -99,6 +99,9 @@ struct Member {
     Ctl* ctl;
     BslDev* bsl;
     PhysDev* phys;
+    // domains sharing one bsl object: the first in batch order leads and
+    // applies the group's updates in order (next: batch index, -1 at the end)
+    int bsl_lead, bsl_next;
 };
 
 }  // namespace fi

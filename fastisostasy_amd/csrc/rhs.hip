@@ -379,16 +379,19 @@ struct Prof {
 };
 Prof g_prof;
 
-inline void prof_mark(int kernel, hipStream_t st) {
-    if (g_prof.kernel != kernel) return;
-    if (g_prof.used == g_prof.ev.size()) {
+}  // namespace
+
+bool prof_pair(int kernel, hipEvent_t* start, hipEvent_t* stop) {
+    if (g_prof.kernel < 0 || g_prof.kernel != kernel) return false;
+    while (g_prof.used + 2 > g_prof.ev.size()) {
         hipEvent_t e;
-        if (hipEventCreate(&e) != hipSuccess) return;
+        if (hipEventCreate(&e) != hipSuccess) return false;
         g_prof.ev.push_back(e);
     }
-    (void)hipEventRecord(g_prof.ev[g_prof.used++], st);
+    *start = g_prof.ev[g_prof.used++];
+    *stop = g_prof.ev[g_prof.used++];
+    return true;
 }
-}  // namespace
 
 void prof_select(int kernel) {
     g_prof.kernel = kernel;
@@ -415,30 +418,22 @@ int prof_collect(double* total_ms, long long* launches) {
 template <typename T>
 void launch_attempt_t(const Member* d_mem, int nmem, const RhsShape& s, hipStream_t st) {
     dim3 gs((s.n + ST_TX - 1) / ST_TX, (s.n + ST_TY - 1) / ST_TY, nmem);
-    prof_mark(0, st);
     // uniform spacing, power-of-two N: stencil and forward row pass in one kernel
     const bool fused = s.fused && s.fast && s.uniform && launch_fast_pass(d_mem, nmem, s, st, 0);
     g_prof.fused = fused;
     if (!fused) {
-        if (s.uniform) hipLaunchKernelGGL(k_stencil_u<T>, gs, dim3(256), 0, st, d_mem);
-        else hipLaunchKernelGGL((k_stencil<T, false>), gs, dim3(256), 0, st, d_mem);
+        if (s.uniform) launch_k(0, k_stencil_u<T>, gs, dim3(256), 0, st, d_mem);
+        else launch_k(0, k_stencil<T, false>, gs, dim3(256), 0, st, d_mem);
     }
-    prof_mark(0, st);
-    prof_mark(1, st);
     if (!fused && (!s.fast || !launch_fast_pass(d_mem, nmem, s, st, 1)))
-        hipLaunchKernelGGL(k_row_fwd<T>, dim3(s.n / (2 * s.L), nmem), dim3(s.threads_row),
-                           lds_bytes<T>(s.n, s.L), st, d_mem, s.plan, s.tw, s.L);
-    prof_mark(1, st);
-    prof_mark(2, st);
+        launch_k(1, k_row_fwd<T>, dim3(s.n / (2 * s.L), nmem), dim3(s.threads_row), lds_bytes<T>(s.n, s.L), st,
+                 d_mem, s.plan, (const void*)s.tw, s.L);
     if (!s.fast || !launch_fast_pass(d_mem, nmem, s, st, 2))
-        hipLaunchKernelGGL(k_col<T>, dim3(s.n / (2 * s.C), nmem), dim3(s.threads_col),
-                           lds_bytes<T>(s.n, s.C), st, d_mem, s.plan, s.tw, s.tw64, s.C);
-    prof_mark(2, st);
-    prof_mark(3, st);
+        launch_k(2, k_col<T>, dim3(s.n / (2 * s.C), nmem), dim3(s.threads_col), lds_bytes<T>(s.n, s.C), st,
+                 d_mem, s.plan, (const void*)s.tw, s.tw64, s.C);
     if (!s.fast || !launch_fast_pass(d_mem, nmem, s, st, 3))
-        hipLaunchKernelGGL(k_row_inv<T>, dim3(s.n / (2 * s.L), nmem), dim3(s.threads_row),
-                           lds_bytes<T>(s.n, s.L), st, d_mem, s.plan, s.tw, s.L, s.n / (2 * s.C));
-    prof_mark(3, st);
+        launch_k(3, k_row_inv<T>, dim3(s.n / (2 * s.L), nmem), dim3(s.threads_row), lds_bytes<T>(s.n, s.L), st,
+                 d_mem, s.plan, (const void*)s.tw, s.L, s.n / (2 * s.C));
     hipLaunchKernelGGL(k_decide, dim3(nmem), dim3(64), 0, st, d_mem, s.n / (2 * s.L));
 }
 

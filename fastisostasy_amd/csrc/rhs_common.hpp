@@ -6,17 +6,35 @@
 namespace fi {
 
 // ------------------------------------------------------------------ helpers
-__device__ __forceinline__ double attempt_dt(const Ctl& c, const PhysDev& ph) {
+__device__ __forceinline__ double attempt_dt(int method, double dt, double t, double tf, double dt_max) {
     // integrators.f90:63,100-101,165
-    double dt = c.dt;
-    const double rem = c.tf - c.t;
-    if (ph.dt_method == 1) return fmin(dt, rem);  // rk4: h, not persisted
+    const double rem = tf - t;
+    if (method == 1) return fmin(dt, rem);  // rk4: h, not persisted
     dt = fmin(dt, rem);
-    if (ph.dt_method == 2) dt = fmin(dt, ph.dt_max);
+    if (method == 2) dt = fmin(dt, dt_max);
     return dt;
 }
+__device__ __forceinline__ double attempt_dt(const Ctl& c, const PhysDev& ph) {
+    return attempt_dt(ph.dt_method, c.dt, c.t, c.tf, ph.dt_max);
+}
+
+// Scalar-cache views of the control block and the physics scalars.  Both are
+// read-only while an attempt pass runs (k_decide, a separate launch, writes
+// Ctl), so their fields load with SMEM into SGPRs rather than with one vector
+// load and wait per field.
+template <typename T> using kptr = const __attribute__((address_space(4))) T*;
+template <typename T> __device__ __forceinline__ kptr<T> kview(const T* p) { return (kptr<T>)p; }
 
 template <typename T> __device__ __forceinline__ T ld(const void* p, size_t i) { return ((const T*)p)[i]; }
+
+// Global-address-space views of the field pointers held in Member: accesses
+// through them compile to global_* instructions, which count only against
+// vmcnt.  Flat accesses also count against lgkmcnt, so every LDS wait after
+// them would wait for all outstanding global loads as well.
+template <typename T> using gptr = const __attribute__((address_space(1))) T*;
+template <typename T> using gmut = __attribute__((address_space(1))) T*;
+template <typename T> __device__ __forceinline__ gptr<T> gin(const void* p) { return (gptr<T>)p; }
+template <typename T> __device__ __forceinline__ gmut<T> gout(void* p) { return (gmut<T>)p; }
 
 // ------------------------------------------------------------------ epilogue
 struct EpiOut { double x1max, x2max, d2; };

@@ -20,14 +20,16 @@ namespace fi {
 
 template <typename T, int N>
 __device__ __forceinline__ void load_tw(cplx<T>* tw_l, const cplx<T>* __restrict__ twg) {
-    for (int q = threadIdx.x; q < N; q += blockDim.x) tw_l[q] = twg[q];
+    gptr<T> g = gin<T>(twg);
+    T* l = (T*)tw_l;
+    for (int q = threadIdx.x; q < 2 * N; q += blockDim.x) l[q] = g[q];
 }
 
 template <typename T, int N, int THREADS>
 __global__ void __launch_bounds__(THREADS) kf_row_fwd(const Member* __restrict__ mem, const void* __restrict__ twv) {
     const Member& M = mem[blockIdx.y];
-    const Ctl& c = *M.ctl;
-    if (c.status != ST_RUN || c.k_valid) return;
+    const kptr<Ctl> c = kview(M.ctl);
+    if ((c->status != ST_RUN) | (c->k_valid != 0)) return;
     using PL = P2Plan<N>;
     constexpr int TPL = PL::TPL, E = PL::E, L = THREADS / TPL;
     extern __shared__ __align__(16) unsigned char smem[];
@@ -36,7 +38,7 @@ __global__ void __launch_bounds__(THREADS) kf_row_fwd(const Member* __restrict__
     const int tid = threadIdx.x, line = tid / TPL, t = tid % TPL;
     load_tw<T, N>(tw_l, (const cplx<T>*)twv);
     const int ra = blockIdx.x * 2 * L + line, rb = ra + L;
-    const T* __restrict__ src = (const T*)M.g1;
+    gptr<T> src = gin<T>(M.g1);
     cplx<T> v[E], m[E];
 #pragma unroll
     for (int e = 0; e < E; ++e)
@@ -103,9 +105,9 @@ __global__ void __launch_bounds__(THREADS) kf_srow_fwd(const Member* __restrict_
     int mi, rblk;
     xcd_order(mi, rblk);
     const Member& M = mem[mi];
-    const Ctl& c = *M.ctl;
-    if (c.status != ST_RUN || c.k_valid) return;
-    const PhysDev& ph = *M.phys;
+    const kptr<Ctl> c = kview(M.ctl);
+    if ((c->status != ST_RUN) | (c->k_valid != 0)) return;
+    const kptr<PhysDev> ph = kview(M.phys);
     extern __shared__ __align__(16) unsigned char smem[];
     cplx<T>* tw_l = (cplx<T>*)smem;
     double* rw = (double*)(tw_l + N);     // w    rows, slot (r + 3) % 3
@@ -115,31 +117,45 @@ __global__ void __launch_bounds__(THREADS) kf_srow_fwd(const Member* __restrict_
     const int tid = threadIdx.x;
     const int R0 = rblk * ROWS;
     const bool top = R0 == 0, bottom = R0 + ROWS == N;
-    const double* __restrict__ w = c.w_is_x ? M.X[c.cur] : M.W;
-    const T* __restrict__ Dg = (const T*)M.D;
-    const T* __restrict__ Cg = (const T*)M.cfull;
-    const T* __restrict__ Eg = (const T*)M.eta;
+    gptr<double> w = gin<double>(c->w_is_x ? M.X[c->cur] : M.W);
+    gptr<T> Dg = gin<T>(M.D), Cg = gin<T>(M.cfull), Eg = gin<T>(M.eta);
     auto crow = [](int r) { return r < 0 ? 0 : (r > N - 1 ? N - 1 : r); };
+    // twiddles first: their LDS store then waits only for their own loads
+    constexpr int TWQ = (2 * N + THREADS - 1) / THREADS;
+    T twr[TWQ];
+    {
+        gptr<T> g = gin<T>(twv);
+#pragma unroll
+        for (int k = 0; k < TWQ; ++k)
+            if (tid + THREADS * k < 2 * N) twr[k] = g[tid + THREADS * k];
+    }
     double wv[ROWS + 4][CPT];   // local rows -2 .. ROWS+1
     T dv[ROWS + 2][CPT];        // -1 .. ROWS
     T cv[ROWS][CPT], ev[ROWS][CPT];
+    // loads in the order the march consumes them
 #pragma unroll
-    for (int r = 0; r < ROWS + 4; ++r)
+    for (int q = 0; q < CPT; ++q) {
+        wv[0][q] = w[(size_t)crow(R0 - 2) * N + tid + THREADS * q];
+        wv[1][q] = w[(size_t)crow(R0 - 1) * N + tid + THREADS * q];
+    }
 #pragma unroll
-        for (int q = 0; q < CPT; ++q) wv[r][q] = w[(size_t)crow(R0 - 2 + r) * N + tid + THREADS * q];
-#pragma unroll
-    for (int r = 0; r < ROWS + 2; ++r)
-#pragma unroll
-        for (int q = 0; q < CPT; ++q) dv[r][q] = Dg[(size_t)crow(R0 - 1 + r) * N + tid + THREADS * q];
-#pragma unroll
-    for (int r = 0; r < ROWS; ++r)
+    for (int m = -1; m <= ROWS + 1; ++m)
 #pragma unroll
         for (int q = 0; q < CPT; ++q) {
-            cv[r][q] = Cg[(size_t)(R0 + r) * N + tid + THREADS * q];
-            ev[r][q] = Eg[(size_t)(R0 + r) * N + tid + THREADS * q];
+            const int col = tid + THREADS * q;
+            if (m <= ROWS) {
+                wv[m + 3][q] = w[(size_t)crow(R0 + m + 1) * N + col];
+                dv[m + 1][q] = Dg[(size_t)crow(R0 + m) * N + col];
+            }
+            if (m >= 2) {
+                cv[m - 2][q] = Cg[(size_t)(R0 + m - 2) * N + col];
+                ev[m - 2][q] = Eg[(size_t)(R0 + m - 2) * N + col];
+            }
         }
-    load_tw<T, N>(tw_l, (const cplx<T>*)twv);
-    const double invq = ph.inv_dx2, inv1 = ph.inv_dx, inv2 = ph.inv_2dx, nu = ph.nu, g = ph.g;
+#pragma unroll
+    for (int k = 0; k < TWQ; ++k)
+        if (tid + THREADS * k < 2 * N) ((T*)tw_l)[tid + THREADS * k] = twr[k];
+    const double invq = ph->inv_dx2, inv1 = ph->inv_dx, inv2 = ph->inv_2dx, nu = ph->nu, g = ph->g;
     auto s3 = [](int r) { return (r + 3) % 3; };
     auto s4 = [](int r) { return (r + 4) % 4; };
 #pragma unroll
@@ -232,7 +248,7 @@ __global__ void __launch_bounds__(THREADS) kf_srow_fwd(const Member* __restrict_
     RowXch<T, N> xc((cplx<T>*)(tw_l + N), line);
     reg_fft<T, N>(v, t, tw_l, xc);
     reg_mirror<T, N>(v, mr, t, xc);
-    T* __restrict__ dst = (T*)M.g2;
+    gmut<T> dst = gout<T>(M.g2);
     const int ra = R0 + line, rb = ra + L;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -247,8 +263,8 @@ template <typename T, int N, int C>
 __global__ void __launch_bounds__(C * (N / 8)) kf_col(const Member* __restrict__ mem, const void* __restrict__ twv,
                                                      const double2* __restrict__ tw64) {
     const Member& M = mem[blockIdx.y];
-    const Ctl& c = *M.ctl;
-    if (c.status != ST_RUN || c.k_valid) return;
+    const kptr<Ctl> c = kview(M.ctl);
+    if ((c->status != ST_RUN) | (c->k_valid != 0)) return;
     using PL = P2Plan<N>;
     constexpr int TPL = PL::TPL, E = PL::E;
     extern __shared__ __align__(16) unsigned char smem[];
@@ -259,8 +275,8 @@ __global__ void __launch_bounds__(C * (N / 8)) kf_col(const Member* __restrict__
     load_tw<T, N>(tw_l, (const cplx<T>*)twv);
     const int c0 = blockIdx.x * 2 * C;
     const int ca = c0 + line, cb = c0 + C + line;
-    const T* __restrict__ src = (const T*)M.g2;
-    const T* __restrict__ kR = (const T*)M.kR;
+    gptr<T> src = gin<T>(M.g2);
+    gptr<T> kR = gin<T>(M.kR);
     cplx<T> v[E], m[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -310,63 +326,94 @@ template <typename T, int N, int THREADS>
 __global__ void __launch_bounds__(THREADS) kf_row_inv(const Member* __restrict__ mem, const void* __restrict__ twv,
                                                       int ncolblk) {
     const Member& M = mem[blockIdx.y];
-    const Ctl& c = *M.ctl;
-    if (c.status != ST_RUN) return;
-    const PhysDev& ph = *M.phys;
+    const kptr<Ctl> c = kview(M.ctl);
+    if (c->status != ST_RUN) return;
+    const kptr<PhysDev> ph = kview(M.phys);
     using PL = P2Plan<N>;
     constexpr int TPL = PL::TPL, E = PL::E, L = THREADS / TPL;
     extern __shared__ __align__(16) unsigned char smem[];
     cplx<T>* tw_l = (cplx<T>*)smem;
     cplx<T>* buf = tw_l + N;
-    __shared__ double corner_c;
     const int tid = threadIdx.x, line = tid / TPL, t = tid % TPL;
     const int ra = blockIdx.x * 2 * L + line, rb = ra + L;
-    const double dt = attempt_dt(c, ph);
-    const int meth = ph.dt_method;
-    const double* __restrict__ X = M.X[c.cur];
-    double* __restrict__ Xn = M.X[c.cur ^ 1];
-    double* __restrict__ dw = M.dwdt;
+    const int meth = ph->dt_method;
+    const double dt = attempt_dt(meth, c->dt, c->t, c->tf, ph->dt_max);
+    const int cur = c->cur;
+    const bool k_valid = c->k_valid != 0;
+    gptr<double> X = gin<double>(M.X[cur]);
+    gmut<double> Xn = gout<double>(M.X[cur ^ 1]);
+    gmut<double> dw = gout<double>(M.dwdt);
     double a1 = 0.0, a2 = 0.0, s2 = 0.0;
-    auto update = [&](size_t cidx, double k) {
+    auto update = [&](size_t cidx, double x, double k) {
         double x1;
-        const double x2 = stage_sums(meth, X[cidx], k, dt, x1);
+        const double x2 = stage_sums(meth, x, k, dt, x1);
         Xn[cidx] = x2;
         a1 = fmax(a1, fabs(x1));
         a2 = fmax(a2, fabs(x2));
         const double d = x2 - x1;
         s2 += d * d;
     };
-    if (c.k_valid) {
+    // All loads are issued before the first wait, in the order of use; X is
+    // independent of the transform and is consumed last.
+    double xa[E], xb[E];
+    if (k_valid) {
         // rejected attempt before: the RHS is unchanged, only dt differs
+        double ka[E], kb[E];
 #pragma unroll
         for (int e = 0; e < E; ++e) {
-            const size_t ca = (size_t)ra * N + t + TPL * e, cb = (size_t)rb * N + t + TPL * e;
-            update(ca, dw[ca]);
-            update(cb, dw[cb]);
+            ka[e] = dw[(size_t)ra * N + t + TPL * e];
+            kb[e] = dw[(size_t)rb * N + t + TPL * e];
+        }
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            xa[e] = X[(size_t)ra * N + t + TPL * e];
+            xb[e] = X[(size_t)rb * N + t + TPL * e];
+        }
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            update((size_t)ra * N + t + TPL * e, xa[e], ka[e]);
+            update((size_t)rb * N + t + TPL * e, xb[e], kb[e]);
         }
     } else {
-        load_tw<T, N>(tw_l, (const cplx<T>*)twv);
+        __shared__ double qs[4];
         const double S = (double)(float)((double)N * (double)N);
-        if (tid == 0) {
-            double q0 = 0, q1 = 0, q2 = 0, q3 = 0;
-            for (int b = 0; b < ncolblk; ++b) {
-                q0 += M.corners[(size_t)b * 4 + 0]; q1 += M.corners[(size_t)b * 4 + 1];
-                q2 += M.corners[(size_t)b * 4 + 2]; q3 += M.corners[(size_t)b * 4 + 3];
-            }
-            // x(1,1) + x(nx,ny) + x(1,ny) + x(nx,1)  (isos_utils.f90:116)
-            corner_c = 0.25 * (q0 / S + q3 / S + q2 / S + q1 / S);
-        }
-        const T* __restrict__ src = (const T*)M.g1;
+        gptr<T> src = gin<T>(M.g1);
         cplx<T> v[E], m[E];
 #pragma unroll
         for (int e = 0; e < E; ++e)
             v[e] = mk<T>(src[(size_t)ra * N + t + TPL * e], src[(size_t)rb * N + t + TPL * e]);
+        constexpr int TWQ = (2 * N + THREADS - 1) / THREADS;
+        T twr[TWQ];
+        {
+            gptr<T> g = gin<T>(twv);
+#pragma unroll
+            for (int k = 0; k < TWQ; ++k)
+                if (tid + THREADS * k < 2 * N) twr[k] = g[tid + THREADS * k];
+        }
+        if (tid < 4) {
+            // corner partial sums of the column pass, one corner per thread
+            gptr<double> cp = gin<double>(M.corners);
+            double q = 0;
+#pragma unroll 8
+            for (int b = 0; b < ncolblk; ++b) q += cp[(size_t)b * 4 + tid];
+            qs[tid] = q;
+        }
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            xa[e] = X[(size_t)ra * N + t + TPL * e];
+            xb[e] = X[(size_t)rb * N + t + TPL * e];
+        }
+#pragma unroll
+        for (int k = 0; k < TWQ; ++k)
+            if (tid + THREADS * k < 2 * N) ((T*)tw_l)[tid + THREADS * k] = twr[k];
         __syncthreads();
         RowXch<T, N> xc(buf, line);
         reg_fft<T, N>(v, t, tw_l, xc);
         reg_mirror<T, N>(v, m, t, xc);
-        const double cc = corner_c, spy = ph.sec_per_year;
-        const bool store_k = !(c.t + dt < c.tf);
+        // x(1,1) + x(nx,ny) + x(1,ny) + x(nx,1)  (isos_utils.f90:116)
+        const double cc = 0.25 * (qs[0] / S + qs[3] / S + qs[2] / S + qs[1] / S);
+        const double spy = ph->sec_per_year;
+        const bool store_k = !(c->t + dt < c->tf);
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             T ha, hb;
@@ -377,8 +424,8 @@ __global__ void __launch_bounds__(THREADS) kf_row_inv(const Member* __restrict__
                 dw[ca] = ka;
                 dw[cb] = kb;
             }
-            update(ca, ka);
-            update(cb, kb);
+            update(ca, xa[e], ka);
+            update(cb, xb[e], kb);
         }
     }
     block_reduce3(a1, a2, s2, M.partials + (size_t)blockIdx.x * 4);
@@ -397,13 +444,13 @@ template <typename T, int N, int THREADS>
 static void rows_fwd(const Member* d, int nmem, hipStream_t st, const void* tw) {
     constexpr int L = THREADS / (N / 8);
     const size_t lds = row_lds<T, N>(THREADS);
-    kf_row_fwd<T, N, THREADS><<<dim3(N / (2 * L), nmem), dim3(THREADS), lds, st>>>(d, tw);
+    launch_k(1, kf_row_fwd<T, N, THREADS>, dim3(N / (2 * L), nmem), dim3(THREADS), lds, st, d, tw);
 }
 template <typename T, int N, int THREADS>
 static bool srow_fwd(const Member* d, int nmem, hipStream_t st, const void* tw) {
     using SR = SRow<T, N, THREADS>;
     if constexpr (SR::CPT >= 1 && SR::CPT * THREADS == N && SR::ROWS >= 4 && SR::lds <= 65536) {
-        kf_srow_fwd<T, N, THREADS><<<dim3(N / SR::ROWS, nmem), dim3(THREADS), SR::lds, st>>>(d, tw);
+        launch_k(0, kf_srow_fwd<T, N, THREADS>, dim3(N / SR::ROWS, nmem), dim3(THREADS), SR::lds, st, d, tw);
         return true;
     }
     return false;
@@ -412,12 +459,12 @@ template <typename T, int N, int THREADS>
 static void rows_inv(const Member* d, int nmem, hipStream_t st, const void* tw, int ncolblk) {
     constexpr int L = THREADS / (N / 8);
     const size_t lds = row_lds<T, N>(THREADS);
-    kf_row_inv<T, N, THREADS><<<dim3(N / (2 * L), nmem), dim3(THREADS), lds, st>>>(d, tw, ncolblk);
+    launch_k(3, kf_row_inv<T, N, THREADS>, dim3(N / (2 * L), nmem), dim3(THREADS), lds, st, d, tw, ncolblk);
 }
 template <typename T, int N, int C>
 static void cols(const Member* d, int nmem, hipStream_t st, const void* tw, const double2* tw64) {
     const size_t lds = col_lds<T, N>(C);
-    kf_col<T, N, C><<<dim3(N / (2 * C), nmem), dim3(C * (N / 8)), lds, st>>>(d, tw, tw64);
+    launch_k(2, kf_col<T, N, C>, dim3(N / (2 * C), nmem), dim3(C * (N / 8)), lds, st, d, tw, tw64);
 }
 
 template <typename T, int N>

@@ -3,9 +3,10 @@
 //   k_pre_haf     Hice embedding (out2in), calc_Haf and the two domain sums of
 //                 calc_sl_contributions (sealevel.f90:132-145, 189-205)
 //   k_pre_bsl     V_af / V_den / deltaV and the barystatic sea-level update
-//                 (fastisostasy.f90:763-778).  One thread walks the domains in
-//                 batch order, so domains sharing a bsl object see the same
-//                 sequence of updates as consecutive isos_update calls.
+//                 (fastisostasy.f90:763-778).  One wave per group of domains
+//                 sharing a bsl object walks the group in batch order, so the
+//                 group sees the same sequence of updates as consecutive
+//                 isos_update calls; independent groups run in parallel.
 //   k_pre_fields  z_ss, masks and the column anomalies (sealevel.f90:25-187,
 //                 fastisostasy.f90:778-833) in three phases around the two
 //                 Green's-function convolutions
@@ -82,7 +83,8 @@ __global__ void __launch_bounds__(PRE_THREADS) k_pre_haf(const Member* __restric
 
 __global__ void __launch_bounds__(64) k_pre_bsl(const Member* __restrict__ mem, int nmem, int nblk) {
     const int lane = threadIdx.x;
-    for (int m = 0; m < nmem; ++m) {
+    if (!mem[blockIdx.x].bsl_lead) return;
+    for (int m = blockIdx.x; m >= 0 && m < nmem; m = mem[m].bsl_next) {
         const Member& M = mem[m];
         double s1 = 0.0, s2 = 0.0;
         for (int b = lane; b < nblk; b += 64) { s1 += M.partials[(size_t)b * 4]; s2 += M.partials[(size_t)b * 4 + 1]; }
@@ -293,7 +295,7 @@ void launch_pre_haf(const Member* d_mem, int nmem, int n, bool new_H, hipStream_
     hipLaunchKernelGGL(k_pre_haf, dim3(pre_blocks(n), nmem), dim3(PRE_THREADS), 0, st, d_mem, (int)new_H);
 }
 void launch_pre_bsl(const Member* d_mem, int nmem, int n, hipStream_t st) {
-    hipLaunchKernelGGL(k_pre_bsl, dim3(1), dim3(64), 0, st, d_mem, nmem, pre_blocks(n));
+    hipLaunchKernelGGL(k_pre_bsl, dim3(nmem), dim3(64), 0, st, d_mem, nmem, pre_blocks(n));
 }
 void launch_pre_fields(const Member* d_mem, int nmem, int n, int phase, bool diag, bool fp64, hipStream_t st) {
     if (fp64) hipLaunchKernelGGL(k_pre_fields<double>, dim3(pre_blocks(n), nmem), dim3(PRE_THREADS), 0, st, d_mem, phase, (int)diag);

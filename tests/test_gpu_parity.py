@@ -204,3 +204,40 @@ def test_fused_stencil_matches_separate_kernels(case, prec, threads, monkeypatch
     for f in ("w", "dwdt"):
         assert np.array_equal(a[f], b[f]), (case, f, np.abs(a[f] - b[f]).max())
     assert np.abs(a["w"]).max() > 0
+
+
+def test_batch_shared_bsl_matches_consecutive_updates():
+    """Domains sharing one barystatic sea-level object: a batch update applies
+    their volume changes to the shared bsl in member order, exactly as
+    consecutive isos_update calls do (the last two members share one object,
+    the first has its own)."""
+    wl = small_sealevel()
+    Hmax = wl["H_max"]
+    scales = (1.0, 0.6, 0.8)
+    p = G.make_params(**wl["params"])
+    singles, gs = [], []
+    for _ in scales:
+        singles.append(G.IsosClass(p, wl["nx"], wl["nx"], wl["dx"]))
+        gs.append(G.IsosClass(p, wl["nx"], wl["nx"], wl["dx"]))
+    sb = [G.bsl_init("fastiso", 0.0), G.bsl_init("fastiso", 0.0)]
+    bb = [G.bsl_init("fastiso", 0.0), G.bsl_init("fastiso", 0.0)]
+    s_of = [sb[0], sb[1], sb[1]]
+    b_of = [bb[0], bb[1], bb[1]]
+    for a, b, sa, ba in zip(singles, gs, s_of, b_of):
+        a.init_state(wl["z_bed"], np.zeros_like(wl["z_bed"]), 0.0, sa)
+        b.init_state(wl["z_bed"], np.zeros_like(wl["z_bed"]), 0.0, ba)
+    batch = G.Batch(gs)
+    for k in range(1, 16):
+        t = 10.0 * k
+        Hs = [s * Hmax * min(1.0, t / 100.0) * np.ones_like(wl["z_bed"]) for s in scales]
+        for o in sb + bb:
+            G.bsl_update(o, t)
+        for a, sa, H in zip(singles, s_of, Hs):
+            a.update(H, t, sa)
+        batch.update(t, b_of, H_ice=Hs)
+    for a, b in zip(singles, gs):
+        for f in ("w", "z_bed", "dz_ss", "z_ss"):
+            assert np.array_equal(a.out(f), b.out(f)), f
+        assert a.scalar("bsl") == b.scalar("bsl")
+    assert sb[1].bsl_now == bb[1].bsl_now != 0.0
+    assert sb[0].bsl_now == bb[0].bsl_now
