@@ -33,6 +33,21 @@ KERNEL_NAMES = {0: "k_stencil", 1: "k_row_fwd", 2: "k_col", 3: "k_row_inv"}
 FUSED_NAME = "kf_srow_fwd (stencil + forward row pass)"
 
 
+def measured_traffic(kernel_name, cells):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE measurement (scaled by the copy-kernel calibration), if it
+    covers this kernel at this size."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_ensemble256.json")
+    try:
+        with open(path) as f:
+            k = json.load(f)["kernels"].get(kernel_name)
+    except (OSError, ValueError, KeyError):
+        return None
+    if not k or k["cells"] != cells:
+        return None
+    return k["read_bytes"] + k["write_bytes"]
+
+
 def kernel_bytes_per_cell(kernel, s, fused):
     """Algorithmic HBM bytes per cell per launch (s = bytes per RHS real).
 
@@ -230,9 +245,12 @@ def main():
         bytes_per_launch = kernel_bytes_per_cell(args.roofline_kernel, 4, fused) * cells
         achieved = bytes_per_launch / avg_s / 1e9
         name = FUSED_NAME if (fused and args.roofline_kernel == 0) else KERNEL_NAMES[args.roofline_kernel]
+        traffic = None
+        if fused and args.roofline_kernel == 0 and args.members == 64 and n == 256:
+            traffic = measured_traffic("kf_srow_fwd<float, 256, 256>", cells)
         roof = {"kernel": name, "bound": "hbm", "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": None, "avg_launch_us": round(avg_s * 1e6, 2), "launches": cnt,
+                "traffic": traffic, "avg_launch_us": round(avg_s * 1e6, 2), "launches": cnt,
                 "bytes_per_launch": bytes_per_launch}
 
     cpu = None

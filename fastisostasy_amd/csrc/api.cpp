@@ -132,10 +132,18 @@ int make_shape(int n, int nmem, bool fp64, bool uniform, RhsShape& s) {
             if ((th == 64 || th == 128 || th == 256) && row_ok(th)) best_t = th;
         }
         int best_c = 0;
+        auto col_ok = [&](int C) {
+            return C * tpl <= 1024 && ((size_t)n + (size_t)n * (C + 1)) * cb <= 65536 && n % (2 * C) == 0;
+        };
         for (int C : {16, 8, 4, 2, 1}) {
-            if (C * tpl > 1024 || ((size_t)n + (size_t)n * (C + 1)) * cb > 65536 || n % (2 * C)) continue;
+            if (!col_ok(C)) continue;
             best_c = C;
             if ((size_t)(n / (2 * C)) * nmem >= 256) break;
+        }
+        // FI_COL_LINES = 1 | 2 | 4 | 8 | 16 overrides the column block (tuning, tests)
+        if (const char* e = std::getenv("FI_COL_LINES")) {
+            const int C = std::atoi(e);
+            if ((C == 1 || C == 2 || C == 4 || C == 8 || C == 16) && col_ok(C)) best_c = C;
         }
         if (best_t && best_c) {
             s.fused = uniform && !std::getenv("FI_NO_FUSED_STENCIL");

@@ -88,7 +88,8 @@ struct Member {
     uint8_t* mask_ref;
     double* H_in;                // ice-grid staging buffer (nx_ice * ny_ice)
     // RHS inputs in RHS precision
-    void *D, *eta, *kR, *cfull;  // D_lith, eta_eff (plain: eta, mixed: 1/(2 eta)), kappa*R, canom_full
+    void *D, *eta, *kR, *cfull;  // D_lith, eta_eff (plain: eta, mixed: 1/(2 eta)),
+                                 // kappa*R (plain; mixed: 1/(kappa*R)), canom_full
     void *g1, *g2;               // FFT work arrays (N x N)
     double* corners;             // per column-block Hartley corner partials [nblk][4]
     double* partials;            // per block reduction partials [nblk][4]

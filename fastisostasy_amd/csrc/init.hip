@@ -148,7 +148,9 @@ __global__ void k_vfinal(InitArgs a, const double* __restrict__ red, double* __r
     D_t[c] = (T)Dd[c];
     if (sizeof(T) == 8) eta_t[c] = (T)e;
     else eta_t[c] = (T)(1.0 / (2.0 * e));
-    kR_t[c] = (T)(R[c] * kappa[c]);
+    // mixed: the reciprocal, so the FP32 column pass multiplies instead of dividing
+    if (sizeof(T) == 8) kR_t[c] = (T)(R[c] * kappa[c]);
+    else kR_t[c] = (T)(1.0 / (R[c] * kappa[c]));
 }
 
 __global__ void k_green(int which, int n, double dx, double r_earth, double m_earth, double* __restrict__ out) {

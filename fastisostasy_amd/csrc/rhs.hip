@@ -261,12 +261,12 @@ __global__ void __launch_bounds__(512) k_col(const Member* __restrict__ mem, Fft
         T ha, hb, ham, hbm;
         dht_pair(zk, zm, ha, hb);
         dht_pair(zm, zk, ham, hbm);
-        ha = ha / kR[(size_t)k * N + c0 + l];
-        hb = hb / kR[(size_t)k * N + c0 + C + l];
+        ha = kdiv(ha, kR[(size_t)k * N + c0 + l]);
+        hb = kdiv(hb, kR[(size_t)k * N + c0 + C + l]);
         buf[k * C + l] = mk<T>(ha, hb);
         if (km != k) {
-            ham = ham / kR[(size_t)km * N + c0 + l];
-            hbm = hbm / kR[(size_t)km * N + c0 + C + l];
+            ham = kdiv(ham, kR[(size_t)km * N + c0 + l]);
+            hbm = kdiv(hbm, kR[(size_t)km * N + c0 + C + l]);
             buf[km * C + l] = mk<T>(ham, hbm);
         }
     }

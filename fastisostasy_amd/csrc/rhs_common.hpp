@@ -18,6 +18,11 @@ __device__ __forceinline__ double attempt_dt(const Ctl& c, const PhysDev& ph) {
     return attempt_dt(ph.dt_method, c.dt, c.t, c.tf, ph.dt_max);
 }
 
+// Spectral divide by kappa*R (lv_elva.f90:115): FP64 divides; the FP32 path
+// holds 1/(kappa*R) and multiplies.
+__device__ __forceinline__ double kdiv(double h, double kr) { return h / kr; }
+__device__ __forceinline__ float kdiv(float h, float rkr) { return h * rkr; }
+
 // Scalar-cache views of the control block and the physics scalars.  Both are
 // read-only while an attempt pass runs (k_decide, a separate launch, writes
 // Ctl), so their fields load with SMEM into SGPRs rather than with one vector

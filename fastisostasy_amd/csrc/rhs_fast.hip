@@ -278,10 +278,18 @@ __global__ void __launch_bounds__(C * (N / 8)) kf_col(const Member* __restrict__
     gptr<T> src = gin<T>(M.g2);
     gptr<T> kR = gin<T>(M.kR);
     cplx<T> v[E], m[E];
+    T ka[E], kb[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         const size_t row = (size_t)(t + TPL * e) * N;
         v[e] = mk<T>(src[row + ca], src[row + cb]);
+    }
+    // kappa*R is independent of the transform: load it now
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const size_t row = (size_t)(t + TPL * e) * N;
+        ka[e] = kR[row + ca];
+        kb[e] = kR[row + cb];
     }
     __syncthreads();
     ColXch<T, N> xc(buf, line, C);
@@ -290,10 +298,9 @@ __global__ void __launch_bounds__(C * (N / 8)) kf_col(const Member* __restrict__
     // Hartley values and the spectral divide (lv_elva.f90:115)
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-        const size_t row = (size_t)(t + TPL * e) * N;
         T ha, hb;
         dht_pair(v[e], m[e], ha, hb);
-        v[e] = mk<T>(ha / kR[row + ca], hb / kR[row + cb]);
+        v[e] = mk<T>(kdiv(ha, ka[e]), kdiv(hb, kb[e]));
     }
     reg_fft<T, N>(v, t, tw_l, xc);
     reg_mirror<T, N>(v, m, t, xc);
@@ -376,7 +383,8 @@ __global__ void __launch_bounds__(THREADS) kf_row_inv(const Member* __restrict__
         }
     } else {
         __shared__ double qs[4];
-        const double S = (double)(float)((double)N * (double)N);
+        // N*N is a power of two here: the division by it is a multiplication
+        const double S = (double)(float)((double)N * (double)N), rS = 1.0 / S;
         gptr<T> src = gin<T>(M.g1);
         cplx<T> v[E], m[E];
 #pragma unroll
@@ -411,7 +419,7 @@ __global__ void __launch_bounds__(THREADS) kf_row_inv(const Member* __restrict__
         reg_fft<T, N>(v, t, tw_l, xc);
         reg_mirror<T, N>(v, m, t, xc);
         // x(1,1) + x(nx,ny) + x(1,ny) + x(nx,1)  (isos_utils.f90:116)
-        const double cc = 0.25 * (qs[0] / S + qs[3] / S + qs[2] / S + qs[1] / S);
+        const double cc = 0.25 * (qs[0] * rS + qs[3] * rS + qs[2] * rS + qs[1] * rS);
         const double spy = ph->sec_per_year;
         const bool store_k = !(c->t + dt < c->tf);
 #pragma unroll
@@ -419,7 +427,7 @@ __global__ void __launch_bounds__(THREADS) kf_row_inv(const Member* __restrict__
             T ha, hb;
             dht_pair(v[e], m[e], ha, hb);
             const size_t ca = (size_t)ra * N + t + TPL * e, cb = (size_t)rb * N + t + TPL * e;
-            const double ka = ((double)ha / S - cc) * spy, kb = ((double)hb / S - cc) * spy;
+            const double ka = ((double)ha * rS - cc) * spy, kb = ((double)hb * rS - cc) * spy;
             if (store_k) {
                 dw[ca] = ka;
                 dw[cb] = kb;
