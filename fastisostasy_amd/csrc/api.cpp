@@ -160,16 +160,18 @@ int make_shape(int n, int nmem, bool fp64, bool uniform, RhsShape& s) {
         }
         s.fast = false;
     }
-    // aim for >= 512 row blocks and >= 256 column blocks over the batch
-    // LDS per block stays within 64 KiB (n * lines complex elements)
+    // Generic kernels: the error-norm and corner partials are per block, so the
+    // blocking depends on N alone (a domain's results do not depend on the
+    // batch it is launched in).  LDS per block stays within 64 KiB.
     const size_t cbytes = fp64 ? 16 : 8;
     auto fits = [&](int lines) { return (size_t)n * lines * cbytes <= 65536; };
     if (!fits(1)) return fail(FI_E_UNSUPPORTED, "padded size N=%d too large for one LDS line", n);
+    (void)nmem;
     int L = 1;
-    while (L < 8 && (n / 2) % (2 * L) == 0 && fits(2 * L) && (size_t)(n / (4 * L)) * nmem >= 512) L *= 2;
+    while (L < 8 && (n / 2) % (2 * L) == 0 && fits(2 * L) && n / (4 * L) >= 8) L *= 2;
     const int cmax = fp64 ? 8 : 16;
     int C = 1;
-    while (C < cmax && (n / 2) % (2 * C) == 0 && fits(2 * C) && (size_t)(n / (4 * C)) * nmem >= 256) C *= 2;
+    while (C < cmax && (n / 2) % (2 * C) == 0 && fits(2 * C) && n / (4 * C) >= 4) C *= 2;
     s.L = L;
     s.C = C;
     int rc = fft_threads(s.plan, L, s.threads_row);

@@ -431,10 +431,14 @@ void launch_attempt_t(const Member* d_mem, int nmem, const RhsShape& s, hipStrea
     if (!s.fast || !launch_fast_pass(d_mem, nmem, s, st, 2))
         launch_k(2, k_col<T>, dim3(s.n / (2 * s.C), nmem), dim3(s.threads_col), lds_bytes<T>(s.n, s.C), st,
                  d_mem, s.plan, (const void*)s.tw, s.tw64, s.C);
-    if (!s.fast || !launch_fast_pass(d_mem, nmem, s, st, 3))
+    // partials of the error norm: one per row (fast pass), one per row block (generic)
+    int nblk = s.n;
+    if (!s.fast || !launch_fast_pass(d_mem, nmem, s, st, 3)) {
         launch_k(3, k_row_inv<T>, dim3(s.n / (2 * s.L), nmem), dim3(s.threads_row), lds_bytes<T>(s.n, s.L), st,
                  d_mem, s.plan, (const void*)s.tw, s.L, s.n / (2 * s.C));
-    hipLaunchKernelGGL(k_decide, dim3(nmem), dim3(64), 0, st, d_mem, s.n / (2 * s.L));
+        nblk = s.n / (2 * s.L);
+    }
+    hipLaunchKernelGGL(k_decide, dim3(nmem), dim3(64), 0, st, d_mem, nblk);
 }
 
 void launch_attempt(const Member* d_mem, int nmem, const RhsShape& s, hipStream_t st) {

@@ -8,10 +8,12 @@
 //   kf_row_inv   row Hartley transform, /N^2, corner BC, * sec_per_year,
 //                stage sums, reduction partials             (g1 -> dwdt, x')
 //
-// Rows: L complex lines (2L real rows: r0+l in the real part, r0+L+l in the
+// Rows: L complex lines (2L real rows: r0+2l in the real part, r0+2l+1 in the
 // imaginary part) per block, TPL = N/8 threads per line.  Columns: C complex
-// lines (columns c0+l and c0+C+l), lane -> (line = tid % C, t = tid / C) so
-// that consecutive lanes read consecutive columns.
+// lines (columns c0+2l and c0+2l+1, read and written as one vector), lane ->
+// (line = tid % C, t = tid / C) so that consecutive lanes read consecutive
+// column pairs.  Packing neighbours (not r and r+L) makes every line's
+// rounding independent of the block shape, hence of the batch size.
 #include "fft_reg.hpp"
 #include "fi_kernels.h"
 #include "rhs_common.hpp"
@@ -25,6 +27,10 @@ __device__ __forceinline__ void load_tw(cplx<T>* tw_l, const cplx<T>* __restrict
     for (int q = threadIdx.x; q < 2 * N; q += blockDim.x) l[q] = g[q];
 }
 
+template <typename T> struct vec2;
+template <> struct vec2<float> { typedef float type __attribute__((ext_vector_type(2))); };
+template <> struct vec2<double> { typedef double type __attribute__((ext_vector_type(2))); };
+
 template <typename T, int N, int THREADS>
 __global__ void __launch_bounds__(THREADS) kf_row_fwd(const Member* __restrict__ mem, const void* __restrict__ twv) {
     const Member& M = mem[blockIdx.y];
@@ -37,7 +43,7 @@ __global__ void __launch_bounds__(THREADS) kf_row_fwd(const Member* __restrict__
     cplx<T>* buf = tw_l + N;
     const int tid = threadIdx.x, line = tid / TPL, t = tid % TPL;
     load_tw<T, N>(tw_l, (const cplx<T>*)twv);
-    const int ra = blockIdx.x * 2 * L + line, rb = ra + L;
+    const int ra = blockIdx.x * 2 * L + 2 * line, rb = ra + 1;
     gptr<T> src = gin<T>(M.g1);
     cplx<T> v[E], m[E];
 #pragma unroll
@@ -243,13 +249,13 @@ __global__ void __launch_bounds__(THREADS) kf_srow_fwd(const Member* __restrict_
     const int line = tid / TPL, t = tid % TPL;
     cplx<T> v[E], mr[E];
 #pragma unroll
-    for (int e = 0; e < E; ++e) v[e] = mk<T>(sf[line * N + t + TPL * e], sf[(L + line) * N + t + TPL * e]);
+    for (int e = 0; e < E; ++e) v[e] = mk<T>(sf[2 * line * N + t + TPL * e], sf[(2 * line + 1) * N + t + TPL * e]);
     __syncthreads();
     RowXch<T, N> xc((cplx<T>*)(tw_l + N), line);
     reg_fft<T, N>(v, t, tw_l, xc);
     reg_mirror<T, N>(v, mr, t, xc);
     gmut<T> dst = gout<T>(M.g2);
-    const int ra = R0 + line, rb = ra + L;
+    const int ra = R0 + 2 * line, rb = ra + 1;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         T ha, hb;
@@ -270,26 +276,26 @@ __global__ void __launch_bounds__(C * (N / 8)) kf_col(const Member* __restrict__
     extern __shared__ __align__(16) unsigned char smem[];
     cplx<T>* tw_l = (cplx<T>*)smem;
     cplx<T>* buf = tw_l + N;
-    __shared__ double sc[2][2 * C];
     const int tid = threadIdx.x, line = tid % C, t = tid / C;
     load_tw<T, N>(tw_l, (const cplx<T>*)twv);
     const int c0 = blockIdx.x * 2 * C;
-    const int ca = c0 + line, cb = c0 + C + line;
-    gptr<T> src = gin<T>(M.g2);
-    gptr<T> kR = gin<T>(M.kR);
+    const int ca = c0 + 2 * line, cb = ca + 1;   // adjacent columns share a complex line
+    using V2 = typename vec2<T>::type;
+    gptr<V2> src = gin<V2>(M.g2);
+    gptr<V2> kR = gin<V2>(M.kR);
     cplx<T> v[E], m[E];
     T ka[E], kb[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-        const size_t row = (size_t)(t + TPL * e) * N;
-        v[e] = mk<T>(src[row + ca], src[row + cb]);
+        const V2 x = src[((size_t)(t + TPL * e) * N + ca) >> 1];
+        v[e] = mk<T>(x.x, x.y);
     }
     // kappa*R is independent of the transform: load it now
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-        const size_t row = (size_t)(t + TPL * e) * N;
-        ka[e] = kR[row + ca];
-        kb[e] = kR[row + cb];
+        const V2 x = kR[((size_t)(t + TPL * e) * N + ca) >> 1];
+        ka[e] = x.x;
+        kb[e] = x.y;
     }
     __syncthreads();
     ColXch<T, N> xc(buf, line, C);
@@ -304,28 +310,66 @@ __global__ void __launch_bounds__(C * (N / 8)) kf_col(const Member* __restrict__
     }
     reg_fft<T, N>(v, t, tw_l, xc);
     reg_mirror<T, N>(v, m, t, xc);
-    T* __restrict__ dst = (T*)M.g1;
+    gmut<V2> dst = gout<V2>(M.g1);
+    gmut<double> cor = gout<double>(M.corners);
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         const int k = t + TPL * e;
         T ha, hb;
         dht_pair(v[e], m[e], ha, hb);
-        dst[(size_t)k * N + ca] = ha;
-        dst[(size_t)k * N + cb] = hb;
-        if (k == 0) { sc[0][line] = (double)ha; sc[0][C + line] = (double)hb; }
-        if (k == N - 1) { sc[1][line] = (double)ha; sc[1][C + line] = (double)hb; }
-    }
-    __syncthreads();
-    if (tid < 4) {
-        const int r = tid >> 1;
-        const bool cs = tid & 1;
-        double acc = 0.0;
-        for (int mc = 0; mc < 2 * C; ++mc) {
-            const double val = sc[r][mc];
-            const int col = c0 + mc;
-            acc += cs ? val * (tw64[col].x + tw64[col].y) : val;
+        V2 o;
+        o.x = ha;
+        o.y = hb;
+        dst[((size_t)k * N + ca) >> 1] = o;
+        // per-column terms of the four corners of the inverse transform (rows 0
+        // and N-1, weights 1 and cas(2 pi col (N-1) / N)); the row pass sums them
+        // in an order fixed by N alone
+        if (k == 0 || k == N - 1) {
+            const int q = (k == 0) ? 0 : 2;
+            const double va = (double)ha, vb = (double)hb;
+            cor[(size_t)ca * 4 + q] = va;
+            cor[(size_t)ca * 4 + q + 1] = va * (tw64[ca].x + tw64[ca].y);
+            cor[(size_t)cb * 4 + q] = vb;
+            cor[(size_t)cb * 4 + q + 1] = vb * (tw64[cb].x + tw64[cb].y);
         }
-        M.corners[(size_t)blockIdx.x * 4 + tid] = acc;
+    }
+}
+
+// Reduce the per-thread row statistics of two rows over the TPL threads of a
+// line with a fixed butterfly (lines wider than a wave combine their wave
+// halves through LDS in wave order), and store them as the row's partials.
+// The pattern depends on N alone, so the error norm of a domain does not
+// depend on how many domains share the launch.
+template <int TPL>
+__device__ __forceinline__ void row_reduce_store(double a1a, double a2a, double s2a, double a1b, double a2b,
+                                                 double s2b, int line, int t, gmut<double> pa, gmut<double> pb) {
+    constexpr int W = TPL < 64 ? TPL : 64;
+#pragma unroll
+    for (int o = W / 2; o > 0; o >>= 1) {
+        a1a = fmax(a1a, __shfl_xor(a1a, o, W)); a2a = fmax(a2a, __shfl_xor(a2a, o, W));
+        s2a += __shfl_xor(s2a, o, W);
+        a1b = fmax(a1b, __shfl_xor(a1b, o, W)); a2b = fmax(a2b, __shfl_xor(a2b, o, W));
+        s2b += __shfl_xor(s2b, o, W);
+    }
+    if constexpr (TPL > 64) {
+        constexpr int NW = TPL / 64;
+        __shared__ double red[16][NW][6];
+        if ((t & 63) == 0) {
+            double* r = red[line][t >> 6];
+            r[0] = a1a; r[1] = a2a; r[2] = s2a; r[3] = a1b; r[4] = a2b; r[5] = s2b;
+        }
+        __syncthreads();
+        if (t == 0) {
+            for (int w = 1; w < NW; ++w) {
+                const double* r = red[line][w];
+                a1a = fmax(a1a, r[0]); a2a = fmax(a2a, r[1]); s2a += r[2];
+                a1b = fmax(a1b, r[3]); a2b = fmax(a2b, r[4]); s2b += r[5];
+            }
+        }
+    }
+    if (t == 0) {
+        pa[0] = a1a; pa[1] = a2a; pa[2] = s2a;
+        pb[0] = a1b; pb[1] = a2b; pb[2] = s2b;
     }
 }
 
@@ -342,7 +386,7 @@ __global__ void __launch_bounds__(THREADS) kf_row_inv(const Member* __restrict__
     cplx<T>* tw_l = (cplx<T>*)smem;
     cplx<T>* buf = tw_l + N;
     const int tid = threadIdx.x, line = tid / TPL, t = tid % TPL;
-    const int ra = blockIdx.x * 2 * L + line, rb = ra + L;
+    const int ra = blockIdx.x * 2 * L + 2 * line, rb = ra + 1;
     const int meth = ph->dt_method;
     const double dt = attempt_dt(meth, c->dt, c->t, c->tf, ph->dt_max);
     const int cur = c->cur;
@@ -350,8 +394,9 @@ __global__ void __launch_bounds__(THREADS) kf_row_inv(const Member* __restrict__
     gptr<double> X = gin<double>(M.X[cur]);
     gmut<double> Xn = gout<double>(M.X[cur ^ 1]);
     gmut<double> dw = gout<double>(M.dwdt);
-    double a1 = 0.0, a2 = 0.0, s2 = 0.0;
-    auto update = [&](size_t cidx, double x, double k) {
+    // per-row max|x1|, max|x2|, sum (x2-x1)^2 of rows ra (a) and rb (b)
+    double a1a = 0.0, a2a = 0.0, s2a = 0.0, a1b = 0.0, a2b = 0.0, s2b = 0.0;
+    auto update = [&](size_t cidx, double x, double k, double& a1, double& a2, double& s2) {
         double x1;
         const double x2 = stage_sums(meth, x, k, dt, x1);
         Xn[cidx] = x2;
@@ -378,8 +423,8 @@ __global__ void __launch_bounds__(THREADS) kf_row_inv(const Member* __restrict__
         }
 #pragma unroll
         for (int e = 0; e < E; ++e) {
-            update((size_t)ra * N + t + TPL * e, xa[e], ka[e]);
-            update((size_t)rb * N + t + TPL * e, xb[e], kb[e]);
+            update((size_t)ra * N + t + TPL * e, xa[e], ka[e], a1a, a2a, s2a);
+            update((size_t)rb * N + t + TPL * e, xb[e], kb[e], a1b, a2b, s2b);
         }
     } else {
         __shared__ double qs[4];
@@ -398,13 +443,18 @@ __global__ void __launch_bounds__(THREADS) kf_row_inv(const Member* __restrict__
             for (int k = 0; k < TWQ; ++k)
                 if (tid + THREADS * k < 2 * N) twr[k] = g[tid + THREADS * k];
         }
-        if (tid < 4) {
-            // corner partial sums of the column pass, one corner per thread
+        if (tid < 64) {
+            // the four corner sums over the N per-column terms of the column
+            // pass: lane-strided, then a fixed butterfly (order set by N alone)
             gptr<double> cp = gin<double>(M.corners);
-            double q = 0;
-#pragma unroll 8
-            for (int b = 0; b < ncolblk; ++b) q += cp[(size_t)b * 4 + tid];
-            qs[tid] = q;
+            double q0 = 0, q1 = 0, q2 = 0, q3 = 0;
+#pragma unroll
+            for (int col = tid; col < N; col += 64) {
+                q0 += cp[(size_t)col * 4 + 0]; q1 += cp[(size_t)col * 4 + 1];
+                q2 += cp[(size_t)col * 4 + 2]; q3 += cp[(size_t)col * 4 + 3];
+            }
+            q0 = wave_sum(q0); q1 = wave_sum(q1); q2 = wave_sum(q2); q3 = wave_sum(q3);
+            if (tid == 0) { qs[0] = q0; qs[1] = q1; qs[2] = q2; qs[3] = q3; }
         }
 #pragma unroll
         for (int e = 0; e < E; ++e) {
@@ -432,11 +482,12 @@ __global__ void __launch_bounds__(THREADS) kf_row_inv(const Member* __restrict__
                 dw[ca] = ka;
                 dw[cb] = kb;
             }
-            update(ca, xa[e], ka);
-            update(cb, xb[e], kb);
+            update(ca, xa[e], ka, a1a, a2a, s2a);
+            update(cb, xb[e], kb, a1b, a2b, s2b);
         }
     }
-    block_reduce3(a1, a2, s2, M.partials + (size_t)blockIdx.x * 4);
+    row_reduce_store<TPL>(a1a, a2a, s2a, a1b, a2b, s2b, line, t, gout<double>(M.partials) + (size_t)ra * 4,
+                          gout<double>(M.partials) + (size_t)rb * 4);
 }
 
 // ------------------------------------------------------------------ dispatch
