@@ -19,7 +19,8 @@ FIELDS = ["z_bed", "z_ss", "dz_ss", "w", "we", "dwdt", "canom_full", "Hice",
           "ode_x", "z_ss_ref", "z_bed_ref"]
 FIELD_ID = {n: i for i, n in enumerate(FIELDS)}
 OUT_FIELDS = FIELDS[:11]
-SCALARS = ["n", "t", "dt", "bsl", "deltaV", "V_af", "V_den", "attempts", "accepts", "D_lith11", "err", "tol"]
+SCALARS = ["n", "t", "dt", "bsl", "deltaV", "V_af", "V_den", "attempts", "accepts", "D_lith11", "err", "tol",
+           "hbm_bytes"]
 SCALAR_ID = {n: i for i, n in enumerate(SCALARS)}
 
 # every entry point of include/fastiso.h
@@ -27,7 +28,9 @@ EXPORTS = ["fi_last_error", "fi_version", "fi_padded_size", "fi_isos_create", "f
            "fi_isos_init_state", "fi_isos_update", "fi_isos_get", "fi_isos_get_mask", "fi_isos_get_full",
            "fi_isos_get_scalar", "fi_isos_destroy", "fi_bsl_create", "fi_bsl_update", "fi_bsl_get",
            "fi_bsl_destroy", "fi_batch_create", "fi_batch_update", "fi_batch_destroy", "fi_sync",
-           "fi_prof_select", "fi_prof_collect", "fi_prof_fused"]
+           "fi_prof_select", "fi_prof_collect", "fi_prof_fused",
+           "fi_server_create", "fi_server_admit", "fi_server_release", "fi_server_submit", "fi_server_pending",
+           "fi_server_run", "fi_server_completed", "fi_server_stats", "fi_server_destroy"]
 
 
 class FiParams(ctypes.Structure):
@@ -89,6 +92,16 @@ def load():
     L.fi_prof_select.argtypes = [i]
     L.fi_prof_collect.argtypes = [dp, ctypes.POINTER(ctypes.c_longlong)]
     L.fi_prof_fused.argtypes = []
+    ll = ctypes.c_longlong
+    L.fi_server_create.argtypes = [d, i, ctypes.POINTER(vp)]
+    L.fi_server_admit.argtypes = [vp, vp, vp, ctypes.POINTER(i)]
+    L.fi_server_release.argtypes = [vp, i]
+    L.fi_server_submit.argtypes = [vp, i, dp, d, ll]
+    L.fi_server_pending.argtypes = [vp, ctypes.POINTER(i)]
+    L.fi_server_run.argtypes = [vp, ctypes.POINTER(i)]
+    L.fi_server_completed.argtypes = [vp, ctypes.POINTER(ll), i, ctypes.POINTER(i)]
+    L.fi_server_stats.argtypes = [vp, ctypes.POINTER(ll), ctypes.POINTER(ll), ctypes.POINTER(ll), dp]
+    L.fi_server_destroy.argtypes = [vp]
     for name in EXPORTS:
         if name not in ("fi_last_error", "fi_version"):
             getattr(L, name).restype = i

@@ -12,8 +12,11 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
+#include <deque>
 #include <map>
 #include <mutex>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -167,11 +170,14 @@ int make_shape(int n, int nmem, bool fp64, bool uniform, RhsShape& s) {
     auto fits = [&](int lines) { return (size_t)n * lines * cbytes <= 65536; };
     if (!fits(1)) return fail(FI_E_UNSUPPORTED, "padded size N=%d too large for one LDS line", n);
     (void)nmem;
+    int nbmax = 0;   // butterflies of the largest stage of one line
+    for (int q = 0; q < s.plan.nf; ++q) nbmax = std::max(nbmax, n / s.plan.radix[q]);
+    auto ok = [&](int lines) { return fits(lines) && nbmax * lines <= FFT_MAXIT * 512; };
     int L = 1;
-    while (L < 8 && (n / 2) % (2 * L) == 0 && fits(2 * L) && n / (4 * L) >= 8) L *= 2;
+    while (L < 8 && (n / 2) % (2 * L) == 0 && ok(2 * L) && n / (4 * L) >= 8) L *= 2;
     const int cmax = fp64 ? 8 : 16;
     int C = 1;
-    while (C < cmax && (n / 2) % (2 * C) == 0 && fits(2 * C) && n / (4 * C) >= 4) C *= 2;
+    while (C < cmax && (n / 2) % (2 * C) == 0 && ok(2 * C) && n / (4 * C) >= 4) C *= 2;
     s.L = L;
     s.C = C;
     int rc = fft_threads(s.plan, L, s.threads_row);
@@ -239,7 +245,9 @@ struct fi_isos {
     bool fp64 = true, uniform = true;
     bool ref_was_set = false, state_init = false;
     double time_diag = 0, time_prog = 0;
+    double last_t = 0, last_dt = 0;  // ode%t and ode%dt after the last update (host copy; 0 = unknown)
     char* arena = nullptr;
+    size_t arena_bytes = 0;      // HBM footprint of the domain (one allocation)
     Member hm{};
     Member* d_mem = nullptr;
     double *eta_eff = nullptr, *kappa = nullptr, *R = nullptr, *He = nullptr, *Dd = nullptr;
@@ -319,28 +327,32 @@ void sum_free(SumBuf& b) {
     b.h = nullptr;
 }
 
-int gather(Member* d_mem, SumBuf& sb) {
-    launch_gather(d_mem, sb.n, sb.d, g_stream);
+int gather(Member* d_mem, int nmem, SumBuf& sb) {
+    launch_gather(d_mem, nmem, sb.d, g_stream);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(sb.h, sb.d, sizeof(CtlSummary) * sb.n, hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipMemcpyAsync(sb.h, sb.d, sizeof(CtlSummary) * nmem, hipMemcpyDeviceToHost, g_stream));
     HIPCHK(hipStreamSynchronize(g_stream));
     return 0;
 }
 
-// One isos_update for a group of members that share N, precision and call time.
+// Diagnostics cadence of one isos_update call (fastisostasy.f90:782-787): a
+// function of the call times only.
+bool diag_due(const fi_isos* h, double time) { return time - h->time_diag >= (double)1e-3f; }
+
+// One isos_update for a group of members that share N, precision and options.
+// All members advance to `time`, or member m to times[m] when `times` is given
+// (the Member copies on the device then carry the targets in tf_req).
 int run_update(std::vector<fi_isos*>& hs, Member* d_mem, const RhsShape& shape, SumBuf& sb, double time,
-               bool stepping) {
+               bool stepping, const double* times = nullptr) {
     const int nmem = (int)hs.size();
     const fi_isos& h0 = *hs[0];
     const int n = h0.n;
-    // diagnostics cadence (fastisostasy.f90:782-787) is a function of the call times only
-    bool diag = false;
-    for (fi_isos* h : hs) {
-        const bool d = (time - h->time_diag >= (double)1e-3f);
-        if (d) h->time_diag = h->time_diag + h->par.dt_diagnostics;
-        if (h != hs[0] && d != diag) return fail(FI_E_STATE, "batch members disagree on the diagnostics cadence");
-        diag = d;
-    }
+    auto t_of = [&](int m) { return times ? times[m] : time; };
+    const bool diag = diag_due(hs[0], t_of(0));
+    for (int m = 1; m < nmem; ++m)
+        if (diag_due(hs[m], t_of(m)) != diag) return fail(FI_E_STATE, "batch members disagree on the diagnostics cadence");
+    if (diag)
+        for (fi_isos* h : hs) h->time_diag = h->time_diag + h->par.dt_diagnostics;
     launch_pre_haf(d_mem, nmem, n, true, g_stream);
     launch_pre_bsl(d_mem, nmem, n, g_stream);
     if (!diag) {
@@ -352,20 +364,28 @@ int run_update(std::vector<fi_isos*>& hs, Member* d_mem, const RhsShape& shape, 
         if (h0.par.heterogeneous_ssh) launch_convolution(d_mem, nmem, conv_shape_of(&h0), 1, g_stream);
         launch_pre_fields(d_mem, nmem, n, 2, true, h0.fp64, g_stream);
     }
-    launch_begin(d_mem, nmem, time, stepping ? 1 : 0, g_stream);
+    launch_begin(d_mem, nmem, time, stepping ? 1 : 0, times != nullptr, g_stream);
     HIPCHK(hipGetLastError());
     if (stepping) {
         // Attempts until every member reached `time`.  Members that are done
         // skip the launches on the device, so the host may launch ahead: after
         // each readback it launches as many attempts as the slowest member
         // still needs at its current dt (at least one).
+        // first round: as many attempts as the slowest member needs at the dt it
+        // ended its previous update with (an estimate; the loop below corrects it)
         int ahead = 1;
+        {
+            double need = 0.0;
+            for (int m = 0; m < nmem; ++m)
+                if (hs[m]->last_dt > 0.0) need = std::max(need, std::ceil((t_of(m) - hs[m]->last_t) / hs[m]->last_dt));
+            ahead = (int)std::min(256.0, std::max(1.0, need));
+        }
         long long total = 0;
         for (;;) {
             for (int a = 0; a < ahead; ++a) launch_attempt(d_mem, nmem, shape, g_stream);
             HIPCHK(hipGetLastError());
             total += ahead;
-            int rc = gather(d_mem, sb);
+            int rc = gather(d_mem, nmem, sb);
             if (rc) return rc;
             double need = 0.0;
             bool running = false;
@@ -382,11 +402,13 @@ int run_update(std::vector<fi_isos*>& hs, Member* d_mem, const RhsShape& shape, 
     }
     launch_finalize(d_mem, nmem, n, g_stream);
     HIPCHK(hipGetLastError());
-    int rcg = gather(d_mem, sb);
+    int rcg = gather(d_mem, nmem, sb);
     if (rcg) return rcg;
     for (int m = 0; m < nmem; ++m) {
         const CtlSummary& c = sb.h[m];
-        hs[m]->time_prog = time;
+        hs[m]->time_prog = t_of(m);
+        hs[m]->last_t = c.t;
+        hs[m]->last_dt = stepping ? c.dt : 0.0;
         // check_isos_instabilities + NaN check (fastisostasy.f90:860-869)
         if (c.maxw > 1e5) return fail(FI_E_UNSTABLE, "large_check:: Error: Large values detected in array w");
         if (c.maxwe > 1e5) return fail(FI_E_UNSTABLE, "large_check:: Error: Large values detected in array we");
@@ -557,6 +579,7 @@ int fi_isos_create(const fi_params* p, int nx_ice, int ny_ice, double dx, double
     if (h->has_conv) { ocw = take(2 * (size_t)n * hc); oFGE = take(2 * (size_t)n * hc); oFGN = take(2 * (size_t)n * hc); }
     size_t octl = take(sizeof(Ctl)), ophys = take(sizeof(PhysDev)), omem = take(sizeof(Member));
     if (hipMalloc(&h->arena, off) != hipSuccess) { delete h; return fail(FI_E_HIP, "hipMalloc arena (%zu bytes)", off); }
+    h->arena_bytes = off;
     HIPCHK(hipMemsetAsync(h->arena, 0, off, g_stream));
     rc = sum_alloc(h->sb, 1);
     if (rc) return rc;
@@ -881,6 +904,7 @@ int fi_isos_get_scalar(fi_isos* h, int which, double* out) {
     case FI_S_D_LITH11: HIPCHK(hipMemcpy(out, h->Dd, 8, hipMemcpyDeviceToHost)); break;
     case FI_S_ERR: *out = c.err; break;
     case FI_S_TOL: *out = c.tol; break;
+    case FI_S_HBM_BYTES: *out = (double)h->arena_bytes; break;
     default: return fail(FI_E_ARG, "unknown scalar %d", which);
     }
     return 0;
@@ -948,6 +972,231 @@ int fi_batch_update(fi_batch* b, const double* const* H_ice, double time, fi_bsl
     }
     if (dirty) HIPCHK(hipMemcpy(b->d_mem, b->hm.data(), sizeof(Member) * nmem, hipMemcpyHostToDevice));
     return run_update(b->mem, b->d_mem, b->shape, b->sb, time, true);
+}
+
+// ------------------------------------------------------------------ serving
+// Continuous batching over resident domains.  Each request advances one
+// domain to a target time (one isos_update).  A round takes the oldest
+// pending request of every domain that has one, groups the domains that can
+// share launches (equal N, precision, spacing, options and diagnostics
+// phase), splits groups at max_batch and runs one batched update per group.
+// Requests that arrive while a round runs join the next round.  Domains are
+// admitted while the sum of their device footprints fits the HBM budget.
+// Results do not depend on the grouping (the attempt kernels' reductions and
+// line pairings are fixed by N), so a request gives the same bits alone or
+// in any mix.
+struct fi_server {
+    struct Req {
+        long long tag;
+        double time;
+        std::vector<double> H;   // empty: keep the domain's ice thickness
+    };
+    struct Slot {
+        fi_isos* h = nullptr;
+        fi_bsl* b = nullptr;
+        std::deque<Req> q;
+    };
+    size_t budget = 0, used = 0;
+    int max_batch = 0;
+    std::vector<Slot> slots;
+    std::vector<long long> done;
+    Member* d_mem = nullptr;     // one round's Member copies, grouped
+    Member* h_stage = nullptr;   // pinned staging of the same
+    int cap = 0;                 // capacity of d_mem / h_stage / sb
+    SumBuf sb;
+    std::map<std::tuple<int, int, int, int>, RhsShape> shapes;  // (n, fp64, uniform, nmem)
+    long long rounds = 0, groups = 0, served = 0;
+};
+
+namespace {
+
+int server_reserve(fi_server* s, int n) {
+    if (n <= s->cap) return 0;
+    (void)hipStreamSynchronize(g_stream);
+    if (s->d_mem) (void)hipFree(s->d_mem);
+    if (s->h_stage) (void)hipHostFree(s->h_stage);
+    sum_free(s->sb);
+    s->d_mem = nullptr;
+    s->h_stage = nullptr;
+    s->cap = 0;
+    const int c = std::max(n, 2 * s->cap);
+    HIPCHK(hipMalloc(&s->d_mem, sizeof(Member) * c));
+    HIPCHK(hipHostMalloc(&s->h_stage, sizeof(Member) * c, hipHostMallocDefault));
+    int rc = sum_alloc(s->sb, c);
+    if (rc) return rc;
+    s->cap = c;
+    return 0;
+}
+
+}  // namespace
+
+int fi_server_create(double hbm_budget_bytes, int max_batch, fi_server** out) {
+    int rc = ensure_device();
+    if (rc) return rc;
+    if (!out || hbm_budget_bytes <= 0 || max_batch < 1) return fail(FI_E_ARG, "bad server parameters");
+    fi_server* s = new fi_server();
+    s->budget = (size_t)hbm_budget_bytes;
+    s->max_batch = max_batch;
+    *out = s;
+    return 0;
+}
+
+int fi_server_admit(fi_server* s, fi_isos* h, fi_bsl* b, int* slot) {
+    if (!s || !h || !b || !slot) return fail(FI_E_ARG, "NULL argument");
+    if (!h->state_init) return fail(FI_E_STATE, "admit after isos_init_state");
+    for (const auto& sl : s->slots)
+        if (sl.h == h) return fail(FI_E_ARG, "domain already admitted");
+    if (s->used + h->arena_bytes > s->budget)
+        return fail(FI_E_CAPACITY, "domain needs %zu bytes of HBM, %zu of %zu in use", h->arena_bytes, s->used,
+                    s->budget);
+    int id = -1;
+    for (int i = 0; i < (int)s->slots.size(); ++i)
+        if (!s->slots[i].h) { id = i; break; }
+    if (id < 0) { id = (int)s->slots.size(); s->slots.emplace_back(); }
+    s->slots[id].h = h;
+    s->slots[id].b = b;
+    s->used += h->arena_bytes;
+    *slot = id;
+    return 0;
+}
+
+int fi_server_release(fi_server* s, int slot) {
+    if (!s || slot < 0 || slot >= (int)s->slots.size() || !s->slots[slot].h) return fail(FI_E_ARG, "bad slot");
+    if (!s->slots[slot].q.empty()) return fail(FI_E_STATE, "slot %d has pending requests", slot);
+    s->used -= s->slots[slot].h->arena_bytes;
+    s->slots[slot] = fi_server::Slot();
+    return 0;
+}
+
+int fi_server_submit(fi_server* s, int slot, const double* H_ice, double time, long long tag) {
+    if (!s || slot < 0 || slot >= (int)s->slots.size() || !s->slots[slot].h) return fail(FI_E_ARG, "bad slot");
+    fi_server::Slot& sl = s->slots[slot];
+    fi_server::Req r;
+    r.tag = tag;
+    r.time = time;
+    if (H_ice) r.H.assign(H_ice, H_ice + (size_t)sl.h->nx_ice * sl.h->ny_ice);
+    sl.q.push_back(std::move(r));
+    return 0;
+}
+
+int fi_server_pending(fi_server* s, int* n) {
+    if (!s || !n) return fail(FI_E_ARG, "NULL argument");
+    size_t k = 0;
+    for (const auto& sl : s->slots) k += sl.q.size();
+    *n = (int)k;
+    return 0;
+}
+
+int fi_server_run(fi_server* s, int* completed) {
+    if (!s || !completed) return fail(FI_E_ARG, "NULL argument");
+    *completed = 0;
+    // oldest pending request of each domain, oldest first
+    std::vector<int> cand;
+    for (int i = 0; i < (int)s->slots.size(); ++i)
+        if (s->slots[i].h && !s->slots[i].q.empty()) cand.push_back(i);
+    if (cand.empty()) return 0;
+    std::stable_sort(cand.begin(), cand.end(),
+                     [&](int a, int b) { return s->slots[a].q.front().tag < s->slots[b].q.front().tag; });
+    // group by launch compatibility, in order of each group's oldest request
+    auto key_of = [&](int i) {
+        const fi_isos* h = s->slots[i].h;
+        const bool d = diag_due(h, s->slots[i].q.front().time);
+        return std::make_tuple(h->n, (int)h->fp64, (int)h->uniform, h->par.dt_method, h->par.include_elastic,
+                               h->par.heterogeneous_ssh, (int)d);
+    };
+    std::vector<std::vector<int>> grp;
+    std::vector<decltype(key_of(0))> keys;
+    for (int i : cand) {
+        const auto k = key_of(i);
+        size_t g = 0;
+        for (; g < keys.size(); ++g)
+            if (keys[g] == k && (int)grp[g].size() < s->max_batch) break;
+        if (g == keys.size()) { keys.push_back(k); grp.emplace_back(); }
+        grp[g].push_back(i);
+    }
+    int rc = server_reserve(s, (int)cand.size());
+    if (rc) return rc;
+    int base = 0;
+    for (const auto& g : grp) {
+        const int nm = (int)g.size();
+        std::vector<fi_isos*> hs;
+        std::vector<double> times;
+        for (int q = 0; q < nm; ++q) {
+            fi_server::Slot& sl = s->slots[g[q]];
+            fi_isos* h = sl.h;
+            const fi_server::Req& r = sl.q.front();
+            if (!r.H.empty())
+                HIPCHK(hipMemcpyAsync(h->hm.H_in, r.H.data(), sizeof(double) * r.H.size(), hipMemcpyHostToDevice,
+                                      g_stream));
+            Member M = h->hm;
+            M.bsl = sl.b->d;
+            M.tf_req = r.time;
+            // domains of this group sharing one bsl object: batch order
+            M.bsl_lead = 1;
+            M.bsl_next = -1;
+            for (int p = q - 1; p >= 0; --p)
+                if (s->slots[g[p]].b == sl.b) { M.bsl_lead = 0; break; }
+            for (int p = q + 1; p < nm; ++p)
+                if (s->slots[g[p]].b == sl.b) { M.bsl_next = p; break; }
+            s->h_stage[base + q] = M;
+            hs.push_back(h);
+            times.push_back(r.time);
+        }
+        // the staged copies of earlier groups are still being read by queued copies;
+        // each group uses its own stage range, so no wait is needed
+        HIPCHK(hipMemcpyAsync(s->d_mem + base, s->h_stage + base, sizeof(Member) * nm, hipMemcpyHostToDevice,
+                              g_stream));
+        const fi_isos* h0 = hs[0];
+        auto sk = std::make_tuple(h0->n, (int)h0->fp64, (int)h0->uniform, nm);
+        auto it = s->shapes.find(sk);
+        if (it == s->shapes.end()) {
+            RhsShape shp{};
+            rc = make_shape(h0->n, nm, h0->fp64, h0->uniform, shp);
+            if (rc) return rc;
+            it = s->shapes.emplace(sk, shp).first;
+        }
+        rc = run_update(hs, s->d_mem + base, it->second, s->sb, 0.0, true, times.data());
+        if (rc) return rc;
+        for (int q = 0; q < nm; ++q) {
+            fi_server::Slot& sl = s->slots[g[q]];
+            s->done.push_back(sl.q.front().tag);
+            sl.q.pop_front();
+        }
+        *completed += nm;
+        base += nm;
+        ++s->groups;
+    }
+    ++s->rounds;
+    s->served += *completed;
+    return 0;
+}
+
+int fi_server_completed(fi_server* s, long long* tags, int max, int* n) {
+    if (!s || !n || (max > 0 && !tags)) return fail(FI_E_ARG, "NULL argument");
+    const int k = std::min<int>(max, (int)s->done.size());
+    for (int i = 0; i < k; ++i) tags[i] = s->done[i];
+    s->done.erase(s->done.begin(), s->done.begin() + k);
+    *n = k;
+    return 0;
+}
+
+int fi_server_stats(fi_server* s, long long* rounds, long long* groups, long long* served, double* hbm_used) {
+    if (!s) return fail(FI_E_ARG, "NULL argument");
+    if (rounds) *rounds = s->rounds;
+    if (groups) *groups = s->groups;
+    if (served) *served = s->served;
+    if (hbm_used) *hbm_used = (double)s->used;
+    return 0;
+}
+
+int fi_server_destroy(fi_server* s) {
+    if (!s) return 0;
+    (void)hipStreamSynchronize(g_stream);
+    if (s->d_mem) (void)hipFree(s->d_mem);
+    if (s->h_stage) (void)hipHostFree(s->h_stage);
+    sum_free(s->sb);
+    delete s;
+    return 0;
 }
 
 int fi_prof_select(int kernel) {

@@ -62,7 +62,7 @@ void launch_pre_bsl(const Member* d_mem, int nmem, int n, hipStream_t st);
 // phase 1: (after the elastic convolution) litho/full/mass anomaly
 // phase 2: (after the sea-surface convolution) z_ss/masks/loads/full
 void launch_pre_fields(const Member* d_mem, int nmem, int n, int phase, bool diag, bool fp64, hipStream_t st);
-void launch_begin(const Member* d_mem, int nmem, double tf, int stepping, hipStream_t st);
+void launch_begin(const Member* d_mem, int nmem, double tf, int stepping, bool per_member, hipStream_t st);
 void launch_finalize(const Member* d_mem, int nmem, int n, hipStream_t st);
 void launch_gather(const Member* d_mem, int nmem, CtlSummary* d_out, hipStream_t st);
 void launch_bsl_update(BslDev* b, double year_bp, hipStream_t st);

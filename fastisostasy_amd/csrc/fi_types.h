@@ -103,6 +103,7 @@ struct Member {
     // domains sharing one bsl object: the first in batch order leads and
     // applies the group's updates in order (next: batch index, -1 at the end)
     int bsl_lead, bsl_next;
+    double tf_req;               // per-domain target time of a serving round
 };
 
 }  // namespace fi

@@ -217,10 +217,11 @@ __global__ void __launch_bounds__(64) k_fin_check(const Member* __restrict__ mem
     }
 }
 
-__global__ void k_begin(const Member* __restrict__ mem, int nmem, double tf, int stepping) {
+__global__ void k_begin(const Member* __restrict__ mem, int nmem, double tf, int stepping, int per_member) {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= nmem) return;
     Ctl& c = *mem[m].ctl;
+    if (per_member) tf = mem[m].tf_req;
     c.tf = tf;
     c.k_valid = 0;
     c.status = (stepping && c.t < tf) ? ST_RUN : ST_IDLE;
@@ -301,8 +302,9 @@ void launch_pre_fields(const Member* d_mem, int nmem, int n, int phase, bool dia
     if (fp64) hipLaunchKernelGGL(k_pre_fields<double>, dim3(pre_blocks(n), nmem), dim3(PRE_THREADS), 0, st, d_mem, phase, (int)diag);
     else hipLaunchKernelGGL(k_pre_fields<float>, dim3(pre_blocks(n), nmem), dim3(PRE_THREADS), 0, st, d_mem, phase, (int)diag);
 }
-void launch_begin(const Member* d_mem, int nmem, double tf, int stepping, hipStream_t st) {
-    hipLaunchKernelGGL(k_begin, dim3((nmem + 63) / 64), dim3(64), 0, st, d_mem, nmem, tf, stepping);
+void launch_begin(const Member* d_mem, int nmem, double tf, int stepping, bool per_member, hipStream_t st) {
+    hipLaunchKernelGGL(k_begin, dim3((nmem + 63) / 64), dim3(64), 0, st, d_mem, nmem, tf, stepping,
+                       per_member ? 1 : 0);
 }
 void launch_finalize(const Member* d_mem, int nmem, int n, hipStream_t st) {
     hipLaunchKernelGGL(k_fin_corner, dim3((nmem + 63) / 64), dim3(64), 0, st, d_mem, nmem);

@@ -142,9 +142,9 @@ def lv_rheology():
                 dtt=10.0, dt_out=1e3, he_field=he, eta_field=eta)
 
 
-def ensemble_member(m, nx=256, dx=16e3, pad=0.0):
-    """Member m of the ensemble (seed 1000 + m)."""
-    rng = np.random.default_rng(1000 + m)
+def ensemble_member(m, nx=256, dx=16e3, pad=0.0, seed=None):
+    """Member m of the ensemble (seed 1000 + m unless given)."""
+    rng = np.random.default_rng(1000 + m if seed is None else seed)
     n = padded_size(nx, dx, pad)
     grf = gaussian_random_field(n, dx, 300e3, rng)
     log10_eta = np.clip(20.5 + 0.7 * grf, 19.0, 22.0)
@@ -158,6 +158,35 @@ def ensemble_member(m, nx=256, dx=16e3, pad=0.0):
                 z_bed=np.full((nx, nx), 1e6), H_init=np.zeros_like(H), t0=0.0, t1=10e3,
                 dtt=10.0, dt_out=1e3, he_field=np.full((n, n), he),
                 eta_field=(10.0 ** log10_eta)[None])
+
+
+SERVING_SIZES = (128, 256, 384, 512)
+SERVING_RATIO = (4, 3, 2, 1)
+
+
+def serving_domains(count=40, seed=5555):
+    """Resident domains of the serving workload: ice grids of 128, 256, 384 and
+    512 cells at 16 km in the ratio 4:3:2:1 (no padding, so N equals the ice
+    grid), each drawn like an ensemble member with seed `seed + k`."""
+    tot = sum(SERVING_RATIO)
+    sizes = []
+    for nx, r in zip(SERVING_SIZES, SERVING_RATIO):
+        sizes += [nx] * int(round(count * r / tot))
+    sizes = sizes[:count] + [SERVING_SIZES[0]] * max(0, count - len(sizes))
+    out = []
+    for k, nx in enumerate(sizes):
+        wl = ensemble_member(k, nx=nx, seed=seed + k)
+        wl["name"] = f"serving_{k}_{nx}"
+        out.append(wl)
+    return out
+
+
+def serving_requests(n_domains, count, rate, seed=5555):
+    """Request stream: Poisson arrivals at `rate` per second and a uniformly
+    drawn domain per request.  Returns (arrival seconds, domain index)."""
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, count))
+    return arrivals, rng.integers(0, n_domains, count)
 
 
 def sealevel_cycle(nx=384, dx=16e3, pad=1024e3, seed=4242):

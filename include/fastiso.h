@@ -36,7 +36,8 @@ enum fi_status {
     FI_E_HIP = -3,        /* HIP runtime error */
     FI_E_UNSTABLE = -4,   /* large_check threshold exceeded (fastisostasy.f90:874-899) */
     FI_E_NAN = -5,        /* NaN in z_bed (fastisostasy.f90:864-869) */
-    FI_E_STATE = -6       /* call sequence violated (e.g. update before init_state) */
+    FI_E_STATE = -6,      /* call sequence violated (e.g. update before init_state) */
+    FI_E_CAPACITY = -7    /* serving: the domain does not fit the HBM budget */
 };
 
 /* Precision of the RHS (stencil output, Hartley passes, spectral divide,
@@ -83,7 +84,8 @@ enum fi_field {
 
 enum fi_scalar {
     FI_S_N = 0, FI_S_T, FI_S_DT, FI_S_BSL, FI_S_DELTAV, FI_S_V_AF, FI_S_V_DEN,
-    FI_S_ATTEMPTS, FI_S_ACCEPTS, FI_S_D_LITH11, FI_S_ERR, FI_S_TOL
+    FI_S_ATTEMPTS, FI_S_ACCEPTS, FI_S_D_LITH11, FI_S_ERR, FI_S_TOL,
+    FI_S_HBM_BYTES          /* device memory held by the handle (bytes) */
 };
 
 typedef struct fi_isos fi_isos;
@@ -140,6 +142,32 @@ int fi_batch_create(fi_isos* const* members, int nmem, fi_batch** out);
 /* H_ice[m] may be NULL to keep member m's current ice thickness. */
 int fi_batch_update(fi_batch* b, const double* const* H_ice, double time, fi_bsl* const* bsl);
 int fi_batch_destroy(fi_batch* b);
+
+/* Serving (no reference counterpart; the batch / serving extension of the
+ * boundary): continuous batching of update requests over resident domains.
+ *   fi_server_create   budget for the device memory of admitted domains (bytes)
+ *                      and the largest number of domains per launch group
+ *   fi_server_admit    make an initialised domain resident, with the bsl object
+ *                      its updates use; FI_E_CAPACITY when its footprint
+ *                      (FI_S_HBM_BYTES) does not fit the remaining budget
+ *   fi_server_submit   queue one isos_update of a domain to `time` (H_ice may be
+ *                      NULL to keep the current ice); tags order requests
+ *   fi_server_run      one round: the oldest request of every domain with one,
+ *                      grouped by N, precision, options and diagnostics phase,
+ *                      one batched update per group; returns the count served
+ *   fi_server_completed  tags of served requests, in completion order
+ * A request's results are bitwise those of the same update run alone.
+ * Handles must not be updated through fi_isos_update while admitted. */
+typedef struct fi_server fi_server;
+int fi_server_create(double hbm_budget_bytes, int max_batch, fi_server** out);
+int fi_server_admit(fi_server* s, fi_isos* h, fi_bsl* b, int* slot);
+int fi_server_release(fi_server* s, int slot);
+int fi_server_submit(fi_server* s, int slot, const double* H_ice, double time, long long tag);
+int fi_server_pending(fi_server* s, int* n);
+int fi_server_run(fi_server* s, int* completed);
+int fi_server_completed(fi_server* s, long long* tags, int max, int* n);
+int fi_server_stats(fi_server* s, long long* rounds, long long* groups, long long* served, double* hbm_used);
+int fi_server_destroy(fi_server* s);
 
 /* Synchronise the library stream. */
 int fi_sync(void);

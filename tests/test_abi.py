@@ -79,3 +79,13 @@ def test_compute_fails_loudly_without_gpu():
     with pytest.raises(_lib.FiError) as e:
         G.IsosClass(W.params(), 32, 32, 10e3)
     assert e.value.code == _lib.FI_E_HIP
+
+
+def test_server_fails_loudly_without_gpu():
+    import ctypes
+    from fastisostasy_amd._lib import load
+    if _has_gpu():
+        pytest.skip("a GPU is present")
+    h = ctypes.c_void_p()
+    rc = load().fi_server_create(1e9, 8, ctypes.byref(h))
+    assert rc == -3 and b"no HIP device" in load().fi_last_error()

@@ -1,0 +1,81 @@
+"""GPU tests of the continuous-batching server (fi_server_*): results of a
+request served in a mixed batch are bitwise those of the same update run
+alone, and admission honours the HBM budget."""
+import numpy as np
+import pytest
+
+from fastisostasy_amd import isostasy as G
+from fastisostasy_amd import serving as S
+from fastisostasy_amd import workloads as W
+from fastisostasy_amd._lib import FiError
+
+pytestmark = pytest.mark.gpu
+
+FI_E_CAPACITY = -7
+
+
+def _domain(k, nx, prec):
+    wl = W.ensemble_member(k, nx=nx, dx=50e3, seed=7000 + k)
+    p = G.make_params(precision=prec, **wl["params"])
+    h = G.IsosClass(p, wl["nx"], wl["nx"], wl["dx"], he_field=wl["he_field"], eta_field=wl["eta_field"])
+    b = G.bsl_init("const", 0.0)
+    h.init_state(wl["z_bed"], np.zeros_like(wl["H_ice"]), 0.0, b)
+    return h, b, wl
+
+
+def test_server_mix_matches_single_updates():
+    """Six domains (N = 64 and 128 on the power-of-two path, N = 96 on the
+    generic path; FP64 and mixed) served through one server with random
+    interleaving reproduce single-domain isos_update calls bit for bit."""
+    spec = [(64, "fp64"), (64, "mixed"), (96, "fp64"), (96, "mixed"), (128, "mixed"), (128, "mixed")]
+    alone = [_domain(k, nx, p) for k, (nx, p) in enumerate(spec)]
+    served = [_domain(k, nx, p) for k, (nx, p) in enumerate(spec)]
+    srv = S.Server(hbm_budget_bytes=8e9, max_batch=64)
+    slots = [srv.admit(h, b) for h, b, _ in served]
+    rng = np.random.default_rng(3)
+    steps = 6
+    order = np.concatenate([np.full(steps, k) for k in range(len(spec))])
+    rng.shuffle(order)
+    t_now = [0.0] * len(spec)
+    tag = 0
+    for k in order:
+        t_now[k] += 10.0
+        first = t_now[k] == 10.0
+        srv.submit(slots[k], t_now[k], tag, H_ice=served[k][2]["H_ice"] if first else None)
+        tag += 1
+        if rng.random() < 0.4:
+            srv.run()
+    while srv.pending():
+        srv.run()
+    done = srv.completed()
+    assert sorted(done) == list(range(tag))
+    st = srv.stats()
+    assert st["served"] == tag and st["groups"] < tag   # some requests shared launches
+    for h, b, wl in alone:
+        for k in range(1, steps + 1):
+            h.update(wl["H_ice"], 10.0 * k, b)
+    for (a, _, _), (s, _, _) in zip(alone, served):
+        assert a.scalar("t") == s.scalar("t") == 10.0 * steps
+        assert a.scalar("attempts") == s.scalar("attempts")
+        for f in ("w", "z_bed", "dwdt"):
+            assert np.array_equal(a.out(f), s.out(f)), f
+    srv.close()
+
+
+def test_server_admission_budget():
+    doms = [_domain(k, 64, "mixed") for k in range(3)]
+    fp = doms[0][0].scalar("hbm_bytes")
+    assert fp > 0
+    srv = S.Server(hbm_budget_bytes=2.5 * fp, max_batch=8)
+    s0 = srv.admit(doms[0][0], doms[0][1])
+    s1 = srv.admit(doms[1][0], doms[1][1])
+    with pytest.raises(FiError) as e:
+        srv.admit(doms[2][0], doms[2][1])
+    assert e.value.code == FI_E_CAPACITY
+    srv.release(s0)
+    s2 = srv.admit(doms[2][0], doms[2][1])
+    assert srv.stats()["hbm_used"] == 2 * fp
+    srv.submit(s1, 10.0, 0, H_ice=doms[1][2]["H_ice"])
+    srv.submit(s2, 10.0, 1, H_ice=doms[2][2]["H_ice"])
+    assert srv.run() == 2 and sorted(srv.completed()) == [0, 1]
+    srv.close()
