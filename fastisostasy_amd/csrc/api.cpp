@@ -150,6 +150,8 @@ int make_shape(int n, int nmem, bool fp64, bool uniform, RhsShape& s) {
         }
         if (best_t && best_c) {
             s.fused = uniform && !std::getenv("FI_NO_FUSED_STENCIL");
+            s.row_tiles = 1;
+            if (const char* e = std::getenv("FI_ROW_TILES")) s.row_tiles = std::atoi(e) == 2 ? 2 : 1;
             s.threads_row = best_t;
             s.L = best_t / tpl;
             s.C = best_c;
@@ -357,12 +359,18 @@ int run_update(std::vector<fi_isos*>& hs, Member* d_mem, const RhsShape& shape, 
     launch_pre_bsl(d_mem, nmem, n, g_stream);
     if (!diag) {
         launch_pre_fields(d_mem, nmem, n, 0, false, h0.fp64, g_stream);
+    } else if (!h0.par.include_elastic && !h0.par.heterogeneous_ssh) {
+        launch_pre_fields(d_mem, nmem, n, 3, true, h0.fp64, g_stream);   // phases 0-2 in one pass
     } else {
         launch_pre_fields(d_mem, nmem, n, 0, true, h0.fp64, g_stream);
         if (h0.par.include_elastic) launch_convolution(d_mem, nmem, conv_shape_of(&h0), 0, g_stream);
-        launch_pre_fields(d_mem, nmem, n, 1, true, h0.fp64, g_stream);
-        if (h0.par.heterogeneous_ssh) launch_convolution(d_mem, nmem, conv_shape_of(&h0), 1, g_stream);
-        launch_pre_fields(d_mem, nmem, n, 2, true, h0.fp64, g_stream);
+        if (h0.par.heterogeneous_ssh) {
+            launch_pre_fields(d_mem, nmem, n, 1, true, h0.fp64, g_stream);
+            launch_convolution(d_mem, nmem, conv_shape_of(&h0), 1, g_stream);
+            launch_pre_fields(d_mem, nmem, n, 2, true, h0.fp64, g_stream);
+        } else {
+            launch_pre_fields(d_mem, nmem, n, 4, true, h0.fp64, g_stream);   // phases 1-2 in one pass
+        }
     }
     launch_begin(d_mem, nmem, time, stepping ? 1 : 0, times != nullptr, g_stream);
     HIPCHK(hipGetLastError());

@@ -20,6 +20,7 @@ struct RhsShape {
     bool uniform;     // K == 1
     bool fast;        // power-of-two N: register-FFT kernels (threads_row = block size)
     bool fused;       // fast && uniform: stencil fused into the forward row pass
+    int row_tiles;    // row blocks per thread block of the inverse row pass (1 or 2)
 };
 
 // Geometry of the 2N-point linear convolution.

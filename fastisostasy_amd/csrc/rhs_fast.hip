@@ -373,7 +373,11 @@ __device__ __forceinline__ void row_reduce_store(double a1a, double a2a, double 
     }
 }
 
-template <typename T, int N, int THREADS>
+// TILES row blocks per thread block: every load of all tiles is issued before
+// the first transform, so the later tiles' data streams in while the first
+// tile is transformed (the grid is otherwise a single occupancy round in
+// which all blocks load, then compute, in lockstep).
+template <typename T, int N, int THREADS, int TILES>
 __global__ void __launch_bounds__(THREADS) kf_row_inv(const Member* __restrict__ mem, const void* __restrict__ twv,
                                                       int ncolblk) {
     const Member& M = mem[blockIdx.y];
@@ -386,7 +390,9 @@ __global__ void __launch_bounds__(THREADS) kf_row_inv(const Member* __restrict__
     cplx<T>* tw_l = (cplx<T>*)smem;
     cplx<T>* buf = tw_l + N;
     const int tid = threadIdx.x, line = tid / TPL, t = tid % TPL;
-    const int ra = blockIdx.x * 2 * L + 2 * line, rb = ra + 1;
+    int ra[TILES];
+#pragma unroll
+    for (int k = 0; k < TILES; ++k) ra[k] = (blockIdx.x * TILES + k) * 2 * L + 2 * line;   // rb = ra + 1
     const int meth = ph->dt_method;
     const double dt = attempt_dt(meth, c->dt, c->t, c->tf, ph->dt_max);
     const int cur = c->cur;
@@ -394,8 +400,7 @@ __global__ void __launch_bounds__(THREADS) kf_row_inv(const Member* __restrict__
     gptr<double> X = gin<double>(M.X[cur]);
     gmut<double> Xn = gout<double>(M.X[cur ^ 1]);
     gmut<double> dw = gout<double>(M.dwdt);
-    // per-row max|x1|, max|x2|, sum (x2-x1)^2 of rows ra (a) and rb (b)
-    double a1a = 0.0, a2a = 0.0, s2a = 0.0, a1b = 0.0, a2b = 0.0, s2b = 0.0;
+    gmut<double> part = gout<double>(M.partials);
     auto update = [&](size_t cidx, double x, double k, double& a1, double& a2, double& s2) {
         double x1;
         const double x2 = stage_sums(meth, x, k, dt, x1);
@@ -405,36 +410,43 @@ __global__ void __launch_bounds__(THREADS) kf_row_inv(const Member* __restrict__
         const double d = x2 - x1;
         s2 += d * d;
     };
+    auto idx = [&](int r, int e) { return (size_t)r * N + t + TPL * e; };
     // All loads are issued before the first wait, in the order of use; X is
     // independent of the transform and is consumed last.
-    double xa[E], xb[E];
+    double xa[TILES][E], xb[TILES][E];
     if (k_valid) {
         // rejected attempt before: the RHS is unchanged, only dt differs
-        double ka[E], kb[E];
+        double ka[TILES][E], kb[TILES][E];
 #pragma unroll
-        for (int e = 0; e < E; ++e) {
-            ka[e] = dw[(size_t)ra * N + t + TPL * e];
-            kb[e] = dw[(size_t)rb * N + t + TPL * e];
-        }
+        for (int k = 0; k < TILES; ++k)
 #pragma unroll
-        for (int e = 0; e < E; ++e) {
-            xa[e] = X[(size_t)ra * N + t + TPL * e];
-            xb[e] = X[(size_t)rb * N + t + TPL * e];
-        }
+            for (int e = 0; e < E; ++e) {
+                ka[k][e] = dw[idx(ra[k], e)];
+                kb[k][e] = dw[idx(ra[k] + 1, e)];
+            }
 #pragma unroll
-        for (int e = 0; e < E; ++e) {
-            update((size_t)ra * N + t + TPL * e, xa[e], ka[e], a1a, a2a, s2a);
-            update((size_t)rb * N + t + TPL * e, xb[e], kb[e], a1b, a2b, s2b);
+        for (int k = 0; k < TILES; ++k)
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                xa[k][e] = X[idx(ra[k], e)];
+                xb[k][e] = X[idx(ra[k] + 1, e)];
+            }
+#pragma unroll
+        for (int k = 0; k < TILES; ++k) {
+            double a1a = 0.0, a2a = 0.0, s2a = 0.0, a1b = 0.0, a2b = 0.0, s2b = 0.0;
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                update(idx(ra[k], e), xa[k][e], ka[k][e], a1a, a2a, s2a);
+                update(idx(ra[k] + 1, e), xb[k][e], kb[k][e], a1b, a2b, s2b);
+            }
+            row_reduce_store<TPL>(a1a, a2a, s2a, a1b, a2b, s2b, line, t, part + (size_t)ra[k] * 4,
+                                  part + (size_t)(ra[k] + 1) * 4);
         }
     } else {
         __shared__ double qs[4];
         // N*N is a power of two here: the division by it is a multiplication
         const double S = (double)(float)((double)N * (double)N), rS = 1.0 / S;
         gptr<T> src = gin<T>(M.g1);
-        cplx<T> v[E], m[E];
-#pragma unroll
-        for (int e = 0; e < E; ++e)
-            v[e] = mk<T>(src[(size_t)ra * N + t + TPL * e], src[(size_t)rb * N + t + TPL * e]);
         constexpr int TWQ = (2 * N + THREADS - 1) / THREADS;
         T twr[TWQ];
         {
@@ -443,6 +455,9 @@ __global__ void __launch_bounds__(THREADS) kf_row_inv(const Member* __restrict__
             for (int k = 0; k < TWQ; ++k)
                 if (tid + THREADS * k < 2 * N) twr[k] = g[tid + THREADS * k];
         }
+        cplx<T> v[TILES][E], m[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) v[0][e] = mk<T>(src[idx(ra[0], e)], src[idx(ra[0] + 1, e)]);
         if (tid < 64) {
             // the four corner sums over the N per-column terms of the column
             // pass: lane-strided, then a fixed butterfly (order set by N alone)
@@ -457,37 +472,48 @@ __global__ void __launch_bounds__(THREADS) kf_row_inv(const Member* __restrict__
             if (tid == 0) { qs[0] = q0; qs[1] = q1; qs[2] = q2; qs[3] = q3; }
         }
 #pragma unroll
-        for (int e = 0; e < E; ++e) {
-            xa[e] = X[(size_t)ra * N + t + TPL * e];
-            xb[e] = X[(size_t)rb * N + t + TPL * e];
-        }
+        for (int k = 1; k < TILES; ++k)
+#pragma unroll
+            for (int e = 0; e < E; ++e) v[k][e] = mk<T>(src[idx(ra[k], e)], src[idx(ra[k] + 1, e)]);
+#pragma unroll
+        for (int k = 0; k < TILES; ++k)
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                xa[k][e] = X[idx(ra[k], e)];
+                xb[k][e] = X[idx(ra[k] + 1, e)];
+            }
 #pragma unroll
         for (int k = 0; k < TWQ; ++k)
             if (tid + THREADS * k < 2 * N) ((T*)tw_l)[tid + THREADS * k] = twr[k];
         __syncthreads();
-        RowXch<T, N> xc(buf, line);
-        reg_fft<T, N>(v, t, tw_l, xc);
-        reg_mirror<T, N>(v, m, t, xc);
         // x(1,1) + x(nx,ny) + x(1,ny) + x(nx,1)  (isos_utils.f90:116)
         const double cc = 0.25 * (qs[0] * rS + qs[3] * rS + qs[2] * rS + qs[1] * rS);
         const double spy = ph->sec_per_year;
         const bool store_k = !(c->t + dt < c->tf);
+        RowXch<T, N> xc(buf, line);
 #pragma unroll
-        for (int e = 0; e < E; ++e) {
-            T ha, hb;
-            dht_pair(v[e], m[e], ha, hb);
-            const size_t ca = (size_t)ra * N + t + TPL * e, cb = (size_t)rb * N + t + TPL * e;
-            const double ka = ((double)ha * rS - cc) * spy, kb = ((double)hb * rS - cc) * spy;
-            if (store_k) {
-                dw[ca] = ka;
-                dw[cb] = kb;
+        for (int k = 0; k < TILES; ++k) {
+            if (k > 0) __syncthreads();   // the exchange buffer is reused
+            reg_fft<T, N>(v[k], t, tw_l, xc);
+            reg_mirror<T, N>(v[k], m, t, xc);
+            double a1a = 0.0, a2a = 0.0, s2a = 0.0, a1b = 0.0, a2b = 0.0, s2b = 0.0;
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                T ha, hb;
+                dht_pair(v[k][e], m[e], ha, hb);
+                const size_t ca = idx(ra[k], e), cb = idx(ra[k] + 1, e);
+                const double ka = ((double)ha * rS - cc) * spy, kb = ((double)hb * rS - cc) * spy;
+                if (store_k) {
+                    dw[ca] = ka;
+                    dw[cb] = kb;
+                }
+                update(ca, xa[k][e], ka, a1a, a2a, s2a);
+                update(cb, xb[k][e], kb, a1b, a2b, s2b);
             }
-            update(ca, xa[e], ka, a1a, a2a, s2a);
-            update(cb, xb[e], kb, a1b, a2b, s2b);
+            row_reduce_store<TPL>(a1a, a2a, s2a, a1b, a2b, s2b, line, t, part + (size_t)ra[k] * 4,
+                                  part + (size_t)(ra[k] + 1) * 4);
         }
     }
-    row_reduce_store<TPL>(a1a, a2a, s2a, a1b, a2b, s2b, line, t, gout<double>(M.partials) + (size_t)ra * 4,
-                          gout<double>(M.partials) + (size_t)rb * 4);
 }
 
 // ------------------------------------------------------------------ dispatch
@@ -515,10 +541,13 @@ static bool srow_fwd(const Member* d, int nmem, hipStream_t st, const void* tw) 
     return false;
 }
 template <typename T, int N, int THREADS>
-static void rows_inv(const Member* d, int nmem, hipStream_t st, const void* tw, int ncolblk) {
+static void rows_inv(const Member* d, int nmem, hipStream_t st, const void* tw, int ncolblk, int tiles) {
     constexpr int L = THREADS / (N / 8);
     const size_t lds = row_lds<T, N>(THREADS);
-    launch_k(3, kf_row_inv<T, N, THREADS>, dim3(N / (2 * L), nmem), dim3(THREADS), lds, st, d, tw, ncolblk);
+    if (tiles == 2 && (N / (2 * L)) % 2 == 0)
+        launch_k(3, kf_row_inv<T, N, THREADS, 2>, dim3(N / (4 * L), nmem), dim3(THREADS), lds, st, d, tw, ncolblk);
+    else
+        launch_k(3, kf_row_inv<T, N, THREADS, 1>, dim3(N / (2 * L), nmem), dim3(THREADS), lds, st, d, tw, ncolblk);
 }
 template <typename T, int N, int C>
 static void cols(const Member* d, int nmem, hipStream_t st, const void* tw, const double2* tw64) {
@@ -549,9 +578,9 @@ static bool fast_n(const Member* d, int nmem, const RhsShape& s, hipStream_t st,
         return false;
     }
     switch (s.threads_row) {
-    case 64: if constexpr (N / 8 <= 64) { which == 1 ? rows_fwd<T, N, 64>(d, nmem, st, s.tw) : rows_inv<T, N, 64>(d, nmem, st, s.tw, ncolblk); return true; } break;
-    case 128: if constexpr (N / 8 <= 128) { which == 1 ? rows_fwd<T, N, 128>(d, nmem, st, s.tw) : rows_inv<T, N, 128>(d, nmem, st, s.tw, ncolblk); return true; } break;
-    case 256: if constexpr (N / 8 <= 256) { which == 1 ? rows_fwd<T, N, 256>(d, nmem, st, s.tw) : rows_inv<T, N, 256>(d, nmem, st, s.tw, ncolblk); return true; } break;
+    case 64: if constexpr (N / 8 <= 64) { which == 1 ? rows_fwd<T, N, 64>(d, nmem, st, s.tw) : rows_inv<T, N, 64>(d, nmem, st, s.tw, ncolblk, s.row_tiles); return true; } break;
+    case 128: if constexpr (N / 8 <= 128) { which == 1 ? rows_fwd<T, N, 128>(d, nmem, st, s.tw) : rows_inv<T, N, 128>(d, nmem, st, s.tw, ncolblk, s.row_tiles); return true; } break;
+    case 256: if constexpr (N / 8 <= 256) { which == 1 ? rows_fwd<T, N, 256>(d, nmem, st, s.tw) : rows_inv<T, N, 256>(d, nmem, st, s.tw, ncolblk, s.row_tiles); return true; } break;
     }
     return false;
 }

@@ -15,6 +15,7 @@
 // All bookkeeping is FP64; masks are bit-packed in one byte per cell.
 #include "fi_device.hpp"
 #include "fi_kernels.h"
+#include "rhs_common.hpp"
 
 namespace fi {
 
@@ -50,27 +51,36 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 
 __global__ void __launch_bounds__(PRE_THREADS) k_pre_haf(const Member* __restrict__ mem, int new_H) {
     const Member& M = mem[blockIdx.y];
-    const PhysDev& ph = *M.phys;
+    const kptr<PhysDev> ph = kview(M.phys);
     const int N = M.n;
     const size_t nn = (size_t)N * N;
     double s1 = 0.0, s2 = 0.0;
-    const double A_uni = ph.dx * ph.dx;
-    for (int q = threadIdx.x; q < PRE_CELLS; q += PRE_THREADS) {
-        const size_t c = (size_t)blockIdx.x * PRE_CELLS + q;
+    const double A_uni = ph->dx * ph->dx, Vden = ph->Vden_factor, rsw_ri = ph->rho_seawater / ph->rho_ice;
+    const bool uniform = ph->uniform_K;
+    gptr<double> Hin = gin<double>(M.H_in), zss = gin<double>(M.z_ss), zbed = gin<double>(M.z_bed),
+                 Haf_ref = gin<double>(M.Haf_ref), Hice_ref = gin<double>(M.Hice_ref);
+    gmut<double> Hice = gout<double>(M.Hice), Haf = gout<double>(M.Haf);
+#pragma unroll
+    for (int it = 0; it < PRE_CELLS / PRE_THREADS; ++it) {
+        const size_t c = (size_t)blockIdx.x * PRE_CELLS + it * PRE_THREADS + threadIdx.x;
         if (c >= nn) break;
         double H;
         if (new_H) {
             const int i = (int)(c % N) - M.icrop1, j = (int)(c / N) - M.jcrop1;
-            H = (i >= 0 && i < M.nx_ice && j >= 0 && j < M.ny_ice) ? M.H_in[(size_t)j * M.nx_ice + i] : 0.0;
-            M.Hice[c] = H;
+            H = (i >= 0 && i < M.nx_ice && j >= 0 && j < M.ny_ice) ? Hin[(size_t)j * M.nx_ice + i] : 0.0;
+            Hice[c] = H;
         } else {
-            H = M.Hice[c];
+            H = Hice[c];
         }
-        const double haf = calc_haf(M.z_ss[c], M.z_bed[c], H, ph);
-        M.Haf[c] = haf;
-        const double A = ph.uniform_K ? A_uni : M.A[c];
-        s1 += (haf - M.Haf_ref[c]) * A;
-        s2 += (H - M.Hice_ref[c]) * ph.Vden_factor * A;
+        // calc_Haf (sealevel.f90:132-145)
+        double h = (zss[c] - zbed[c]) * rsw_ri;
+        if (h < 0.0) h = 0.0;
+        h = H - h;
+        if (h < 0.0) h = 0.0;
+        Haf[c] = h;
+        const double A = uniform ? A_uni : M.A[c];
+        s1 += (h - Haf_ref[c]) * A;
+        s2 += (H - Hice_ref[c]) * Vden * A;
     }
     __shared__ double red[16];
     s1 = block_sum(s1, red);
@@ -106,54 +116,95 @@ __global__ void __launch_bounds__(64) k_pre_bsl(const Member* __restrict__ mem, 
 }
 
 // phase 0: first pass after the BSL update; phase 1: after the elastic
-// convolution; phase 2: after the sea-surface convolution.
+// convolution; phase 2: after the sea-surface convolution.  Without the
+// sea-surface convolution phases 1 and 2 run as one pass (phase 4), and
+// without either convolution all three do (phase 3); the fused passes apply
+// the same operations to each cell in the same order.
 template <typename T>
 __global__ void __launch_bounds__(PRE_THREADS) k_pre_fields(const Member* __restrict__ mem, int phase, int diag) {
     const Member& M = mem[blockIdx.y];
-    const PhysDev& ph = *M.phys;
-    const Ctl& ctl = *M.ctl;
+    const kptr<PhysDev> ph = kview(M.phys);
     const int N = M.n;
     const size_t nn = (size_t)N * N;
-    const double bsl = ctl.bsl;
-    for (int q = threadIdx.x; q < PRE_CELLS; q += PRE_THREADS) {
-        const size_t c = (size_t)blockIdx.x * PRE_CELLS + q;
+    const double bsl = kview(M.ctl)->bsl;
+    const double rho_ice = ph->rho_ice, rho_water = ph->rho_water, rho_litho = ph->rho_litho, g = ph->g;
+    const bool uniform = ph->uniform_K, interactive = ph->interactive_sealevel, elastic = ph->include_elastic,
+               hssh = ph->heterogeneous_ssh;
+    const double A_uni = ph->dx * ph->dx;
+    gmut<uint8_t> mask = gout<uint8_t>(M.mask);
+    gmut<double> zss = gout<double>(M.z_ss), dzss = gout<double>(M.dz_ss), we = gout<double>(M.we),
+                 cload = gout<double>(M.canom_load), cfull = gout<double>(M.canom_full),
+                 conv = gout<double>(M.conv_in);
+    gptr<double> zss_ref = gin<double>(M.z_ss_ref), zbed = gin<double>(M.z_bed), Hice = gin<double>(M.Hice),
+                 Haf = gin<double>(M.Haf), Haf_ref = gin<double>(M.Haf_ref), we_ref = gin<double>(M.we_ref),
+                 cmantle = gin<double>(M.canom_mantle);
+    gmut<T> cf_rhs = gout<T>(M.cfull);
+    // calc_z_ss, calc_masks, seawater and load anomalies for one cell
+    auto sea_load = [&](size_t c, uint8_t mold, bool active, double dz, uint8_t& mk) {
+        const double z_ss = bsl + zss_ref[c] + dz;
+        const double haf = Haf[c];
+        mk = calc_mask(zbed[c], z_ss, Hice[c], haf, mold);
+        zss[c] = z_ss;
+        mask[c] = mk;
+        const double ice = 0.0 + rho_ice * (haf - Haf_ref[c]);
+        const double sw = 0.0 + ((mk & MB_OCEAN) ? rho_water * (z_ss - zss_ref[c]) : 0.0);
+        double load = ice;
+        if (interactive) load = load + sw;
+        return active ? load : 0.0;
+    };
+    auto full_of = [&](size_t c, bool active, double load, double wec) {
+        const double litho = 0.0 + rho_litho * (wec - we_ref[c]);
+        const double full = active ? (load + litho + cmantle[c]) : 0.0;
+        cfull[c] = full;
+        cf_rhs[c] = (T)full;
+    };
+    // phase 1 alone: canom_litho, canom_full, mass_anom (fastisostasy.f90:811-813)
+    auto phase1 = [&](size_t c, bool active, double wec) {
+        const double litho = 0.0 + rho_litho * (wec - we_ref[c]);
+        const double full = active ? (cload[c] + litho + cmantle[c]) : 0.0;
+        const double A = uniform ? A_uni : M.A[c];
+        if (hssh) conv[c] = active ? A * full : 0.0;
+        else dzss[c] = 0.0;
+    };
+#pragma unroll
+    for (int it = 0; it < PRE_CELLS / PRE_THREADS; ++it) {
+        const size_t c = (size_t)blockIdx.x * PRE_CELLS + it * PRE_THREADS + threadIdx.x;
         if (c >= nn) break;
-        const uint8_t mold = M.mask[c];
+        const uint8_t mold = mask[c];
         const bool active = mold & MB_ACTIVE;
+        uint8_t mk;
         if (phase == 1) {
-            // canom_litho, canom_full, mass_anom (fastisostasy.f90:811-813)
-            const double litho = 0.0 + ph.rho_litho * (M.we[c] - M.we_ref[c]);
-            const double full = active ? (M.canom_load[c] + litho + M.canom_mantle[c]) : 0.0;
-            const double A = ph.uniform_K ? ph.dx * ph.dx : M.A[c];
-            if (ph.heterogeneous_ssh) M.conv_in[c] = active ? A * full : 0.0;
-            else M.dz_ss[c] = 0.0;
+            phase1(c, active, we[c]);
             continue;
         }
-        // calc_z_ss, calc_masks, seawater and load anomalies
-        const double z_ss = bsl + M.z_ss_ref[c] + M.dz_ss[c];
-        const double z_bed = M.z_bed[c], H = M.Hice[c], haf = M.Haf[c];
-        const uint8_t mk = calc_mask(z_bed, z_ss, H, haf, mold);
-        M.z_ss[c] = z_ss;
-        M.mask[c] = mk;
-        const double ice = 0.0 + ph.rho_ice * (haf - M.Haf_ref[c]);
-        const double sw = 0.0 + ((mk & MB_OCEAN) ? ph.rho_water * (z_ss - M.z_ss_ref[c]) : 0.0);
-        double load = ice;
-        if (ph.interactive_sealevel) load = load + sw;
-        load = active ? load : 0.0;
+        if (phase == 4) {
+            // phases 1 and 2 without the sea-surface convolution: dz_ss = 0
+            dzss[c] = 0.0;
+            const double load = sea_load(c, mold, active, 0.0, mk);
+            full_of(c, active, load, we[c]);
+            continue;
+        }
+        const double load = sea_load(c, mold, active, dzss[c], mk);
+        if (phase == 3) {
+            // phases 0, 1 and 2 without convolutions: we = 0, dz_ss = 0
+            cload[c] = load;
+            we[c] = 0.0;
+            dzss[c] = 0.0;
+            const double load2 = sea_load(c, mk, active, 0.0, mk);
+            full_of(c, active, load2, 0.0);
+            continue;
+        }
         if (phase == 0 && diag) {
-            M.canom_load[c] = load;
-            if (ph.include_elastic) {
-                const double K2 = ph.uniform_K ? 1.0 : M.K2[c];
-                M.conv_in[c] = load * ph.g * K2;
+            cload[c] = load;
+            if (elastic) {
+                const double K2 = uniform ? 1.0 : M.K2[c];
+                conv[c] = load * g * K2;
             } else {
-                M.we[c] = 0.0;
+                we[c] = 0.0;
             }
             continue;
         }
-        const double litho = 0.0 + ph.rho_litho * (M.we[c] - M.we_ref[c]);
-        const double full = active ? (load + litho + M.canom_mantle[c]) : 0.0;
-        M.canom_full[c] = full;
-        ((T*)M.cfull)[c] = (T)full;
+        full_of(c, active, load, we[c]);
     }
 }
 
@@ -170,25 +221,29 @@ __global__ void k_fin_corner(const Member* __restrict__ mem, int nmem) {
 
 __global__ void __launch_bounds__(PRE_THREADS) k_finalize(const Member* __restrict__ mem) {
     const Member& M = mem[blockIdx.y];
-    const PhysDev& ph = *M.phys;
-    const Ctl& ctl = *M.ctl;
+    const kptr<PhysDev> ph = kview(M.phys);
+    const kptr<Ctl> ctl = kview(M.ctl);
     const int N = M.n;
     const size_t nn = (size_t)N * N;
-    const double* __restrict__ wsrc = ctl.w_is_x ? M.X[ctl.cur] : M.W;
-    const double cw = ctl.wcorner;
+    gptr<double> wsrc = gin<double>(ctl->w_is_x ? M.X[ctl->cur] : M.W);
+    const double cw = ctl->wcorner, rho_um = ph->rho_uppermantle;
+    gptr<double> w_ref = gin<double>(M.w_ref), wev = gin<double>(M.we), zb_ref = gin<double>(M.z_bed_ref),
+                 dz = gin<double>(M.dz_ss);
+    gmut<double> W = gout<double>(M.W), cm = gout<double>(M.canom_mantle), zbed = gout<double>(M.z_bed);
     double mw = 0.0, mwe = 0.0, mdz = 0.0, nanf = 0.0;
-    for (int q = threadIdx.x; q < PRE_CELLS; q += PRE_THREADS) {
-        const size_t c = (size_t)blockIdx.x * PRE_CELLS + q;
+#pragma unroll
+    for (int it = 0; it < PRE_CELLS / PRE_THREADS; ++it) {
+        const size_t c = (size_t)blockIdx.x * PRE_CELLS + it * PRE_THREADS + threadIdx.x;
         if (c >= nn) break;
         const double w = wsrc[c] - cw;
-        M.W[c] = w;
-        M.canom_mantle[c] = 0.0 + ph.rho_uppermantle * (w - M.w_ref[c]);
-        const double we = M.we[c];
-        const double zb = M.z_bed_ref[c] + w + we;
-        M.z_bed[c] = zb;
+        W[c] = w;
+        cm[c] = 0.0 + rho_um * (w - w_ref[c]);
+        const double we = wev[c];
+        const double zb = zb_ref[c] + w + we;
+        zbed[c] = zb;
         mw = fmax(mw, fabs(w));
         mwe = fmax(mwe, fabs(we));
-        mdz = fmax(mdz, fabs(M.dz_ss[c]));
+        mdz = fmax(mdz, fabs(dz[c]));
         if (zb != zb || w != w) nanf = 1.0;
     }
     __shared__ double red[4][16];
