@@ -238,6 +238,27 @@ def run_cpu_baseline(seconds):
     return dict(value=n * n * 10.0 * steps / wall, steps=steps, wall=wall, n=n)
 
 
+def copy_bandwidth(mib=1024, reps=20):
+    """Achieved device-to-device copy bandwidth (read + write bytes / s) of a
+    plain copy of `mib` MiB, for context beside the HBM peak."""
+    import torch
+    n = mib * (1 << 20) // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        y.copy_(x)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del x, y
+    torch.cuda.empty_cache()
+    return 2.0 * n * 4 / (ms * 1e-3) / 1e9
+
+
 def calibrate_copy(mib):
     """A device-to-device copy of known size, for scaling the FETCH_SIZE /
     WRITE_SIZE counters of rocprofv3 (not part of the benchmark)."""
@@ -312,7 +333,8 @@ def main():
         roof = {"kernel": name, "bound": "hbm", "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": traffic, "avg_launch_us": round(avg_s * 1e6, 2), "launches": cnt,
-                "bytes_per_launch": bytes_per_launch}
+                "bytes_per_launch": bytes_per_launch,
+                "copy_GBs": round(copy_bandwidth(), 1)}
 
     serve = None
     if args.serve_requests > 0:

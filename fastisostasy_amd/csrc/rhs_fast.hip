@@ -157,6 +157,8 @@ __global__ void __launch_bounds__(THREADS) kf_srow_fwd(const Member* __restrict_
                 cv[m - 2][q] = Cg[(size_t)(R0 + m - 2) * N + col];
                 ev[m - 2][q] = Eg[(size_t)(R0 + m - 2) * N + col];
             }
+            // keep the issue order: the march waits on each row group in turn
+            __builtin_amdgcn_sched_barrier(0);
         }
 #pragma unroll
     for (int k = 0; k < TWQ; ++k)
