@@ -109,12 +109,20 @@ def run_batched(nmem, steps, warmup, precision, prof_kernel=None):
     if prof_kernel is not None:
         # Kernel timing: the same number of steps again, continuing the run,
         # with HIP events on every launch of one kernel (the events slow the
-        # stream, so they stay out of the region timed above).
+        # stream, so they stay out of the region timed above), and the whole
+        # batch in one launch per kernel (no concurrent sub-batches), so a
+        # launch covers every member and runs alone on the GPU.
+        saved = os.environ.get("FI_SUBBATCHES")
+        os.environ["FI_SUBBATCHES"] = "1"
         check(load().fi_prof_select(prof_kernel))
         for _ in range(steps):
             t += 10.0
             batch.update(t, bsls)
         check(load().fi_sync())
+        if saved is None:
+            del os.environ["FI_SUBBATCHES"]
+        else:
+            os.environ["FI_SUBBATCHES"] = saved
         ms, cnt = ctypes.c_double(), ctypes.c_longlong()
         check(load().fi_prof_collect(ctypes.byref(ms), ctypes.byref(cnt)))
         check(load().fi_prof_select(-1))
@@ -240,23 +248,12 @@ def run_cpu_baseline(seconds):
 
 def copy_bandwidth(mib=1024, reps=20):
     """Achieved device-to-device copy bandwidth (read + write bytes / s) of a
-    plain copy of `mib` MiB, for context beside the HBM peak."""
-    import torch
-    n = mib * (1 << 20) // 4
-    x = torch.ones(n, dtype=torch.float32, device="cuda")
-    y = torch.empty_like(x)
-    for _ in range(3):
-        y.copy_(x)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(reps):
-        y.copy_(x)
-    e1.record()
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / reps
-    del x, y
-    torch.cuda.empty_cache()
-    return 2.0 * n * 4 / (ms * 1e-3) / 1e9
+    plain copy of `mib` MiB on the library stream, beside the HBM peak."""
+    import ctypes
+    from fastisostasy_amd._lib import check, load
+    v = ctypes.c_double()
+    check(load().fi_bench_copy(float(mib), int(reps), ctypes.byref(v)))
+    return v.value
 
 
 def calibrate_copy(mib):

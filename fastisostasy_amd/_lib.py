@@ -28,7 +28,7 @@ EXPORTS = ["fi_last_error", "fi_version", "fi_padded_size", "fi_isos_create", "f
            "fi_isos_init_state", "fi_isos_update", "fi_isos_get", "fi_isos_get_mask", "fi_isos_get_full",
            "fi_isos_get_scalar", "fi_isos_destroy", "fi_bsl_create", "fi_bsl_update", "fi_bsl_get",
            "fi_bsl_destroy", "fi_batch_create", "fi_batch_update", "fi_batch_destroy", "fi_sync",
-           "fi_prof_select", "fi_prof_collect", "fi_prof_fused",
+           "fi_prof_select", "fi_prof_collect", "fi_prof_fused", "fi_bench_copy",
            "fi_server_create", "fi_server_admit", "fi_server_release", "fi_server_submit", "fi_server_pending",
            "fi_server_run", "fi_server_completed", "fi_server_stats", "fi_server_destroy"]
 
@@ -92,6 +92,7 @@ def load():
     L.fi_prof_select.argtypes = [i]
     L.fi_prof_collect.argtypes = [dp, ctypes.POINTER(ctypes.c_longlong)]
     L.fi_prof_fused.argtypes = []
+    L.fi_bench_copy.argtypes = [d, i, dp]
     ll = ctypes.c_longlong
     L.fi_server_create.argtypes = [d, i, ctypes.POINTER(vp)]
     L.fi_server_admit.argtypes = [vp, vp, vp, ctypes.POINTER(i)]

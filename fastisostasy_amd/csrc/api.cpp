@@ -337,6 +337,55 @@ int gather(Member* d_mem, int nmem, SumBuf& sb) {
     return 0;
 }
 
+// Concurrent sub-batches of the attempt loop.  One launch of an attempt
+// kernel over a large batch is a single occupancy round in which all blocks
+// load, then compute, in step; sub-batches on separate streams (default
+// priority) drift apart, so one sub-batch's transforms overlap another's
+// loads.  Results do not depend on the split (reductions are fixed by N).
+constexpr int MAX_SUB = 4;
+struct SubStreams {
+    hipStream_t s[MAX_SUB] = {};
+    hipEvent_t done[MAX_SUB] = {};
+    hipEvent_t start = nullptr;
+    int n = 0;
+};
+SubStreams g_sub;
+
+int ensure_sub(int k) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!g_sub.start) HIPCHK(hipEventCreateWithFlags(&g_sub.start, hipEventDisableTiming));
+    for (; g_sub.n < k; ++g_sub.n) {
+        HIPCHK(hipStreamCreateWithFlags(&g_sub.s[g_sub.n], hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&g_sub.done[g_sub.n], hipEventDisableTiming));
+    }
+    return 0;
+}
+
+// number of sub-batches for a batch of nmem domains (FI_SUBBATCHES overrides)
+int sub_count(int nmem) {
+    int k = nmem >= 32 ? 2 : 1;
+    if (const char* e = std::getenv("FI_SUBBATCHES")) k = std::atoi(e);
+    return std::max(1, std::min({k, MAX_SUB, nmem}));
+}
+
+std::map<std::tuple<int, int, int, int>, RhsShape> g_sub_shapes;
+
+int sub_shape(int n, int nsub, bool fp64, bool uniform, const RhsShape*& out) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto key = std::make_tuple(n, nsub, (int)fp64, (int)uniform);
+    auto it = g_sub_shapes.find(key);
+    if (it == g_sub_shapes.end()) {
+        RhsShape s{};
+        g_mu.unlock();
+        int rc = make_shape(n, nsub, fp64, uniform, s);
+        g_mu.lock();
+        if (rc) return rc;
+        it = g_sub_shapes.emplace(key, s).first;
+    }
+    out = &it->second;
+    return 0;
+}
+
 // Diagnostics cadence of one isos_update call (fastisostasy.f90:782-787): a
 // function of the call times only.
 bool diag_due(const fi_isos* h, double time) { return time - h->time_diag >= (double)1e-3f; }
@@ -389,8 +438,32 @@ int run_update(std::vector<fi_isos*>& hs, Member* d_mem, const RhsShape& shape, 
             ahead = (int)std::min(256.0, std::max(1.0, need));
         }
         long long total = 0;
+        const int K = sub_count(nmem);
+        int sb_lo[MAX_SUB + 1];
+        const RhsShape* sshape[MAX_SUB] = {};
+        if (K > 1) {
+            int rc = ensure_sub(K);
+            if (rc) return rc;
+            for (int k = 0; k <= K; ++k) sb_lo[k] = (int)((long long)nmem * k / K);
+            for (int k = 0; k < K; ++k) {
+                rc = sub_shape(n, sb_lo[k + 1] - sb_lo[k], h0.fp64, h0.uniform, sshape[k]);
+                if (rc) return rc;
+            }
+        }
         for (;;) {
-            for (int a = 0; a < ahead; ++a) launch_attempt(d_mem, nmem, shape, g_stream);
+            if (K == 1) {
+                for (int a = 0; a < ahead; ++a) launch_attempt(d_mem, nmem, shape, g_stream);
+            } else {
+                HIPCHK(hipEventRecord(g_sub.start, g_stream));
+                for (int k = 0; k < K; ++k) HIPCHK(hipStreamWaitEvent(g_sub.s[k], g_sub.start, 0));
+                for (int a = 0; a < ahead; ++a)
+                    for (int k = 0; k < K; ++k)
+                        launch_attempt(d_mem + sb_lo[k], sb_lo[k + 1] - sb_lo[k], *sshape[k], g_sub.s[k]);
+                for (int k = 0; k < K; ++k) {
+                    HIPCHK(hipEventRecord(g_sub.done[k], g_sub.s[k]));
+                    HIPCHK(hipStreamWaitEvent(g_stream, g_sub.done[k], 0));
+                }
+            }
             HIPCHK(hipGetLastError());
             total += ahead;
             int rc = gather(d_mem, nmem, sb);
@@ -1222,6 +1295,33 @@ int fi_prof_collect(double* total_ms, long long* launches) {
 }
 
 int fi_prof_fused(void) { return prof_fused(); }
+
+int fi_bench_copy(double mib, int reps, double* gbs) {
+    int rc = ensure_device();
+    if (rc) return rc;
+    if (!gbs || mib <= 0 || reps < 1) return fail(FI_E_ARG, "bad copy benchmark parameters");
+    const size_t bytes = (size_t)(mib * 1048576.0);
+    void *a = nullptr, *b = nullptr;
+    HIPCHK(hipMalloc(&a, bytes));
+    if (hipMalloc(&b, bytes) != hipSuccess) { (void)hipFree(a); return fail(FI_E_HIP, "hipMalloc"); }
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipMemsetAsync(a, 0, bytes, g_stream));
+    for (int i = 0; i < 2; ++i) HIPCHK(hipMemcpyAsync(b, a, bytes, hipMemcpyDeviceToDevice, g_stream));
+    HIPCHK(hipEventRecord(e0, g_stream));
+    for (int i = 0; i < reps; ++i) HIPCHK(hipMemcpyAsync(b, a, bytes, hipMemcpyDeviceToDevice, g_stream));
+    HIPCHK(hipEventRecord(e1, g_stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    *gbs = 2.0 * (double)bytes * reps / (ms * 1e-3) / 1e9;
+    return 0;
+}
 
 int fi_batch_destroy(fi_batch* b) {
     if (!b) return 0;

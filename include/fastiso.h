@@ -183,6 +183,10 @@ int fi_prof_collect(double* total_ms, long long* launches);
 /* 1 when the last attempt ran the stencil fused into the forward row pass
  * (uniform spacing, power-of-two N); kernel 0 then times the fused kernel. */
 int fi_prof_fused(void);
+/* Achieved device-to-device copy bandwidth (bytes read + written per second /
+ * 1e9) of `reps` copies of `mib` MiB on the library stream: the practical
+ * ceiling beside the HBM peak. */
+int fi_bench_copy(double mib, int reps, double* gbs);
 
 #ifdef __cplusplus
 }

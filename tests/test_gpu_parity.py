@@ -274,3 +274,32 @@ def test_results_identical_alone_and_in_batch(prec, monkeypatch):
     assert alone.scalar("attempts") == h.scalar("attempts") > 5
     for f in ("w", "dwdt", "z_bed"):
         assert np.array_equal(alone.out(f), h.out(f)), (f, np.abs(alone.out(f) - h.out(f)).max())
+
+
+def test_sub_batches_match_one_launch(monkeypatch):
+    """A batch split into concurrent sub-batches on several streams gives the
+    same bits as one launch over the whole batch."""
+    members = [W.ensemble_member(m, nx=64, dx=50e3) for m in range(32)]
+
+    def run(k):
+        monkeypatch.setenv("FI_SUBBATCHES", str(k))
+        hs, bs = [], []
+        for wl in members:
+            p = G.make_params(precision="mixed", **wl["params"])
+            h = G.IsosClass(p, wl["nx"], wl["nx"], wl["dx"], he_field=wl["he_field"], eta_field=wl["eta_field"])
+            b = G.bsl_init()
+            h.init_state(wl["z_bed"], np.zeros_like(wl["H_ice"]), 0.0, b)
+            hs.append(h)
+            bs.append(b)
+        batch = G.Batch(hs)
+        for k_ in range(1, 4):
+            batch.update(10.0 * k_, bs, H_ice=[wl["H_ice"] for wl in members] if k_ == 1 else None)
+        out = [h.out("w") for h in hs]
+        batch.close()
+        for h in hs:
+            h.close()
+        return out
+
+    one, four = run(1), run(4)
+    for a, b in zip(one, four):
+        assert np.array_equal(a, b)
