@@ -3,8 +3,8 @@
 Not a pytest module (long runs).  The two sides run separately so that the
 oracle can run on CPU cores while the GPU side runs on the MI355X:
 
-    python tests/parity_report.py oracle --out DIR [--only NAME ...]
-    python tests/parity_report.py gpu --out DIR [--precision fp64|mixed]
+    python tests/parity_report.py oracle --out DIR [--only NAME ...] [--full] [--jobs J]
+    python tests/parity_report.py gpu --out DIR [--precision fp64|mixed] [--full]
     python tests/parity_report.py compare ORACLE_DIR GPU_DIR
 
 Each side writes one .npz per workload with out%w, out%z_bed and the three
@@ -28,18 +28,25 @@ from fastisostasy_amd import workloads as W  # noqa: E402
 MASKS = ("maskocean", "maskgrounded", "maskcontinent")
 
 
-def cases():
-    """(name, workload dict, years, snapshot interval, H(t) callable, bsl method)"""
+def cases(full=False, lv_years=100.0):
+    """(name, workload dict, years, snapshot interval, H(t) callable, bsl method).
+
+    full: the workloads' whole horizons (ensemble 10 kyr, sea-level cycle
+    120 kyr) instead of the short default ones.  lv_years: horizon of the
+    Gaussian-rheology workload (its dt_max of 0.02 yr makes the FP64 CPU
+    oracle slow: about 12 min per 100 yr)."""
     out = []
     wl = W.plain_disc()
     out.append(("plain_disc", wl, 50e3, 1e3, lambda t, wl=wl: W.step_on_load(wl["H_ice"], t), "const"))
     wl = W.lv_rheology()
-    out.append(("lv_rheology", wl, 100.0, 20.0, lambda t, wl=wl: W.step_on_load(wl["H_ice"], t), "const"))
+    out.append(("lv_rheology", wl, lv_years, max(20.0, lv_years / 10), lambda t, wl=wl: W.step_on_load(wl["H_ice"], t),
+                "const"))
     for m in range(4):
         wl = W.ensemble_member(m)
-        out.append((f"ensemble_{m}", wl, 1e3, 200.0, lambda t, wl=wl: W.step_on_load(wl["H_ice"], t), "const"))
+        out.append((f"ensemble_{m}", wl, 10e3 if full else 1e3, 1e3 if full else 200.0,
+                    lambda t, wl=wl: W.step_on_load(wl["H_ice"], t), "const"))
     wl = W.sealevel_cycle()
-    out.append(("sealevel", wl, 3e3, 500.0,
+    out.append(("sealevel", wl, 120e3 if full else 3e3, 5e3 if full else 500.0,
                 lambda t, wl=wl: wl["H_max"] * float(W.sawtooth(t)), "fastiso"))
     return out
 
@@ -86,6 +93,9 @@ def main():
     ap.add_argument("--out")
     ap.add_argument("--only", nargs="*")
     ap.add_argument("--precision", default="fp64")
+    ap.add_argument("--full", action="store_true", help="whole horizons (ensemble 10 kyr, sea level 120 kyr)")
+    ap.add_argument("--lv-years", type=float, default=100.0)
+    ap.add_argument("--jobs", type=int, default=1, help="oracle side: workloads in parallel processes")
     args = ap.parse_args()
     if args.side == "compare":
         a, b = args.dirs
@@ -104,12 +114,25 @@ def main():
                   f"{np.sqrt((dw ** 2).mean()):.3e} | {dz.max():.3e} | {dm} |")
         return
     os.makedirs(args.out, exist_ok=True)
-    for name, wl, years, every, H, bm in cases():
-        if args.only and name not in args.only:
+    todo = [c for c in cases(args.full, args.lv_years) if not args.only or c[0] in args.only]
+    if args.side == "oracle" and args.jobs > 1:
+        import multiprocessing as mp
+        with mp.get_context("fork").Pool(args.jobs) as pool:
+            for line in pool.imap_unordered(_one, [(args, c[0]) for c in todo]):
+                print(line, flush=True)
+        return
+    for c in todo:
+        print(_one((args, c[0])), flush=True)
+
+
+def _one(job):
+    args, want = job
+    for name, wl, years, every, H, bm in cases(args.full, args.lv_years):
+        if name != want:
             continue
         snaps, wall, att = run_case(args.side, name, wl, years, every, H, bm, args.precision)
         np.savez_compressed(os.path.join(args.out, name + ".npz"), **snaps)
-        print(f"{args.side} {name}: {years:.0f} yr in {wall:.1f} s, {att:.0f} attempts", flush=True)
+        return f"{args.side} {name}: {years:.0f} yr in {wall:.1f} s, {att:.0f} attempts"
 
 
 if __name__ == "__main__":
