@@ -386,6 +386,86 @@ int sub_shape(int n, int nsub, bool fp64, bool uniform, const RhsShape*& out) {
     return 0;
 }
 
+// Attempt graphs: a block of GRAPH_REPS attempted steps captured once per
+// (descriptor array, batch size, shape, stream) and replayed with one
+// hipGraphLaunch, so the host issues one call instead of 4 x GRAPH_REPS
+// kernel launches.  Kernel nodes only (no memset or copy nodes); the kernels
+// read the descriptors and the control block at run time, so one graph
+// serves every update of a batch.  FI_NO_GRAPH disables them; benchmark
+// kernel timing and the request server (whose groupings change every round)
+// bypass them.  Measured on the 64-member ensemble they are neutral (the GPU,
+// not the host, bounds the attempt rate); they cut the host's launch work.
+constexpr int GRAPH_REPS = 8, GRAPH_CACHE = 128;
+struct GraphEntry {
+    const Member* d;
+    int nmem;
+    RhsShape s;
+    hipStream_t st;
+    hipGraphExec_t exec;
+    unsigned long long use;
+};
+std::vector<GraphEntry> g_graphs;
+unsigned long long g_graph_clock = 0;
+
+bool same_shape(const RhsShape& a, const RhsShape& b) {
+    return a.n == b.n && a.L == b.L && a.C == b.C && a.threads_row == b.threads_row &&
+           a.threads_col == b.threads_col && a.fp64 == b.fp64 && a.uniform == b.uniform && a.fast == b.fast &&
+           a.fused == b.fused && a.row_tiles == b.row_tiles && a.tw == b.tw && a.tw64 == b.tw64 &&
+           a.plan.n == b.plan.n && a.plan.nf == b.plan.nf;
+}
+
+int attempt_graph(const Member* d, int nmem, const RhsShape& s, hipStream_t st, hipGraphExec_t* out) {
+    ++g_graph_clock;
+    for (auto& e : g_graphs)
+        if (e.d == d && e.nmem == nmem && e.st == st && same_shape(e.s, s)) {
+            e.use = g_graph_clock;
+            *out = e.exec;
+            return 0;
+        }
+    HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    for (int r = 0; r < GRAPH_REPS; ++r) launch_attempt(d, nmem, s, st);
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamEndCapture(st, &g));
+    hipGraphExec_t ex = nullptr;
+    const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(FI_E_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    if ((int)g_graphs.size() >= GRAPH_CACHE) {
+        auto lru = std::min_element(g_graphs.begin(), g_graphs.end(),
+                                    [](const GraphEntry& a, const GraphEntry& b) { return a.use < b.use; });
+        (void)hipGraphExecDestroy(lru->exec);
+        g_graphs.erase(lru);
+    }
+    g_graphs.push_back({d, nmem, s, st, ex, g_graph_clock});
+    *out = ex;
+    return 0;
+}
+
+// graphs over descriptor arrays in [lo, hi) (before that memory is freed)
+void purge_graphs(const Member* lo, const Member* hi) {
+    for (size_t i = 0; i < g_graphs.size();) {
+        if (g_graphs[i].d >= lo && g_graphs[i].d < hi) {
+            (void)hipGraphExecDestroy(g_graphs[i].exec);
+            g_graphs.erase(g_graphs.begin() + i);
+        } else {
+            ++i;
+        }
+    }
+}
+
+// `count` attempts: whole graphs, then the remainder as plain launches
+int launch_attempts(const Member* d, int nmem, const RhsShape& s, hipStream_t st, int count, bool graphs) {
+    int done = 0;
+    if (graphs && count >= GRAPH_REPS && !prof_active() && !std::getenv("FI_NO_GRAPH")) {
+        hipGraphExec_t ex;
+        int rc = attempt_graph(d, nmem, s, st, &ex);
+        if (rc) return rc;
+        for (; done + GRAPH_REPS <= count; done += GRAPH_REPS) HIPCHK(hipGraphLaunch(ex, st));
+    }
+    for (; done < count; ++done) launch_attempt(d, nmem, s, st);
+    return 0;
+}
+
 // Diagnostics cadence of one isos_update call (fastisostasy.f90:782-787): a
 // function of the call times only.
 bool diag_due(const fi_isos* h, double time) { return time - h->time_diag >= (double)1e-3f; }
@@ -394,7 +474,7 @@ bool diag_due(const fi_isos* h, double time) { return time - h->time_diag >= (do
 // All members advance to `time`, or member m to times[m] when `times` is given
 // (the Member copies on the device then carry the targets in tf_req).
 int run_update(std::vector<fi_isos*>& hs, Member* d_mem, const RhsShape& shape, SumBuf& sb, double time,
-               bool stepping, const double* times = nullptr) {
+               bool stepping, const double* times = nullptr, bool graphs = true) {
     const int nmem = (int)hs.size();
     const fi_isos& h0 = *hs[0];
     const int n = h0.n;
@@ -452,13 +532,18 @@ int run_update(std::vector<fi_isos*>& hs, Member* d_mem, const RhsShape& shape, 
         }
         for (;;) {
             if (K == 1) {
-                for (int a = 0; a < ahead; ++a) launch_attempt(d_mem, nmem, shape, g_stream);
+                int rc = launch_attempts(d_mem, nmem, shape, g_stream, ahead, graphs);
+                if (rc) return rc;
             } else {
                 HIPCHK(hipEventRecord(g_sub.start, g_stream));
                 for (int k = 0; k < K; ++k) HIPCHK(hipStreamWaitEvent(g_sub.s[k], g_sub.start, 0));
-                for (int a = 0; a < ahead; ++a)
-                    for (int k = 0; k < K; ++k)
-                        launch_attempt(d_mem + sb_lo[k], sb_lo[k + 1] - sb_lo[k], *sshape[k], g_sub.s[k]);
+                // interleave the sub-batches' launches so both queues fill together
+                for (int a = 0; a < ahead; a += GRAPH_REPS)
+                    for (int k = 0; k < K; ++k) {
+                        int rc = launch_attempts(d_mem + sb_lo[k], sb_lo[k + 1] - sb_lo[k], *sshape[k], g_sub.s[k],
+                                                 std::min(GRAPH_REPS, ahead - a), graphs);
+                        if (rc) return rc;
+                    }
                 for (int k = 0; k < K; ++k) {
                     HIPCHK(hipEventRecord(g_sub.done[k], g_sub.s[k]));
                     HIPCHK(hipStreamWaitEvent(g_stream, g_sub.done[k], 0));
@@ -994,6 +1079,7 @@ int fi_isos_get_scalar(fi_isos* h, int which, double* out) {
 int fi_isos_destroy(fi_isos* h) {
     if (!h) return 0;
     (void)hipStreamSynchronize(g_stream);
+    purge_graphs(h->d_mem, h->d_mem + 1);
     (void)hipFree(h->arena);
     sum_free(h->sb);
     delete h;
@@ -1094,6 +1180,7 @@ namespace {
 int server_reserve(fi_server* s, int n) {
     if (n <= s->cap) return 0;
     (void)hipStreamSynchronize(g_stream);
+    purge_graphs(s->d_mem, s->d_mem + s->cap);
     if (s->d_mem) (void)hipFree(s->d_mem);
     if (s->h_stage) (void)hipHostFree(s->h_stage);
     sum_free(s->sb);
@@ -1236,7 +1323,8 @@ int fi_server_run(fi_server* s, int* completed) {
             if (rc) return rc;
             it = s->shapes.emplace(sk, shp).first;
         }
-        rc = run_update(hs, s->d_mem + base, it->second, s->sb, 0.0, true, times.data());
+        // groupings change from round to round: plain launches, no graph captures
+        rc = run_update(hs, s->d_mem + base, it->second, s->sb, 0.0, true, times.data(), false);
         if (rc) return rc;
         for (int q = 0; q < nm; ++q) {
             fi_server::Slot& sl = s->slots[g[q]];
@@ -1273,6 +1361,7 @@ int fi_server_stats(fi_server* s, long long* rounds, long long* groups, long lon
 int fi_server_destroy(fi_server* s) {
     if (!s) return 0;
     (void)hipStreamSynchronize(g_stream);
+    purge_graphs(s->d_mem, s->d_mem + s->cap);
     if (s->d_mem) (void)hipFree(s->d_mem);
     if (s->h_stage) (void)hipHostFree(s->h_stage);
     sum_free(s->sb);
@@ -1326,6 +1415,7 @@ int fi_bench_copy(double mib, int reps, double* gbs) {
 int fi_batch_destroy(fi_batch* b) {
     if (!b) return 0;
     (void)hipStreamSynchronize(g_stream);
+    purge_graphs(b->d_mem, b->d_mem + b->mem.size());
     (void)hipFree(b->d_mem);
     sum_free(b->sb);
     delete b;

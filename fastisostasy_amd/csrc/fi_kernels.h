@@ -40,6 +40,7 @@ bool launch_fast_pass(const Member* d_mem, int nmem, const RhsShape& s, hipStrea
 void prof_select(int kernel);
 int prof_collect(double* total_ms, long long* launches);
 int prof_fused();
+bool prof_active();
 // a start/stop event pair when launches of `kernel` are being timed
 bool prof_pair(int kernel, hipEvent_t* start, hipEvent_t* stop);
 

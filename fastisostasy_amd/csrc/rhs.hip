@@ -399,6 +399,7 @@ void prof_select(int kernel) {
 }
 
 int prof_fused() { return g_prof.fused ? 1 : 0; }
+bool prof_active() { return g_prof.kernel >= 0; }
 
 int prof_collect(double* total_ms, long long* launches) {
     double tot = 0.0;
