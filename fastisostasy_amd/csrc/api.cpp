@@ -534,6 +534,7 @@ int run_update(std::vector<fi_isos*>& hs, Member* d_mem, const RhsShape& shape, 
             if (K == 1) {
                 int rc = launch_attempts(d_mem, nmem, shape, g_stream, ahead, graphs);
                 if (rc) return rc;
+                launch_decide(d_mem, nmem, shape, g_stream);
             } else {
                 HIPCHK(hipEventRecord(g_sub.start, g_stream));
                 for (int k = 0; k < K; ++k) HIPCHK(hipStreamWaitEvent(g_sub.s[k], g_sub.start, 0));
@@ -545,6 +546,7 @@ int run_update(std::vector<fi_isos*>& hs, Member* d_mem, const RhsShape& shape, 
                         if (rc) return rc;
                     }
                 for (int k = 0; k < K; ++k) {
+                    launch_decide(d_mem + sb_lo[k], sb_lo[k + 1] - sb_lo[k], *sshape[k], g_sub.s[k]);
                     HIPCHK(hipEventRecord(g_sub.done[k], g_sub.s[k]));
                     HIPCHK(hipStreamWaitEvent(g_stream, g_sub.done[k], 0));
                 }
@@ -743,7 +745,8 @@ int fi_isos_create(const fi_params* p, int nx_ice, int ny_ice, double dx, double
     const size_t hc = (size_t)(n + 1) * (h->fp64 ? 16 : 8);
     size_t ocw = 0, oFGE = 0, oFGN = 0;
     if (h->has_conv) { ocw = take(2 * (size_t)n * hc); oFGE = take(2 * (size_t)n * hc); oFGN = take(2 * (size_t)n * hc); }
-    size_t octl = take(sizeof(Ctl)), ophys = take(sizeof(PhysDev)), omem = take(sizeof(Member));
+    size_t octl = take(sizeof(Ctl)), octl2 = take(sizeof(Ctl)), ophys = take(sizeof(PhysDev)),
+           omem = take(sizeof(Member));
     if (hipMalloc(&h->arena, off) != hipSuccess) { delete h; return fail(FI_E_HIP, "hipMalloc arena (%zu bytes)", off); }
     h->arena_bytes = off;
     HIPCHK(hipMemsetAsync(h->arena, 0, off, g_stream));
@@ -764,7 +767,7 @@ int fi_isos_create(const fi_params* p, int nx_ice, int ny_ice, double dx, double
     M.H_in = (double*)(a + oHin);
     M.D = a + oD; M.eta = a + oeta; M.kR = a + okR; M.cfull = a + ocfr; M.g1 = a + og1; M.g2 = a + og2;
     M.corners = (double*)(a + ocor); M.partials = (double*)(a + opart);
-    M.ctl = (Ctl*)(a + octl); M.phys = (PhysDev*)(a + ophys);
+    M.ctl = (Ctl*)(a + octl); M.ctl_next = (Ctl*)(a + octl2); M.phys = (PhysDev*)(a + ophys);
     M.bsl_lead = 1;
     M.bsl_next = -1;
     h->d_mem = (Member*)(a + omem);

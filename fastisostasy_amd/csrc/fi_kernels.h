@@ -34,6 +34,8 @@ struct ConvShape {
 };
 
 void launch_attempt(const Member* d_mem, int nmem, const RhsShape& s, hipStream_t st);
+// applies a pending step decision left by the last attempt (no-op otherwise)
+void launch_decide(const Member* d_mem, int nmem, const RhsShape& s, hipStream_t st);
 // power-of-two passes: which = 1 row forward, 2 column, 3 row inverse
 bool launch_fast_pass(const Member* d_mem, int nmem, const RhsShape& s, hipStream_t st, int which);
 // event timing of one attempt kernel: 0 stencil (fused: stencil + row DHT), 1 row DHT, 2 column DHT+divide, 3 inverse row DHT+update

@@ -33,6 +33,8 @@ struct Ctl {
     int status;                  // 1 stepping, 0 idle; negative = error
     int diag;                    // diagnostics updated in the running update
     int nonfinite;               // a NaN was seen in z_bed
+    int pending;                 // the last attempt's accept/reject is not applied yet
+    int skip;                    // this attempt slot does nothing (its inputs were loaded from the wrong state)
     long long attempts, accepts;
 };
 
@@ -98,6 +100,7 @@ struct Member {
     void* FGN;                   // sea-surface kernel half-spectrum
     double kc_E[4], kc_N[4];     // Green-kernel corner values (00, NN, N0, 0N)
     Ctl* ctl;
+    Ctl* ctl_next;               // decision of the previous attempt, written by the first pass
     BslDev* bsl;
     PhysDev* phys;
     // domains sharing one bsl object: the first in batch order leads and

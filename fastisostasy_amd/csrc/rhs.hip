@@ -354,13 +354,12 @@ __global__ void __launch_bounds__(512) k_row_inv(const Member* __restrict__ mem,
             update(cidx, k);
         }
     }
+    if (blockIdx.x == 0 && threadIdx.x == 0) M.ctl->pending = 1;
     block_reduce3(a1, a2, s2, M.partials + (size_t)blockIdx.x * 4);
 }
 
 __global__ void __launch_bounds__(64) k_decide(const Member* __restrict__ mem, int nblk) {
-    const Member& M = mem[blockIdx.x];
-    if (M.ctl->status != ST_RUN) return;
-    decide_wave(M, nblk);
+    decide_wave(mem[blockIdx.x], nblk);
 }
 
 // ------------------------------------------------------------------ launchers
@@ -429,16 +428,25 @@ void launch_attempt_t(const Member* d_mem, int nmem, const RhsShape& s, hipStrea
     if (!fused && (!s.fast || !launch_fast_pass(d_mem, nmem, s, st, 1)))
         launch_k(1, k_row_fwd<T>, dim3(s.n / (2 * s.L), nmem), dim3(s.threads_row), lds_bytes<T>(s.n, s.L), st,
                  d_mem, s.plan, (const void*)s.tw, s.L);
-    if (!s.fast || !launch_fast_pass(d_mem, nmem, s, st, 2))
+    // after the fused pass the column pass commits its control block
+    if (!s.fast || !launch_fast_pass(d_mem, nmem, s, st, fused ? 4 : 2))
         launch_k(2, k_col<T>, dim3(s.n / (2 * s.C), nmem), dim3(s.threads_col), lds_bytes<T>(s.n, s.C), st,
                  d_mem, s.plan, (const void*)s.tw, s.tw64, s.C);
-    // partials of the error norm: one per row (fast pass), one per row block (generic)
-    int nblk = s.n;
+    // partials of the error norm: one per line pair of rows (fast pass), one per row block (generic)
+    int nblk = s.n / 2;
     if (!s.fast || !launch_fast_pass(d_mem, nmem, s, st, 3)) {
         launch_k(3, k_row_inv<T>, dim3(s.n / (2 * s.L), nmem), dim3(s.threads_row), lds_bytes<T>(s.n, s.L), st,
                  d_mem, s.plan, (const void*)s.tw, s.L, s.n / (2 * s.C));
         nblk = s.n / (2 * s.L);
     }
+    // the fused pass of the next attempt applies this attempt's decision; the
+    // last one of a launch round is applied by launch_decide
+    if (!fused) hipLaunchKernelGGL(k_decide, dim3(nmem), dim3(64), 0, st, d_mem, nblk);
+}
+
+void launch_decide(const Member* d_mem, int nmem, const RhsShape& s, hipStream_t st) {
+    // partials: one per row on the power-of-two path, one per row block otherwise
+    const int nblk = s.fast ? s.n / 2 : s.n / (2 * s.L);
     hipLaunchKernelGGL(k_decide, dim3(nmem), dim3(64), 0, st, d_mem, nblk);
 }
 

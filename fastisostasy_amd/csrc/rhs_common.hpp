@@ -71,55 +71,117 @@ __device__ __forceinline__ void block_reduce3(double a, double b, double s, doub
         const int nw = (blockDim.x + 63) >> 6;
         double A = red[0][0], B = red[1][0], S = red[2][0];
         for (int q = 1; q < nw; ++q) { A = fmax(A, red[0][q]); B = fmax(B, red[1][q]); S += red[2][q]; }
-        out[0] = A; out[1] = B; out[2] = S;
+        out[0] = fmax(A, B); out[1] = S;
     }
 }
 
 
-// Error norm and step control of one domain (integrators.f90:186-199), one
-// wave per domain, after every block of the attempt has written its partials.
-// dwdt is stored only by attempts that can end the update (t + dt >= tf, the
+// A block-uniform 64-bit value moved to scalar registers.
+__device__ __forceinline__ double uni(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffLL));
+    const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ long long uni(long long v) {
+    const int lo = __builtin_amdgcn_readfirstlane((int)(v & 0xffffffffLL));
+    const int hi = __builtin_amdgcn_readfirstlane((int)(v >> 32));
+    return ((long long)hi << 32) | (unsigned int)lo;
+}
+
+// The fields the step control reads and writes.
+struct StepState {
+    double t, dt, tf, err, tol;
+    int cur, w_is_x, k_valid, status;
+    long long attempts, accepts;
+};
+
+__device__ __forceinline__ StepState load_state(const kptr<Ctl> c) {
+    StepState s;
+    s.t = c->t; s.dt = c->dt; s.tf = c->tf; s.err = c->err; s.tol = c->tol;
+    s.cur = c->cur; s.w_is_x = c->w_is_x; s.k_valid = c->k_valid; s.status = c->status;
+    s.attempts = c->attempts; s.accepts = c->accepts;
+    return s;
+}
+
+// Sum the per-row (or per-block) partials of the last attempt: lane-strided,
+// then fixed butterflies -- an order fixed by the partial count alone.
+// Partials hold (max(max|x1|, max|x2|), sum (x2-x1)^2) per row or block: the
+// tolerance uses only the larger maximum, and max is exact in any order.
+__device__ __forceinline__ void reduce_partials(gptr<double> part, int nblk, double& amax, double& s2) {
+    const int lane = threadIdx.x & 63;
+    amax = 0.0; s2 = 0.0;
+    for (int b = lane; b < nblk; b += 64) {
+        amax = fmax(amax, part[(size_t)b * 4 + 0]);
+        s2 += part[(size_t)b * 4 + 1];
+    }
+    amax = wave_max(amax); s2 = wave_sum(s2);
+}
+
+// Error norm and step control of one domain (integrators.f90:186-199) after
+// an attempt whose partials give max|x1|, max|x2| and sum (x2-x1)^2.  dwdt is
+// stored only by attempts that can end the update (t + dt >= tf, the
 // candidate whose k is out%dwdt); a rejected attempt reuses it when present
 // and otherwise recomputes the RHS.
-__device__ __forceinline__ void decide_wave(const Member& M, int nblk) {
-    Ctl& c = *M.ctl;
-    const PhysDev& ph = *M.phys;
-    const int lane = threadIdx.x & 63;
-    double a1 = 0.0, a2 = 0.0, s2 = 0.0;
-    for (int b = lane; b < nblk; b += 64) {
-        const double* p = M.partials + (size_t)b * 4;
-        a1 = fmax(a1, p[0]);
-        a2 = fmax(a2, p[1]);
-        s2 += p[2];
-    }
-    a1 = wave_max(a1); a2 = wave_max(a2); s2 = wave_sum(s2);
-    if (lane != 0) return;
-    const double dt = attempt_dt(c, ph);
-    const bool stored = c.k_valid || !(c.t + dt < c.tf);
-    c.attempts++;
-    const int m = ph.dt_method;
+__device__ __forceinline__ StepState decide_step(StepState s, double amax, double s2, const kptr<PhysDev> ph, int n) {
+    const int m = ph->dt_method;
+    const double dt = attempt_dt(m, s.dt, s.t, s.tf, ph->dt_max);
+    const bool stored = s.k_valid || !(s.t + dt < s.tf);
+    s.attempts += 1;
     bool accept = true;
     if (m == 2 || m == 3) {
-        const double tol = ph.atol + ph.rtol * fmax(a1, a2);
-        const double nn = (double)M.n * (double)M.n;
+        const double tol = ph->atol + ph->rtol * amax;
+        const double nn = (double)n * (double)n;
         const double err = sqrt(s2 / (tol * tol) / nn);
-        c.tol = tol; c.err = err;
+        s.tol = tol; s.err = err;
         accept = err < 1;
-        if (accept) c.dt = fmin(dt * pow(1 / err, 1 / ph.q), ph.dt_max);
-        else c.dt = dt * 0.5;
+        if (accept) s.dt = fmin(dt * pow(1 / err, 1 / ph->q), ph->dt_max);
+        else s.dt = dt * 0.5;
     } else if (m == 0) {
-        c.dt = dt;
+        s.dt = dt;
     }
     if (accept) {
-        c.t = c.t + dt;
-        c.cur ^= 1;
-        c.w_is_x = 1;
-        c.k_valid = 0;
-        c.accepts++;
+        s.t = s.t + dt;
+        s.cur ^= 1;
+        s.w_is_x = 1;
+        s.k_valid = 0;
+        s.accepts += 1;
     } else {
-        c.k_valid = stored ? 1 : 0;
+        s.k_valid = stored ? 1 : 0;
     }
-    if (!(c.t < c.tf)) c.status = ST_IDLE;
+    if (!(s.t < s.tf)) s.status = ST_IDLE;
+    return s;
+}
+
+// Field-wise copy of a control block read through the scalar cache.
+__device__ __forceinline__ void copy_ctl(Ctl* d, const kptr<Ctl> s) {
+    d->t = s->t; d->dt = s->dt; d->tf = s->tf; d->err = s->err; d->tol = s->tol;
+    d->V_af = s->V_af; d->V_den = s->V_den; d->V_pov = s->V_pov; d->deltaV = s->deltaV;
+    d->dbsl_total = s->dbsl_total; d->bsl = s->bsl; d->time_diag = s->time_diag; d->time_prog = s->time_prog;
+    d->maxw = s->maxw; d->maxwe = s->maxwe; d->maxdzss = s->maxdzss; d->wcorner = s->wcorner;
+    d->ref_bsl = s->ref_bsl; d->cur = s->cur; d->w_is_x = s->w_is_x; d->k_valid = s->k_valid;
+    d->status = s->status; d->diag = s->diag; d->nonfinite = s->nonfinite; d->pending = s->pending;
+    d->skip = s->skip; d->attempts = s->attempts; d->accepts = s->accepts;
+}
+
+__device__ __forceinline__ void store_state(Ctl* c, const StepState& s) {
+    c->t = s.t; c->dt = s.dt; c->tf = s.tf; c->err = s.err; c->tol = s.tol;
+    c->cur = s.cur; c->w_is_x = s.w_is_x; c->k_valid = s.k_valid; c->status = s.status;
+    c->attempts = s.attempts; c->accepts = s.accepts;
+}
+
+// Stand-alone decision of one domain (one wave): applies the pending
+// decision of the last attempt, if any.
+__device__ __forceinline__ void decide_wave(const Member& M, int nblk) {
+    const kptr<Ctl> ck = kview(M.ctl);
+    const StepState s0 = load_state(ck);
+    const int pending = ck->pending;
+    double amax, s2;
+    reduce_partials(gin<double>(M.partials), nblk, amax, s2);
+    if (!pending || (threadIdx.x & 63) != 0) return;
+    const StepState s = decide_step(s0, amax, s2, kview(M.phys), M.n);
+    store_state(M.ctl, s);
+    M.ctl->pending = 0;
 }
 
 }  // namespace fi

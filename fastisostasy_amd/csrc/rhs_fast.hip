@@ -87,6 +87,9 @@ template <typename T, int N, int THREADS> struct SRow {
     static constexpr size_t lds = (size_t)N * sizeof(cplx<T>) + (stencil_bytes > xch_bytes ? stencil_bytes : xch_bytes);
 };
 
+// rows of canom_full / eta the march loads ahead of their use
+constexpr int SR_LA = 6;
+
 template <int B, int E, class F> __device__ __forceinline__ void static_for(F&& f) {
     if constexpr (B < E) {
         f(std::integral_constant<int, B>{});
@@ -103,66 +106,23 @@ __device__ __forceinline__ void xcd_order(int& member, int& blk) {
     blk = b % per;
 }
 
-template <typename T, int N, int THREADS>
-__global__ void __launch_bounds__(THREADS) kf_srow_fwd(const Member* __restrict__ mem, const void* __restrict__ twv) {
+// The march and the row transform of one block.  issue(tau) issues the
+// global loads due at march step tau (inputs arrive SR_LA steps ahead of use).
+template <typename T, int N, int THREADS, class Issue>
+__device__ __forceinline__ void srow_body(const Member& M, const kptr<PhysDev> ph, int R0, int tid, bool top,
+                                          bool bottom, unsigned char* smem,
+                                          double (&wv)[SRow<T, N, THREADS>::ROWS + 4][SRow<T, N, THREADS>::CPT],
+                                          T (&dv)[SRow<T, N, THREADS>::ROWS + 2][SRow<T, N, THREADS>::CPT],
+                                          T (&cv)[SRow<T, N, THREADS>::ROWS][SRow<T, N, THREADS>::CPT],
+                                          T (&ev)[SRow<T, N, THREADS>::ROWS][SRow<T, N, THREADS>::CPT],
+                                          Issue&& issue) {
     using SR = SRow<T, N, THREADS>;
     constexpr int TPL = SR::TPL, E = P2Plan<N>::E, L = SR::L, ROWS = SR::ROWS, CPT = SR::CPT;
-    static_assert(CPT >= 1 && CPT * THREADS == N && ROWS >= 4, "fused stencil shape");
-    int mi, rblk;
-    xcd_order(mi, rblk);
-    const Member& M = mem[mi];
-    const kptr<Ctl> c = kview(M.ctl);
-    if ((c->status != ST_RUN) | (c->k_valid != 0)) return;
-    const kptr<PhysDev> ph = kview(M.phys);
-    extern __shared__ __align__(16) unsigned char smem[];
     cplx<T>* tw_l = (cplx<T>*)smem;
     double* rw = (double*)(tw_l + N);     // w    rows, slot (r + 3) % 3
     double* rxx = rw + 3 * N;             // Mxx  rows, slot (r + 3) % 3
     double* rxy = rxx + 3 * N;            // Mxy  rows, slot (r + 4) % 4
     T* sf = (T*)(rxy + 4 * N);            // f    [ROWS][N]
-    const int tid = threadIdx.x;
-    const int R0 = rblk * ROWS;
-    const bool top = R0 == 0, bottom = R0 + ROWS == N;
-    gptr<double> w = gin<double>(c->w_is_x ? M.X[c->cur] : M.W);
-    gptr<T> Dg = gin<T>(M.D), Cg = gin<T>(M.cfull), Eg = gin<T>(M.eta);
-    auto crow = [](int r) { return r < 0 ? 0 : (r > N - 1 ? N - 1 : r); };
-    // twiddles first: their LDS store then waits only for their own loads
-    constexpr int TWQ = (2 * N + THREADS - 1) / THREADS;
-    T twr[TWQ];
-    {
-        gptr<T> g = gin<T>(twv);
-#pragma unroll
-        for (int k = 0; k < TWQ; ++k)
-            if (tid + THREADS * k < 2 * N) twr[k] = g[tid + THREADS * k];
-    }
-    double wv[ROWS + 4][CPT];   // local rows -2 .. ROWS+1
-    T dv[ROWS + 2][CPT];        // -1 .. ROWS
-    T cv[ROWS][CPT], ev[ROWS][CPT];
-    // loads in the order the march consumes them
-#pragma unroll
-    for (int q = 0; q < CPT; ++q) {
-        wv[0][q] = w[(size_t)crow(R0 - 2) * N + tid + THREADS * q];
-        wv[1][q] = w[(size_t)crow(R0 - 1) * N + tid + THREADS * q];
-    }
-#pragma unroll
-    for (int m = -1; m <= ROWS + 1; ++m)
-#pragma unroll
-        for (int q = 0; q < CPT; ++q) {
-            const int col = tid + THREADS * q;
-            if (m <= ROWS) {
-                wv[m + 3][q] = w[(size_t)crow(R0 + m + 1) * N + col];
-                dv[m + 1][q] = Dg[(size_t)crow(R0 + m) * N + col];
-            }
-            if (m >= 2) {
-                cv[m - 2][q] = Cg[(size_t)(R0 + m - 2) * N + col];
-                ev[m - 2][q] = Eg[(size_t)(R0 + m - 2) * N + col];
-            }
-            // keep the issue order: the march waits on each row group in turn
-            __builtin_amdgcn_sched_barrier(0);
-        }
-#pragma unroll
-    for (int k = 0; k < TWQ; ++k)
-        if (tid + THREADS * k < 2 * N) ((T*)tw_l)[tid + THREADS * k] = twr[k];
     const double invq = ph->inv_dx2, inv1 = ph->inv_dx, inv2 = ph->inv_2dx, nu = ph->nu, g = ph->g;
     auto s3 = [](int r) { return (r + 3) % 3; };
     auto s4 = [](int r) { return (r + 4) % 4; };
@@ -174,6 +134,7 @@ __global__ void __launch_bounds__(THREADS) kf_srow_fwd(const Member* __restrict_
     double my[ROWS + 2][CPT];   // Myy of local rows -1 .. ROWS
     static_for<-1, ROWS + 2>([&](auto mc) {
         constexpr int m = decltype(mc)::value;
+        issue(mc);
         if constexpr (m <= ROWS) {
             // moments of local row m (global gm); w rows m-1 .. m+1 in the ring
             const int gm = R0 + m;
@@ -267,12 +228,146 @@ __global__ void __launch_bounds__(THREADS) kf_srow_fwd(const Member* __restrict_
     }
 }
 
-template <typename T, int N, int C>
+template <typename T, int N, int THREADS>
+__global__ void __launch_bounds__(THREADS) kf_srow_fwd(const Member* __restrict__ mem, const void* __restrict__ twv) {
+    using SR = SRow<T, N, THREADS>;
+    constexpr int ROWS = SR::ROWS, CPT = SR::CPT;
+    static_assert(CPT >= 1 && CPT * THREADS == N && ROWS >= 4, "fused stencil shape");
+    int mi, rblk;
+    xcd_order(mi, rblk);
+    const Member& M = mem[mi];
+    const kptr<Ctl> c = kview(M.ctl);
+    const kptr<PhysDev> ph = kview(M.phys);
+    const int tid = threadIdx.x;
+    // Apply the previous attempt's pending accept/reject here instead of in a
+    // separate kernel: every wave recomputes it from the same partials (same
+    // order, same result); block 0 writes the resulting control block to
+    // ctl_next, which the column pass commits.  The field rows are loaded
+    // before the decision is known, from the buffer an accepted step leaves
+    // current; a rejected step without a stored RHS reloads the old state.
+    StepState st = load_state(c);
+    const bool pending = c->pending != 0;
+    if (!pending && ((st.status != ST_RUN) | (st.k_valid != 0))) {
+        if (rblk == 0 && tid == 0) {
+            copy_ctl(M.ctl_next, c);
+            M.ctl_next->skip = 0;
+        }
+        return;
+    }
+    const double *x0 = M.X[0], *x1 = M.X[1], *wW = M.W;
+    const double* wpred = pending ? (st.cur ? x0 : x1) : (st.w_is_x ? (st.cur ? x1 : x0) : wW);
+    extern __shared__ __align__(16) unsigned char smem[];
+    cplx<T>* tw_l = (cplx<T>*)smem;
+    const int R0 = rblk * ROWS;
+    const bool top = R0 == 0, bottom = R0 + ROWS == N;
+    gptr<T> Dg = gin<T>(M.D), Cg = gin<T>(M.cfull), Eg = gin<T>(M.eta);
+    auto crow = [](int r) { return r < 0 ? 0 : (r > N - 1 ? N - 1 : r); };
+    // twiddles and the previous attempt's partials first: the LDS store and
+    // the decision then wait only for their own loads
+    constexpr int TWQ = (2 * N + THREADS - 1) / THREADS;
+    T twr[TWQ];
+    {
+        gptr<T> g = gin<T>(twv);
+#pragma unroll
+        for (int k = 0; k < TWQ; ++k)
+            if (tid + THREADS * k < 2 * N) twr[k] = g[tid + THREADS * k];
+    }
+    // (loaded whether or not a decision is pending: no branch for the
+    // compiler to reorder the decision around)
+    constexpr int NP = N / 2, PJ = (NP + 63) / 64;   // one partial per line
+    double pa[PJ][2];
+    {
+        gptr<double> part = gin<double>(M.partials);
+        const int lane = tid & 63;
+#pragma unroll
+        for (int j = 0; j < PJ; ++j) {
+            const bool in = lane + 64 * j < NP;
+#pragma unroll
+            for (int k = 0; k < 2; ++k) pa[j][k] = in ? part[(size_t)(lane + 64 * j) * 4 + k] : 0.0;
+        }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    double wv[ROWS + 4][CPT];   // local rows -2 .. ROWS+1
+    T dv[ROWS + 2][CPT];        // -1 .. ROWS
+    T cv[ROWS][CPT], ev[ROWS][CPT];
+    // Software-pipelined loads: at march step tau the block issues w row
+    // tau + 1 + SR_LA, D row tau + SR_LA and canom_full / eta row
+    // tau - 2 + SR_LA, each SR_LA steps before its first use.  Row bases are
+    // uniform (scalar); the lane offset is one 32-bit value for every row.
+    gptr<double> w = gin<double>(wpred);
+    auto issue = [&](auto tc) {
+        constexpr int tau = decltype(tc)::value;
+        constexpr int rw_ = tau + 1 + SR_LA, rd = tau + SR_LA, rc = tau - 2 + SR_LA;
+#pragma unroll
+        for (int q = 0; q < CPT; ++q) {
+            const unsigned col = tid + THREADS * q;
+            if constexpr (rw_ >= -2 && rw_ <= ROWS + 1) wv[rw_ + 2][q] = (w + (size_t)crow(R0 + rw_) * N)[col];
+            if constexpr (rd >= -1 && rd <= ROWS) dv[rd + 1][q] = (Dg + (size_t)crow(R0 + rd) * N)[col];
+            if constexpr (rc >= 0 && rc < ROWS) {
+                cv[rc][q] = (Cg + (size_t)(R0 + rc) * N)[col];
+                ev[rc][q] = (Eg + (size_t)(R0 + rc) * N)[col];
+            }
+        }
+        // keep the issue order: each step waits only on loads SR_LA steps old
+        __builtin_amdgcn_sched_barrier(0);
+    };
+    static_for<-3 - SR_LA, -1>(issue);   // everything due before the first march step
+    {
+        // the order of reduce_partials: lane-strided, then butterflies
+        double amax = 0.0, s2 = 0.0;
+#pragma unroll
+        for (int j = 0; j < PJ; ++j) {
+            amax = fmax(amax, pa[j][0]);
+            s2 += pa[j][1];
+        }
+        amax = wave_max(amax); s2 = wave_sum(s2);
+        const StepState sd = decide_step(st, amax, s2, ph, N);
+        if (pending) st = sd;
+        // uniform across the block: keep the state in scalar registers
+        st.cur = __builtin_amdgcn_readfirstlane(st.cur);
+        st.w_is_x = __builtin_amdgcn_readfirstlane(st.w_is_x);
+        st.k_valid = __builtin_amdgcn_readfirstlane(st.k_valid);
+        st.status = __builtin_amdgcn_readfirstlane(st.status);
+        st.t = uni(st.t); st.dt = uni(st.dt); st.tf = uni(st.tf); st.err = uni(st.err); st.tol = uni(st.tol);
+        st.attempts = uni(st.attempts); st.accepts = uni(st.accepts);
+    }
+    // A step rejected without a stored RHS needs the old state, not the one
+    // loaded: this attempt slot then does nothing for the domain (the column
+    // and inverse row passes skip it) and the next attempt runs on the old
+    // state.  Rare: about one attempt in a hundred.
+    const double* wnow = st.w_is_x ? (st.cur ? x1 : x0) : wW;
+    const int skip = __builtin_amdgcn_readfirstlane(wnow != wpred ? 1 : 0);
+    // block 0 writes ctl_next after its last load (stores inside a one-lane
+    // branch would make every later wait a full one)
+    auto write_next = [&]() {
+        if (rblk == 0 && tid == 0) {
+            copy_ctl(M.ctl_next, c);
+            store_state(M.ctl_next, st);
+            M.ctl_next->pending = 0;
+            M.ctl_next->skip = skip;
+        }
+    };
+    if ((st.status != ST_RUN) | (st.k_valid != 0) | skip) {
+        write_next();
+        return;
+    }
+#pragma unroll
+    for (int k = 0; k < TWQ; ++k)
+        if (tid + THREADS * k < 2 * N) ((T*)tw_l)[tid + THREADS * k] = twr[k];
+    srow_body<T, N, THREADS>(M, ph, R0, tid, top, bottom, smem, wv, dv, cv, ev, issue);
+    write_next();
+}
+
+// COMMIT: the attempt began with the fused stencil pass, which left the
+// control block in ctl_next; block 0 commits it (the other blocks read
+// ctl_next, so nothing reads the block being written).
+template <typename T, int N, int C, bool COMMIT>
 __global__ void __launch_bounds__(C * (N / 8)) kf_col(const Member* __restrict__ mem, const void* __restrict__ twv,
                                                      const double2* __restrict__ tw64) {
     const Member& M = mem[blockIdx.y];
-    const kptr<Ctl> c = kview(M.ctl);
-    if ((c->status != ST_RUN) | (c->k_valid != 0)) return;
+    const kptr<Ctl> c = kview(COMMIT ? M.ctl_next : M.ctl);
+    if (COMMIT && blockIdx.x == 0 && threadIdx.x == 0) copy_ctl(M.ctl, c);
+    if ((c->status != ST_RUN) | (c->k_valid != 0) | (c->skip != 0)) return;
     using PL = P2Plan<N>;
     constexpr int TPL = PL::TPL, E = PL::E;
     extern __shared__ __align__(16) unsigned char smem[];
@@ -344,7 +439,7 @@ __global__ void __launch_bounds__(C * (N / 8)) kf_col(const Member* __restrict__
 // depend on how many domains share the launch.
 template <int TPL>
 __device__ __forceinline__ void row_reduce_store(double a1a, double a2a, double s2a, double a1b, double a2b,
-                                                 double s2b, int line, int t, gmut<double> pa, gmut<double> pb) {
+                                                 double s2b, int line, int t, gmut<double> pa) {
     constexpr int W = TPL < 64 ? TPL : 64;
 #pragma unroll
     for (int o = W / 2; o > 0; o >>= 1) {
@@ -370,8 +465,9 @@ __device__ __forceinline__ void row_reduce_store(double a1a, double a2a, double 
         }
     }
     if (t == 0) {
-        pa[0] = a1a; pa[1] = a2a; pa[2] = s2a;
-        pb[0] = a1b; pb[1] = a2b; pb[2] = s2b;
+        // one entry per line: rows 2j and 2j+1 combined in a fixed order
+        pa[0] = fmax(fmax(a1a, a2a), fmax(a1b, a2b));
+        pa[1] = s2a + s2b;
     }
 }
 
@@ -384,7 +480,7 @@ __global__ void __launch_bounds__(THREADS) kf_row_inv(const Member* __restrict__
                                                       int ncolblk) {
     const Member& M = mem[blockIdx.y];
     const kptr<Ctl> c = kview(M.ctl);
-    if (c->status != ST_RUN) return;
+    if ((c->status != ST_RUN) | (c->skip != 0)) return;
     const kptr<PhysDev> ph = kview(M.phys);
     using PL = P2Plan<N>;
     constexpr int TPL = PL::TPL, E = PL::E, L = THREADS / TPL;
@@ -399,6 +495,7 @@ __global__ void __launch_bounds__(THREADS) kf_row_inv(const Member* __restrict__
     const double dt = attempt_dt(meth, c->dt, c->t, c->tf, ph->dt_max);
     const int cur = c->cur;
     const bool k_valid = c->k_valid != 0;
+    if (blockIdx.x == 0 && tid == 0) M.ctl->pending = 1;   // the step control reads this attempt next
     gptr<double> X = gin<double>(M.X[cur]);
     gmut<double> Xn = gout<double>(M.X[cur ^ 1]);
     gmut<double> dw = gout<double>(M.dwdt);
@@ -441,8 +538,7 @@ __global__ void __launch_bounds__(THREADS) kf_row_inv(const Member* __restrict__
                 update(idx(ra[k], e), xa[k][e], ka[k][e], a1a, a2a, s2a);
                 update(idx(ra[k] + 1, e), xb[k][e], kb[k][e], a1b, a2b, s2b);
             }
-            row_reduce_store<TPL>(a1a, a2a, s2a, a1b, a2b, s2b, line, t, part + (size_t)ra[k] * 4,
-                                  part + (size_t)(ra[k] + 1) * 4);
+            row_reduce_store<TPL>(a1a, a2a, s2a, a1b, a2b, s2b, line, t, part + (size_t)(ra[k] / 2) * 4);
         }
     } else {
         __shared__ double qs[4];
@@ -512,8 +608,7 @@ __global__ void __launch_bounds__(THREADS) kf_row_inv(const Member* __restrict__
                 update(ca, xa[k][e], ka, a1a, a2a, s2a);
                 update(cb, xb[k][e], kb, a1b, a2b, s2b);
             }
-            row_reduce_store<TPL>(a1a, a2a, s2a, a1b, a2b, s2b, line, t, part + (size_t)ra[k] * 4,
-                                  part + (size_t)(ra[k] + 1) * 4);
+            row_reduce_store<TPL>(a1a, a2a, s2a, a1b, a2b, s2b, line, t, part + (size_t)(ra[k] / 2) * 4);
         }
     }
 }
@@ -552,21 +647,26 @@ static void rows_inv(const Member* d, int nmem, hipStream_t st, const void* tw, 
         launch_k(3, kf_row_inv<T, N, THREADS, 1>, dim3(N / (2 * L), nmem), dim3(THREADS), lds, st, d, tw, ncolblk);
 }
 template <typename T, int N, int C>
-static void cols(const Member* d, int nmem, hipStream_t st, const void* tw, const double2* tw64) {
+static void cols(const Member* d, int nmem, hipStream_t st, const void* tw, const double2* tw64, bool commit) {
     const size_t lds = col_lds<T, N>(C);
-    launch_k(2, kf_col<T, N, C>, dim3(N / (2 * C), nmem), dim3(C * (N / 8)), lds, st, d, tw, tw64);
+    if (commit)
+        launch_k(2, kf_col<T, N, C, true>, dim3(N / (2 * C), nmem), dim3(C * (N / 8)), lds, st, d, tw, tw64);
+    else
+        launch_k(2, kf_col<T, N, C, false>, dim3(N / (2 * C), nmem), dim3(C * (N / 8)), lds, st, d, tw, tw64);
 }
 
 template <typename T, int N>
 static bool fast_n(const Member* d, int nmem, const RhsShape& s, hipStream_t st, int which) {
-    // which: 0 fused stencil + row forward, 1 row forward, 2 column, 3 row inverse
-    if (which == 2) {
+    // which: 0 fused stencil + row forward, 1 row forward, 2 column (4: column
+    // committing the control block of the fused pass), 3 row inverse
+    if (which == 2 || which == 4) {
+        const bool cm = which == 4;
         switch (s.C) {
-        case 1: cols<T, N, 1>(d, nmem, st, s.tw, s.tw64); return true;
-        case 2: if constexpr (2 * (N / 8) <= 1024) { cols<T, N, 2>(d, nmem, st, s.tw, s.tw64); return true; } break;
-        case 4: if constexpr (4 * (N / 8) <= 1024) { cols<T, N, 4>(d, nmem, st, s.tw, s.tw64); return true; } break;
-        case 8: if constexpr (8 * (N / 8) <= 1024) { cols<T, N, 8>(d, nmem, st, s.tw, s.tw64); return true; } break;
-        case 16: if constexpr (16 * (N / 8) <= 1024) { cols<T, N, 16>(d, nmem, st, s.tw, s.tw64); return true; } break;
+        case 1: cols<T, N, 1>(d, nmem, st, s.tw, s.tw64, cm); return true;
+        case 2: if constexpr (2 * (N / 8) <= 1024) { cols<T, N, 2>(d, nmem, st, s.tw, s.tw64, cm); return true; } break;
+        case 4: if constexpr (4 * (N / 8) <= 1024) { cols<T, N, 4>(d, nmem, st, s.tw, s.tw64, cm); return true; } break;
+        case 8: if constexpr (8 * (N / 8) <= 1024) { cols<T, N, 8>(d, nmem, st, s.tw, s.tw64, cm); return true; } break;
+        case 16: if constexpr (16 * (N / 8) <= 1024) { cols<T, N, 16>(d, nmem, st, s.tw, s.tw64, cm); return true; } break;
         }
         return false;
     }

@@ -279,6 +279,8 @@ __global__ void k_begin(const Member* __restrict__ mem, int nmem, double tf, int
     if (per_member) tf = mem[m].tf_req;
     c.tf = tf;
     c.k_valid = 0;
+    c.pending = 0;
+    c.skip = 0;
     c.status = (stepping && c.t < tf) ? ST_RUN : ST_IDLE;
 }
 
