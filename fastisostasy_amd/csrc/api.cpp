@@ -509,12 +509,15 @@ int run_update(std::vector<fi_isos*>& hs, Member* d_mem, const RhsShape& shape, 
         // each readback it launches as many attempts as the slowest member
         // still needs at its current dt (at least one).
         // first round: as many attempts as the slowest member needs at the dt it
-        // ended its previous update with (an estimate; the loop below corrects it)
+        // ended its previous update with, plus two for rejections (an extra
+        // attempt costs a few microseconds of gated no-op kernels, a second
+        // round a host round trip); the loop below corrects the estimate
         int ahead = 1;
         {
             double need = 0.0;
             for (int m = 0; m < nmem; ++m)
                 if (hs[m]->last_dt > 0.0) need = std::max(need, std::ceil((t_of(m) - hs[m]->last_t) / hs[m]->last_dt));
+            if (need > 0.0) need += 2.0;
             ahead = (int)std::min(256.0, std::max(1.0, need));
         }
         long long total = 0;
@@ -551,6 +554,9 @@ int run_update(std::vector<fi_isos*>& hs, Member* d_mem, const RhsShape& shape, 
                     HIPCHK(hipStreamWaitEvent(g_stream, g_sub.done[k], 0));
                 }
             }
+            // finalize the members that are done (gated per member on the
+            // device) before the readback: usually no second round follows
+            launch_finalize(d_mem, nmem, n, g_stream);
             HIPCHK(hipGetLastError());
             total += ahead;
             int rc = gather(d_mem, nmem, sb);
@@ -564,14 +570,15 @@ int run_update(std::vector<fi_isos*>& hs, Member* d_mem, const RhsShape& shape, 
                 need = std::max(need, std::ceil((c.tf - c.t) / std::max(c.dt, 1e-300)));
             }
             if (!running) break;
-            ahead = (int)std::min(256.0, std::max(1.0, need));
+            ahead = (int)std::min(256.0, std::max(1.0, need + 1.0));
             if (total > 100000000LL) return fail(FI_E_STATE, "integrator did not reach the target time");
         }
+    } else {
+        launch_finalize(d_mem, nmem, n, g_stream);
+        HIPCHK(hipGetLastError());
+        int rcg = gather(d_mem, nmem, sb);
+        if (rcg) return rcg;
     }
-    launch_finalize(d_mem, nmem, n, g_stream);
-    HIPCHK(hipGetLastError());
-    int rcg = gather(d_mem, nmem, sb);
-    if (rcg) return rcg;
     for (int m = 0; m < nmem; ++m) {
         const CtlSummary& c = sb.h[m];
         hs[m]->time_prog = t_of(m);

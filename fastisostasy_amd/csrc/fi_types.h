@@ -35,6 +35,8 @@ struct Ctl {
     int nonfinite;               // a NaN was seen in z_bed
     int pending;                 // the last attempt's accept/reject is not applied yet
     int skip;                    // this attempt slot does nothing (its inputs were loaded from the wrong state)
+    int fin;                     // the running update has been finalized
+    int pad_;
     long long attempts, accepts;
 };
 

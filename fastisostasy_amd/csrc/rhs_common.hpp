@@ -161,7 +161,7 @@ __device__ __forceinline__ void copy_ctl(Ctl* d, const kptr<Ctl> s) {
     d->maxw = s->maxw; d->maxwe = s->maxwe; d->maxdzss = s->maxdzss; d->wcorner = s->wcorner;
     d->ref_bsl = s->ref_bsl; d->cur = s->cur; d->w_is_x = s->w_is_x; d->k_valid = s->k_valid;
     d->status = s->status; d->diag = s->diag; d->nonfinite = s->nonfinite; d->pending = s->pending;
-    d->skip = s->skip; d->attempts = s->attempts; d->accepts = s->accepts;
+    d->skip = s->skip; d->fin = s->fin; d->pad_ = 0; d->attempts = s->attempts; d->accepts = s->accepts;
 }
 
 __device__ __forceinline__ void store_state(Ctl* c, const StepState& s) {

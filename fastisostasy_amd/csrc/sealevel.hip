@@ -140,12 +140,14 @@ __global__ void __launch_bounds__(PRE_THREADS) k_pre_fields(const Member* __rest
                  cmantle = gin<double>(M.canom_mantle);
     gmut<T> cf_rhs = gout<T>(M.cfull);
     // calc_z_ss, calc_masks, seawater and load anomalies for one cell
-    auto sea_load = [&](size_t c, uint8_t mold, bool active, double dz, uint8_t& mk) {
+    auto sea_load = [&](size_t c, uint8_t mold, bool active, double dz, uint8_t& mk, bool store = true) {
         const double z_ss = bsl + zss_ref[c] + dz;
         const double haf = Haf[c];
         mk = calc_mask(zbed[c], z_ss, Hice[c], haf, mold);
-        zss[c] = z_ss;
-        mask[c] = mk;
+        if (store) {
+            zss[c] = z_ss;
+            mask[c] = mk;
+        }
         const double ice = 0.0 + rho_ice * (haf - Haf_ref[c]);
         const double sw = 0.0 + ((mk & MB_OCEAN) ? rho_water * (z_ss - zss_ref[c]) : 0.0);
         double load = ice;
@@ -184,7 +186,8 @@ __global__ void __launch_bounds__(PRE_THREADS) k_pre_fields(const Member* __rest
             full_of(c, active, load, we[c]);
             continue;
         }
-        const double load = sea_load(c, mold, active, dzss[c], mk);
+        // (phase 3 overwrites z_ss and the masks below: no stores here)
+        const double load = sea_load(c, mold, active, dzss[c], mk, phase != 3);
         if (phase == 3) {
             // phases 0, 1 and 2 without convolutions: we = 0, dz_ss = 0
             cload[c] = load;
@@ -209,11 +212,15 @@ __global__ void __launch_bounds__(PRE_THREADS) k_pre_fields(const Member* __rest
 }
 
 // Corner correction of now%w (isos_utils.f90:104-110), one thread per domain.
+// The finalize kernels run once per update and domain, for domains that have
+// stopped stepping: the host may launch them before it knows that every
+// domain is done, and again after further attempts.
 __global__ void k_fin_corner(const Member* __restrict__ mem, int nmem) {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= nmem) return;
     const Member& M = mem[m];
     Ctl& c = *M.ctl;
+    if (c.status == ST_RUN || c.fin) return;
     const int N = M.n;
     const double* w = c.w_is_x ? M.X[c.cur] : M.W;
     c.wcorner = 0.25 * (w[0] + w[(size_t)N * N - 1] + w[(size_t)(N - 1) * N] + w[N - 1]);
@@ -223,6 +230,7 @@ __global__ void __launch_bounds__(PRE_THREADS) k_finalize(const Member* __restri
     const Member& M = mem[blockIdx.y];
     const kptr<PhysDev> ph = kview(M.phys);
     const kptr<Ctl> ctl = kview(M.ctl);
+    if ((ctl->status == ST_RUN) | (ctl->fin != 0)) return;
     const int N = M.n;
     const size_t nn = (size_t)N * N;
     gptr<double> wsrc = gin<double>(ctl->w_is_x ? M.X[ctl->cur] : M.W);
@@ -261,6 +269,7 @@ __global__ void __launch_bounds__(PRE_THREADS) k_finalize(const Member* __restri
 __global__ void __launch_bounds__(64) k_fin_check(const Member* __restrict__ mem, int nblk) {
     const Member& M = mem[blockIdx.x];
     Ctl& c = *M.ctl;
+    if (c.status == ST_RUN || c.fin) return;
     double v[4] = {0, 0, 0, 0};
     for (int b = threadIdx.x; b < nblk; b += 64)
         for (int r = 0; r < 4; ++r) v[r] = fmax(v[r], M.partials[(size_t)b * 4 + r]);
@@ -269,6 +278,7 @@ __global__ void __launch_bounds__(64) k_fin_check(const Member* __restrict__ mem
         c.maxw = v[0]; c.maxwe = v[1]; c.maxdzss = v[2]; c.nonfinite = v[3] > 0.0;
         c.w_is_x = 0;
         c.status = ST_IDLE;
+        c.fin = 1;
     }
 }
 
@@ -281,6 +291,7 @@ __global__ void k_begin(const Member* __restrict__ mem, int nmem, double tf, int
     c.k_valid = 0;
     c.pending = 0;
     c.skip = 0;
+    c.fin = 0;
     c.status = (stepping && c.t < tf) ? ST_RUN : ST_IDLE;
 }
 
