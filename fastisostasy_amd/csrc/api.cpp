@@ -391,10 +391,11 @@ int sub_shape(int n, int nsub, bool fp64, bool uniform, const RhsShape*& out) {
 // hipGraphLaunch, so the host issues one call instead of 4 x GRAPH_REPS
 // kernel launches.  Kernel nodes only (no memset or copy nodes); the kernels
 // read the descriptors and the control block at run time, so one graph
-// serves every update of a batch.  FI_NO_GRAPH disables them; benchmark
-// kernel timing and the request server (whose groupings change every round)
-// bypass them.  Measured on the 64-member ensemble they are neutral (the GPU,
-// not the host, bounds the attempt rate); they cut the host's launch work.
+// serves every update of a batch.  Opt-in (FI_GRAPHS=1): measured on the
+// 64-member ensemble they are neutral (the GPU, not the host, bounds the
+// attempt rate) and rocprofv3's kernel tracing crashes the host process
+// inside the graph launch; benchmark kernel timing and the request server
+// (whose groupings change every round) never use them.
 constexpr int GRAPH_REPS = 8, GRAPH_CACHE = 128;
 struct GraphEntry {
     const Member* d;
@@ -456,7 +457,7 @@ void purge_graphs(const Member* lo, const Member* hi) {
 // `count` attempts: whole graphs, then the remainder as plain launches
 int launch_attempts(const Member* d, int nmem, const RhsShape& s, hipStream_t st, int count, bool graphs) {
     int done = 0;
-    if (graphs && count >= GRAPH_REPS && !prof_active() && !std::getenv("FI_NO_GRAPH")) {
+    if (graphs && count >= GRAPH_REPS && !prof_active() && std::getenv("FI_GRAPHS")) {
         hipGraphExec_t ex;
         int rc = attempt_graph(d, nmem, s, st, &ex);
         if (rc) return rc;

@@ -314,9 +314,9 @@ def test_attempt_graphs_match_plain_launches(prec, monkeypatch):
 
     def run(graphs):
         if graphs:
-            monkeypatch.delenv("FI_NO_GRAPH", raising=False)
+            monkeypatch.setenv("FI_GRAPHS", "1")
         else:
-            monkeypatch.setenv("FI_NO_GRAPH", "1")
+            monkeypatch.delenv("FI_GRAPHS", raising=False)
         hs, bs = [], []
         for wl in members:
             p = G.make_params(precision=prec, **wl["params"])
