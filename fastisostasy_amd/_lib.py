@@ -16,7 +16,7 @@ FI_OK, FI_E_ARG, FI_E_UNSUPPORTED, FI_E_HIP, FI_E_UNSTABLE, FI_E_NAN, FI_E_STATE
 FIELDS = ["z_bed", "z_ss", "dz_ss", "w", "we", "dwdt", "canom_full", "Hice",
           "maskocean", "maskgrounded", "maskcontinent",
           "canom_mantle", "Haf", "eta_eff", "D_lith", "kappa", "R", "maskactive", "He_lith",
-          "ode_x", "z_ss_ref", "z_bed_ref"]
+          "ode_x", "z_ss_ref", "z_bed_ref", "w_equilibrium"]
 FIELD_ID = {n: i for i, n in enumerate(FIELDS)}
 OUT_FIELDS = FIELDS[:11]
 SCALARS = ["n", "t", "dt", "bsl", "deltaV", "V_af", "V_den", "attempts", "accepts", "D_lith11", "err", "tol",
@@ -51,6 +51,7 @@ class FiParams(ctypes.Structure):
         ("dt_method", ctypes.c_int), ("atol", ctypes.c_double), ("rtol", ctypes.c_double),
         ("dt_min", ctypes.c_double), ("dt_max", ctypes.c_double),
         ("dt_init", ctypes.c_double), ("q", ctypes.c_double), ("precision", ctypes.c_int),
+        ("tau", ctypes.c_double),
     ]
 
 

@@ -112,6 +112,7 @@ int make_shape(int n, int nmem, bool fp64, bool uniform, RhsShape& s) {
     s.n = n;
     s.fp64 = fp64;
     s.uniform = uniform;
+    s.method = 3;
     s.fast = (n & (n - 1)) == 0 && n >= 64 && n <= 1024 && !std::getenv("FI_GENERIC_FFT");
     if (s.fast) {
         // register-FFT kernels: TPL = n/8 threads per line, LDS <= 64 KiB
@@ -503,6 +504,13 @@ int run_update(std::vector<fi_isos*>& hs, Member* d_mem, const RhsShape& shape, 
         }
     }
     launch_begin(d_mem, nmem, time, stepping ? 1 : 0, times != nullptr, g_stream);
+    if (stepping && shape.method != 3) {
+        if (shape.method == 2) {
+            launch_weq_input(d_mem, nmem, n, g_stream);
+            launch_convolution(d_mem, nmem, conv_shape_of(&h0), 2, g_stream);
+        }
+        launch_weq(d_mem, nmem, n, shape.method, g_stream);
+    }
     HIPCHK(hipGetLastError());
     if (stepping) {
         // Attempts until every member reached `time`.  Members that are done
@@ -522,7 +530,7 @@ int run_update(std::vector<fi_isos*>& hs, Member* d_mem, const RhsShape& shape, 
             ahead = (int)std::min(256.0, std::max(1.0, need));
         }
         long long total = 0;
-        const int K = sub_count(nmem);
+        const int K = shape.method == 3 ? sub_count(nmem) : 1;
         int sb_lo[MAX_SUB + 1];
         const RhsShape* sshape[MAX_SUB] = {};
         if (K > 1) {
@@ -699,7 +707,8 @@ int fi_isos_create(const fi_params* p, int nx_ice, int ny_ice, double dx, double
     int rc = ensure_device();
     if (rc) return rc;
     if (!p || !out) return fail(FI_E_ARG, "NULL argument");
-    if (p->method != 3) return fail(FI_E_UNSUPPORTED, "only method 3 (LV-ELVA) is supported");
+    if (p->method < 0 || p->method > 3)
+        return fail(FI_E_ARG, "method must be 0 (none), 1 (LLRA), 2 (LV-ELRA) or 3 (LV-ELVA)");
     if (p->nl < 1 || p->nl > FI_MAX_LAYERS) return fail(FI_E_ARG, "nl out of range");
     if (p->dt_method < 0 || p->dt_method > 3) return fail(FI_E_ARG, "unknown dt_method");
     if (p->lithosphere == 3 && !he_field) return fail(FI_E_ARG, "lithosphere=fields needs he_field");
@@ -724,7 +733,8 @@ int fi_isos_create(const fi_params* p, int nx_ice, int ny_ice, double dx, double
     const size_t nn = (size_t)n * n;
     rc = make_shape(n, 1, h->fp64, h->uniform, h->shape);
     if (rc) { delete h; return rc; }
-    h->has_conv = p->include_elastic || p->heterogeneous_ssh;
+    h->shape.method = p->method;
+    h->has_conv = p->include_elastic || p->heterogeneous_ssh || p->method == 2;
     if (h->has_conv) {
         rc = make_conv_shape(n, h->fp64, h->cshape);
         if (rc) { delete h; return rc; }
@@ -750,9 +760,11 @@ int fi_isos_create(const fi_params* p, int nx_ice, int ny_ice, double dx, double
     size_t ocor = take((size_t)n * 4 * 8), opart = take((size_t)h->part_blocks * 4 * 8);
     size_t oee = take(f64), okap = take(f64), oR = take(f64), oHe = take(f64), oDd = take(f64);
     size_t oscr = take(f64 + 64);
+    size_t oweq = (p->method != 3) ? take(f64) : 0;
     const size_t hc = (size_t)(n + 1) * (h->fp64 ? 16 : 8);
-    size_t ocw = 0, oFGE = 0, oFGN = 0;
+    size_t ocw = 0, oFGE = 0, oFGN = 0, oFGV = 0;
     if (h->has_conv) { ocw = take(2 * (size_t)n * hc); oFGE = take(2 * (size_t)n * hc); oFGN = take(2 * (size_t)n * hc); }
+    if (p->method == 2) oFGV = take(2 * (size_t)n * hc);
     size_t octl = take(sizeof(Ctl)), octl2 = take(sizeof(Ctl)), ophys = take(sizeof(PhysDev)),
            omem = take(sizeof(Member));
     if (hipMalloc(&h->arena, off) != hipSuccess) { delete h; return fail(FI_E_HIP, "hipMalloc arena (%zu bytes)", off); }
@@ -783,6 +795,8 @@ int fi_isos_create(const fi_params* p, int nx_ice, int ny_ice, double dx, double
     h->He = (double*)(a + oHe); h->Dd = (double*)(a + oDd);
     double* scratch = (double*)(a + oscr);
     if (h->has_conv) { M.conv_work = (double*)(a + ocw); M.FGE = a + oFGE; M.FGN = a + oFGN; }
+    if (p->method == 2) M.FGV = a + oFGV;
+    if (p->method != 3) M.weq = (double*)(a + oweq);
 
     // ---- physics scalars (fastisostasy.f90:152-164)
     const fi_params& P = h->par;
@@ -797,6 +811,7 @@ int fi_isos_create(const fi_params* p, int nx_ice, int ny_ice, double dx, double
     ph.inv_dx = 1.0 / dx;
     ph.inv_2dx = 1.0 / (2 * dx);
     ph.inv_dx2 = 1.0 / (dx * dx);
+    ph.tau = P.tau;
     ph.dt_method = P.dt_method;
     ph.interactive_sealevel = P.interactive_sealevel; ph.include_elastic = P.include_elastic;
     ph.heterogeneous_ssh = P.heterogeneous_ssh;
@@ -822,8 +837,9 @@ int fi_isos_create(const fi_params* p, int nx_ice, int ny_ice, double dx, double
         std::vector<uint8_t> mk(nn, 0);
         for (int j = 0; j < n; ++j)
             for (int i = 0; i < n; ++i) {
+                // methods 0 and 1 set maskactive everywhere (fastisostasy.f90:444-447)
                 bool act = true;
-                if (P.interactive_sealevel)
+                if (P.interactive_sealevel && P.method >= 2)
                     act = (i >= h->icrop1 && i < h->icrop1 + nx_ice && j >= h->jcrop1 && j < h->jcrop1 + ny_ice);
                 mk[(size_t)j * n + i] = act ? 8 : 0;
             }
@@ -857,6 +873,13 @@ int fi_isos_create(const fi_params* p, int nx_ice, int ny_ice, double dx, double
         HIPCHK(hipStreamSynchronize(g_stream));
         if (d_he) (void)hipFree(d_he);
         if (d_eta) (void)hipFree(d_eta);
+    } else {
+        // He_lith = zl(1), D_lith = D_lith_const (fastisostasy.f90:166-172)
+        const double He0 = P.zl[0];
+        const double D0 = P.E * 1e9 * (He0 * He0 * He0) / (12.0 * (1.0 - P.nu * P.nu));
+        std::vector<double> he(nn, He0), dd(nn, D0);
+        HIPCHK(hipMemcpy(h->He, he.data(), nn * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(h->Dd, dd.data(), nn * 8, hipMemcpyHostToDevice));
     }
 
     // ---- Green's functions and their spectra (fastisostasy.f90:174-183)
@@ -872,6 +895,24 @@ int fi_isos_create(const fi_params* p, int nx_ice, int ny_ice, double dx, double
             HIPCHK(hipMemcpy(&kc[2], M.conv_in + (size_t)(n - 1) * n, 8, hipMemcpyDeviceToHost));
             HIPCHK(hipMemcpy(&kc[3], M.conv_in + (n - 1), 8, hipMemcpyDeviceToHost));
             launch_kernel_spectrum(h->d_mem, 1, h->cshape, which, M.conv_in, g_stream);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipStreamSynchronize(g_stream));
+        }
+        if (P.method == 2) {
+            // LV-ELRA: flexural length scale and viscous Green's function
+            // (isos_utils.f90:66-73, fastisostasy.f90:218-236)
+            const double He0 = P.zl[0];
+            const double D0 = P.E * 1e9 * (He0 * He0 * He0) / (12.0 * (1.0 - P.nu * P.nu));
+            const double L_w = std::pow(D0 / (P.rho_uppermantle * P.g), 0.25);
+            std::vector<double> gv(nn);
+            green_viscous_host(n, dx, L_w, D0, gv.data());
+            M.kc_V[0] = gv[0];
+            M.kc_V[1] = gv[nn - 1];
+            M.kc_V[2] = gv[(size_t)(n - 1) * n];
+            M.kc_V[3] = gv[n - 1];
+            HIPCHK(hipMemcpy(M.conv_in, gv.data(), nn * 8, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(h->d_mem, &M, sizeof M, hipMemcpyHostToDevice));
+            launch_kernel_spectrum(h->d_mem, 1, h->cshape, 2, M.conv_in, g_stream);
             HIPCHK(hipGetLastError());
             HIPCHK(hipStreamSynchronize(g_stream));
         }
@@ -1006,6 +1047,7 @@ static const double* field_ptr(fi_isos* h, int f, bool& is_mask, int& bit) {
     case FI_ODE_X: return nullptr;
     case FI_Z_SS_REF: return M.z_ss_ref;
     case FI_Z_BED_REF: return M.z_bed_ref;
+    case FI_W_EQUILIBRIUM: return M.weq;
     default: return nullptr;
     }
 }
@@ -1104,15 +1146,16 @@ int fi_batch_create(fi_isos* const* members, int nmem, fi_batch** out) {
         fi_isos* h = members[m];
         if (!h || h->n != members[0]->n || h->fp64 != members[0]->fp64 || h->uniform != members[0]->uniform ||
             h->par.dt_method != members[0]->par.dt_method || h->par.include_elastic != members[0]->par.include_elastic ||
-            h->par.heterogeneous_ssh != members[0]->par.heterogeneous_ssh) {
+            h->par.heterogeneous_ssh != members[0]->par.heterogeneous_ssh || h->par.method != members[0]->par.method) {
             delete b;
-            return fail(FI_E_ARG, "batch members must share N, precision, distortion and dt_method");
+            return fail(FI_E_ARG, "batch members must share N, precision, distortion, method and dt_method");
         }
         b->mem.push_back(h);
         b->hm.push_back(h->hm);
     }
     int rc = make_shape(members[0]->n, nmem, members[0]->fp64, members[0]->uniform, b->shape);
     if (rc) { delete b; return rc; }
+    b->shape.method = members[0]->par.method;
     if (hipMalloc(&b->d_mem, sizeof(Member) * nmem) != hipSuccess) { delete b; return fail(FI_E_HIP, "hipMalloc"); }
     rc = sum_alloc(b->sb, nmem);
     if (rc) return rc;
@@ -1182,7 +1225,7 @@ struct fi_server {
     Member* h_stage = nullptr;   // pinned staging of the same
     int cap = 0;                 // capacity of d_mem / h_stage / sb
     SumBuf sb;
-    std::map<std::tuple<int, int, int, int>, RhsShape> shapes;  // (n, fp64, uniform, nmem)
+    std::map<std::tuple<int, int, int, int, int>, RhsShape> shapes;  // (n, fp64, uniform, nmem, method)
     long long rounds = 0, groups = 0, served = 0;
 };
 
@@ -1281,7 +1324,7 @@ int fi_server_run(fi_server* s, int* completed) {
         const fi_isos* h = s->slots[i].h;
         const bool d = diag_due(h, s->slots[i].q.front().time);
         return std::make_tuple(h->n, (int)h->fp64, (int)h->uniform, h->par.dt_method, h->par.include_elastic,
-                               h->par.heterogeneous_ssh, (int)d);
+                               h->par.heterogeneous_ssh, (int)d, h->par.method);
     };
     std::vector<std::vector<int>> grp;
     std::vector<decltype(key_of(0))> keys;
@@ -1326,12 +1369,13 @@ int fi_server_run(fi_server* s, int* completed) {
         HIPCHK(hipMemcpyAsync(s->d_mem + base, s->h_stage + base, sizeof(Member) * nm, hipMemcpyHostToDevice,
                               g_stream));
         const fi_isos* h0 = hs[0];
-        auto sk = std::make_tuple(h0->n, (int)h0->fp64, (int)h0->uniform, nm);
+        auto sk = std::make_tuple(h0->n, (int)h0->fp64, (int)h0->uniform, nm, h0->par.method);
         auto it = s->shapes.find(sk);
         if (it == s->shapes.end()) {
             RhsShape shp{};
             rc = make_shape(h0->n, nm, h0->fp64, h0->uniform, shp);
             if (rc) return rc;
+            shp.method = h0->par.method;
             it = s->shapes.emplace(sk, shp).first;
         }
         // groupings change from round to round: plain launches, no graph captures

@@ -58,7 +58,7 @@ __global__ void __launch_bounds__(512) c_rows_r2c(const Member* __restrict__ mem
 }
 
 // pass 2: columns.  mode 0 = spectrum of a kernel (write all M rows to spec);
-// mode 1 = convolve with spec (which: 0 elastic, 1 sea surface) and keep crop rows.
+// mode 1 = convolve with spec (which: 0 elastic, 1 sea surface, 2 viscous) and keep crop rows.
 template <typename T>
 __global__ void __launch_bounds__(512) c_cols(const Member* __restrict__ mem, FftPlan plan,
                                               const void* __restrict__ twv, int C, int mode, int which) {
@@ -77,7 +77,7 @@ __global__ void __launch_bounds__(512) c_cols(const Member* __restrict__ mem, Ff
     }
     __syncthreads();
     lds_fft<T>(buf, plan, C, C, tw);
-    cplx<T>* spec = (cplx<T>*)(which == 0 ? M.FGE : M.FGN);
+    cplx<T>* spec = (cplx<T>*)(which == 0 ? M.FGE : which == 1 ? M.FGN : M.FGV);
     if (mode == 0) {
         for (int q = tid; q < Mm * C; q += nt) {
             const int r = q / C, l = q % C;
@@ -114,6 +114,8 @@ template <typename T>
 __global__ void __launch_bounds__(512) c_rows_c2r(const Member* __restrict__ mem, FftPlan plan,
                                                   const void* __restrict__ twv, int L, int which) {
     const Member& M = mem[blockIdx.y];
+    // w_equilibrium is only recomputed for domains that step in this update
+    if (which == 2 && M.ctl->status != ST_RUN) return;
     extern __shared__ __align__(16) unsigned char smem[];
     cplx<T>* buf = (cplx<T>*)smem;
     __shared__ double corr;
@@ -123,7 +125,7 @@ __global__ void __launch_bounds__(512) c_rows_c2r(const Member* __restrict__ mem
     const int tid = threadIdx.x, nt = blockDim.x;
     if (tid == 0) {
         const double* u = M.conv_in;
-        const double* kc = (which == 0) ? M.kc_E : M.kc_N;
+        const double* kc = (which == 0) ? M.kc_E : (which == 1) ? M.kc_N : M.kc_V;
         const double f00 = u[0] * kc[0];
         const double fee = u[(size_t)N * N - 1] * kc[1];
         const double fe0 = u[(size_t)(N - 1) * N] * kc[2];
@@ -148,7 +150,7 @@ __global__ void __launch_bounds__(512) c_rows_c2r(const Member* __restrict__ mem
     const double scale = (double)Mm * (double)Mm;
     const double cc = corr;
     const int c0 = N / 2 - 1 + (N % 2);
-    double* out = (which == 0) ? M.we : M.dz_ss;
+    double* out = (which == 0) ? M.we : (which == 1) ? M.dz_ss : M.weq;
     for (int q = tid; q < 2 * L * N; q += nt) {
         const int mr = q / N, i = q % N;
         const int l = mr % L, part = mr / L;

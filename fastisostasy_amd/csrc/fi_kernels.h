@@ -21,6 +21,7 @@ struct RhsShape {
     bool fast;        // power-of-two N: register-FFT kernels (threads_row = block size)
     bool fused;       // fast && uniform: stencil fused into the forward row pass
     int row_tiles;    // row blocks per thread block of the inverse row pass (1 or 2)
+    int method;       // 3 LV-ELVA (transform passes); 0-2 pointwise relaxation RHS
 };
 
 // Geometry of the 2N-point linear convolution.
@@ -70,12 +71,16 @@ void launch_begin(const Member* d_mem, int nmem, double tf, int stepping, bool p
 void launch_finalize(const Member* d_mem, int nmem, int n, hipStream_t st);
 void launch_gather(const Member* d_mem, int nmem, CtlSummary* d_out, hipStream_t st);
 void launch_bsl_update(BslDev* b, double year_bp, hipStream_t st);
+// methods 0-2: w_equilibrium of the update (lv_xlra.f90, fastisostasy.f90:686-714);
+// method 2 first stages -canom_load g K^2 (launch_weq_input), then convolves it
+void launch_weq_input(const Member* d_mem, int nmem, int n, hipStream_t st);
+void launch_weq(const Member* d_mem, int nmem, int n, int method, hipStream_t st);
 void launch_set_ref_zss(const Member* d_mem, int nmem, int n, hipStream_t st);
 void launch_init_state_fields(const Member* d_mem, int nmem, int n, hipStream_t st);
 
 // linear convolution with a precomputed kernel spectrum (convolutions.f90:59-91)
 // src_kind: 0 elastic input (canom_load*g*K^2 taken from conv_work staging),
-// which = 0 elastic (writes we), 1 sea surface (writes dz_ss)
+// which = 0 elastic (writes we), 1 sea surface (writes dz_ss), 2 LV-ELRA (writes weq)
 void launch_convolution(const Member* d_mem, int nmem, const ConvShape& s, int which, hipStream_t st);
 void launch_kernel_spectrum(const Member* d_mem, int nmem, const ConvShape& s, int which,
                             const double* d_kernel, hipStream_t st);
@@ -96,5 +101,8 @@ void launch_init_finalize(const InitArgs& a, const double* d_scratch, double* d_
                           const double* d_R, const double* d_Dd, void* D_t, void* eta_t, void* kR_t,
                           bool fp64, hipStream_t st);
 void launch_green(int which, int n, double dx, double r_earth, double m_earth, double* d_out, hipStream_t st);
+// LV-ELRA viscous Green's function on the host (kelvin.cpp)
+void green_viscous_host(int n, double dx, double L_w, double D_lith, double* out);
+double kelvin_kei_value(double x);
 
 }  // namespace fi

@@ -67,10 +67,11 @@ struct PhysDev {
     double atol, rtol, dt_max, q;
     double dx;
     double inv_dx, inv_2dx, inv_dx2;   // 1/dx, 1/(2 dx), 1/(dx*dx), rounded once on the host
+    double tau;                  // [yr] domain%tau of methods 0-2 (uniform, fastisostasy.f90:236,446)
     int dt_method;              // 0 euler, 1 rk4, 2 bs32, 3 tsit54
     int interactive_sealevel, include_elastic, heterogeneous_ssh;
     int uniform_K;               // K == 1 everywhere (dx_matrix scalar)
-    int method;                  // 0 none, 3 LV-ELVA
+    int method;                  // 0 none, 1 LLRA, 2 LV-ELRA, 3 LV-ELVA
 };
 
 // Per-domain device pointers.  Fields are N x N row-major [y][x] (x fastest)
@@ -100,7 +101,9 @@ struct Member {
     double* conv_work;           // convolution work (complex, 2N x (N+1)) or null
     void* FGE;                   // elastic kernel half-spectrum, (2N) x (N+1) complex
     void* FGN;                   // sea-surface kernel half-spectrum
-    double kc_E[4], kc_N[4];     // Green-kernel corner values (00, NN, N0, 0N)
+    void* FGV;                   // LV-ELRA viscous (Kelvin) kernel half-spectrum
+    double kc_E[4], kc_N[4], kc_V[4];   // Green-kernel corner values (00, NN, N0, 0N)
+    double* weq;                 // now%w_equilibrium (methods 0-2), N x N
     Ctl* ctl;
     Ctl* ctl_next;               // decision of the previous attempt, written by the first pass
     BslDev* bsl;

@@ -358,6 +358,42 @@ __global__ void __launch_bounds__(512) k_row_inv(const Member* __restrict__ mem,
     block_reduce3(a1, a2, s2, M.partials + (size_t)blockIdx.x * 4);
 }
 
+// Attempt of methods 0-2: RHS dwdt = (w_equilibrium - now%w) / tau
+// (calc_asthenosphere_relax, lv_xlra.f90:17-26; 0 for method 0), stage sums
+// and error partials.  One block per pair of rows, so the partials have the
+// layout of the power-of-two LV-ELVA path (one entry per row pair).
+constexpr int XLRA_THREADS = 256;
+__global__ void __launch_bounds__(XLRA_THREADS) k_xlra(const Member* __restrict__ mem) {
+    const Member& M = mem[blockIdx.y];
+    const Ctl& c = *M.ctl;
+    if (c.status != ST_RUN) return;
+    const PhysDev& ph = *M.phys;
+    const int N = M.n;
+    const double dt = attempt_dt(c, ph);
+    const double* __restrict__ X = M.X[c.cur];
+    double* __restrict__ Xn = M.X[c.cur ^ 1];
+    const double* __restrict__ w = c.w_is_x ? X : M.W;
+    const double* __restrict__ weq = M.weq;
+    const bool relax = ph.method != 0, store_k = !(c.t + dt < c.tf);
+    const double tau = ph.tau;
+    const int dtm = ph.dt_method;
+    const int r0 = blockIdx.x * 2, rows = min(2, N - r0);
+    double a1 = 0.0, a2 = 0.0, s2 = 0.0;
+    for (int q = threadIdx.x; q < rows * N; q += XLRA_THREADS) {
+        const size_t cidx = (size_t)r0 * N + q;
+        const double k = relax ? (weq[cidx] - w[cidx]) / tau : 0.0;
+        if (store_k) M.dwdt[cidx] = k;
+        double x1;
+        const double x2 = stage_sums(dtm, X[cidx], k, dt, x1);
+        Xn[cidx] = x2;
+        a1 = fmax(a1, fabs(x1)); a2 = fmax(a2, fabs(x2));
+        const double d = x2 - x1;
+        s2 += d * d;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) M.ctl->pending = 1;
+    block_reduce3(a1, a2, s2, M.partials + (size_t)blockIdx.x * 4);
+}
+
 __global__ void __launch_bounds__(64) k_decide(const Member* __restrict__ mem, int nblk) {
     decide_wave(mem[blockIdx.x], nblk);
 }
@@ -415,6 +451,13 @@ int prof_collect(double* total_ms, long long* launches) {
     return 0;
 }
 
+// methods 0-2: one pointwise kernel, decided at once
+void launch_attempt_xlra(const Member* d_mem, int nmem, const RhsShape& s, hipStream_t st) {
+    const int nblk = (s.n + 1) / 2;
+    launch_k(3, k_xlra, dim3(nblk, nmem), dim3(XLRA_THREADS), 0, st, d_mem);
+    hipLaunchKernelGGL(k_decide, dim3(nmem), dim3(64), 0, st, d_mem, nblk);
+}
+
 template <typename T>
 void launch_attempt_t(const Member* d_mem, int nmem, const RhsShape& s, hipStream_t st) {
     dim3 gs((s.n + ST_TX - 1) / ST_TX, (s.n + ST_TY - 1) / ST_TY, nmem);
@@ -451,7 +494,8 @@ void launch_decide(const Member* d_mem, int nmem, const RhsShape& s, hipStream_t
 }
 
 void launch_attempt(const Member* d_mem, int nmem, const RhsShape& s, hipStream_t st) {
-    if (s.fp64) launch_attempt_t<double>(d_mem, nmem, s, st);
+    if (s.method != 3) launch_attempt_xlra(d_mem, nmem, s, st);
+    else if (s.fp64) launch_attempt_t<double>(d_mem, nmem, s, st);
     else launch_attempt_t<float>(d_mem, nmem, s, st);
 }
 

@@ -130,6 +130,7 @@ __global__ void __launch_bounds__(PRE_THREADS) k_pre_fields(const Member* __rest
     const double rho_ice = ph->rho_ice, rho_water = ph->rho_water, rho_litho = ph->rho_litho, g = ph->g;
     const bool uniform = ph->uniform_K, interactive = ph->interactive_sealevel, elastic = ph->include_elastic,
                hssh = ph->heterogeneous_ssh;
+    const bool keep_load = ph->method != 3;   // methods 0-2 relax towards an equilibrium of canom_load
     const double A_uni = ph->dx * ph->dx;
     gmut<uint8_t> mask = gout<uint8_t>(M.mask);
     gmut<double> zss = gout<double>(M.z_ss), dzss = gout<double>(M.dz_ss), we = gout<double>(M.we),
@@ -155,6 +156,7 @@ __global__ void __launch_bounds__(PRE_THREADS) k_pre_fields(const Member* __rest
         return active ? load : 0.0;
     };
     auto full_of = [&](size_t c, bool active, double load, double wec) {
+        if (keep_load) cload[c] = load;   // the last calc_columnanoms_load of the update (:833)
         const double litho = 0.0 + rho_litho * (wec - we_ref[c]);
         const double full = active ? (load + litho + cmantle[c]) : 0.0;
         cfull[c] = full;
@@ -209,6 +211,66 @@ __global__ void __launch_bounds__(PRE_THREADS) k_pre_fields(const Member* __rest
         }
         full_of(c, active, load, we[c]);
     }
+}
+
+// LV-ELRA input of the viscous convolution, -canom_load g K^2
+// (fastisostasy.f90:702-708), for domains that step in this update.
+__global__ void __launch_bounds__(PRE_THREADS) k_weq_input(const Member* __restrict__ mem) {
+    const Member& M = mem[blockIdx.y];
+    if (kview(M.ctl)->status != ST_RUN) return;
+    const kptr<PhysDev> ph = kview(M.phys);
+    const size_t nn = (size_t)M.n * M.n;
+    const double g = ph->g;
+    const bool uniform = ph->uniform_K;
+    gptr<double> cload = gin<double>(M.canom_load);
+    gmut<double> conv = gout<double>(M.conv_in);
+#pragma unroll
+    for (int it = 0; it < PRE_CELLS / PRE_THREADS; ++it) {
+        const size_t c = (size_t)blockIdx.x * PRE_CELLS + it * PRE_THREADS + threadIdx.x;
+        if (c >= nn) break;
+        const double K2 = uniform ? 1.0 : M.K2[c];
+        conv[c] = -(cload[c] * g * K2);
+    }
+}
+
+// w_equilibrium of methods 0-2 for domains that step in this update (the
+// reference recomputes it in every RHS call from fields that stay fixed
+// during the update: get_dwdt, fastisostasy.f90:686-714).
+//   0: 0;  1: -canom_load / rho_uppermantle (lv_xlra.f90:29-38);
+//   2: corner BC (isos_utils.f90:104-118) of the viscous convolution in weq.
+// One block per domain, so the corner values are read before any is changed.
+constexpr int WEQ_THREADS = 1024;
+__global__ void __launch_bounds__(WEQ_THREADS) k_weq(const Member* __restrict__ mem, int method) {
+    const Member& M = mem[blockIdx.x];
+    if (kview(M.ctl)->status != ST_RUN) return;
+    const int N = M.n;
+    const size_t nn = (size_t)N * N;
+    gmut<double> weq = gout<double>(M.weq);
+    if (method == 0) {
+        for (size_t c = threadIdx.x; c < nn; c += WEQ_THREADS) weq[c] = 0.0;
+        return;
+    }
+    if (method == 1) {
+        const double rho_um = kview(M.phys)->rho_uppermantle;
+        gptr<double> cload = gin<double>(M.canom_load);
+        for (size_t c = threadIdx.x; c < nn; c += WEQ_THREADS) weq[c] = -cload[c] / rho_um;
+        return;
+    }
+    const double* raw = M.weq;
+    const double cc = 0.25 * (raw[0] + raw[nn - 1] + raw[(size_t)(N - 1) * N] + raw[N - 1]);
+    __syncthreads();
+    for (size_t c = threadIdx.x; c < nn; c += WEQ_THREADS) weq[c] = weq[c] - cc;
+}
+
+void launch_weq_input(const Member* d_mem, int nmem, int n, hipStream_t st) {
+    const size_t nn = (size_t)n * n;
+    hipLaunchKernelGGL(k_weq_input, dim3((unsigned)((nn + PRE_CELLS - 1) / PRE_CELLS), nmem), dim3(PRE_THREADS), 0, st,
+                       d_mem);
+}
+
+void launch_weq(const Member* d_mem, int nmem, int n, int method, hipStream_t st) {
+    (void)n;
+    hipLaunchKernelGGL(k_weq, dim3(nmem), dim3(WEQ_THREADS), 0, st, d_mem, method);
 }
 
 // Corner correction of now%w (isos_utils.f90:104-110), one thread per domain.

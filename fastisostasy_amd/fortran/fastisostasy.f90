@@ -47,6 +47,7 @@ module fastisostasy
         integer(c_int) :: dt_method
         real(c_double) :: atol, rtol, dt_min, dt_max, dt_init, q
         integer(c_int) :: precision
+        real(c_double) :: tau
     end type
 
     type, public :: isos_param_class
@@ -314,6 +315,7 @@ contains
         p%dt_method = pick(dt_method, [character(len=16) :: "euler", "rk4", "bs32", "tsit54"])
         p%atol = atol; p%rtol = rtol; p%dt_min = dt_min; p%dt_max = dt_max; p%dt_init = dt_init; p%q = q
         p%precision = default_precision
+        p%tau = tau
     end subroutine read_isostasy_nml
 
     ! isos_init (src/fastisostasy.f90:64-473)

@@ -48,6 +48,7 @@ DEFAULT_PARAMS = dict(
     rho_water=1000.0, rho_ice=910.0, rho_seawater=1028.0, rho_uppermantle=3400.0,
     rho_litho=0.0, E=66.0, nu=0.28, g=9.81, r_earth=6.378e3 * 1e3, m_earth=5.972e24,
     dt_method="tsit54", atol=1e-2, rtol=1e-4, dt_min=1e-10, dt_max=10.0, dt_init=1.0, q=1.0,
+    tau=3000.0,
 )
 
 
@@ -119,6 +120,17 @@ def plain_disc():
     return dict(name="plain_disc", params=p, nx=nx, dx=dx, H_ice=H, z_bed=z_bed,
                 H_init=np.zeros_like(H), t0=0.0, t1=50e3, dtt=10.0, dt_out=1e3,
                 he_field=None, eta_field=None)
+
+
+def relax_disc(method):
+    """The plain-disc setup with the relaxation methods of the reference's
+    test1a-c parameter sets (par/test_isostasy_test1{a,b,c}.nml): method 0
+    (no displacement), 1 (LLRA) or 2 (LV-ELRA), tau = 3000 yr,
+    rho_uppermantle = 3400 kg/m^3, He = 88 km."""
+    wl = plain_disc()
+    wl["name"] = "relax_disc_%d" % method
+    wl["params"] = dict(wl["params"], method=method, tau=3000.0, rho_uppermantle=3400.0)
+    return wl
 
 
 def lv_rheology():

@@ -51,7 +51,7 @@ enum fi_precision { FI_FP64 = 0, FI_MIXED = 1 };
 typedef struct {
     int heterogeneous_ssh, interactive_sealevel, include_elastic;
     int correct_compression, correct_distortion;
-    int method;                   /* 3 = LV-ELVA (0 = no displacement) */
+    int method;                   /* 0 no displacement, 1 LLRA, 2 LV-ELRA, 3 LV-ELVA */
     double dt_diagnostics;        /* [yr] */
     double min_pad;               /* [m]  namelist "pad" */
     int mantle;                   /* 0 uniform, 1 gaussian_plus, 2 gaussian_minus, 3 fields */
@@ -71,6 +71,7 @@ typedef struct {
     int dt_method;                /* 0 euler, 1 rk4, 2 bs32, 3 tsit54 */
     double atol, rtol, dt_min, dt_max, dt_init, q;
     int precision;                /* enum fi_precision */
+    double tau;                   /* [yr] asthenosphere relaxation time (methods 0-2) */
 } fi_params;
 
 /* ice-grid output components of isos_out_class (isostasy_defs.f90:185-200) */
@@ -79,7 +80,9 @@ enum fi_field {
     FI_MASKOCEAN, FI_MASKGROUNDED, FI_MASKCONTINENT,
     /* padded-grid extras (fi_isos_get_full) */
     FI_CANOM_MANTLE, FI_HAF, FI_ETA_EFF, FI_D_LITH, FI_KAPPA, FI_R, FI_MASKACTIVE, FI_HE_LITH,
-    FI_ODE_X, FI_Z_SS_REF, FI_Z_BED_REF, FI_NFIELDS
+    FI_ODE_X, FI_Z_SS_REF, FI_Z_BED_REF,
+    FI_W_EQUILIBRIUM,       /* now%w_equilibrium (methods 0-2) */
+    FI_NFIELDS
 };
 
 enum fi_scalar {

@@ -301,6 +301,71 @@ void or_green_ssh(int n, double dx, double r_earth, double m_earth, double *gk) 
 }
 
 /* ------------------------------------------------------------------------ */
+/* Kelvin function of order zero, Abramowitz & Stegun 9.9.10-9.9.12:
+ *   ber x = sum_k (-1)^k (x^2/4)^(2k) / ((2k)!)^2
+ *   bei x = sum_k (-1)^k (x^2/4)^(2k+1) / ((2k+1)!)^2
+ *   kei x = -ln(x/2) bei x - (pi/4) ber x + sum_k (-1)^k psi(2k+2) (x^2/4)^(2k+1) / ((2k+1)!)^2
+ * with psi(m+1) = -gamma + H_m.  Summed in long double: at x = 10 the terms
+ * reach 1e3 for a result of 3e-4, which leaves ~13 correct digits. */
+double or_kei(double xd) {
+    const long double pi = 3.14159265358979323846264338327950288L;
+    const long double euler_gamma = 0.57721566490153286060651209008240243L;
+    if (xd == 0.0) return (double)(-pi / 4);
+    const long double x = xd, q = x * x / 4;
+    long double t = 1.0L, H = 0.0L, ber = 0.0L, bei = 0.0L, ser = 0.0L;
+    for (int m = 0; m < 80; m++) {
+        if (m > 0) { t = t * q / ((long double)m * (long double)m); H += 1.0L / m; }
+        const long double sg = ((m / 2) % 2) ? -1.0L : 1.0L;
+        if (m % 2 == 0) ber += sg * t;
+        else { bei += sg * t; ser += sg * (H - euler_gamma) * t; }
+    }
+    return (double)(-logl(x / 2) * bei - pi / 4 * ber + ser);
+}
+
+/* load_kei_values (kelvin_function.f90:167-1190) tabulates kei at
+ * x = 0.00, 0.01, ..., 10.00 to five significant digits as default-real
+ * literals; regenerated here from or_kei with the same decimal rounding and
+ * the same conversion to single precision. */
+void or_kei_table(double *rn, double *kei) {
+    char buf[64];
+    for (int k = 0; k <= 1000; k++) {
+        snprintf(buf, sizeof buf, "%.4f", k / 100.0);
+        rn[k] = (double)strtof(buf, NULL);
+        snprintf(buf, sizeof buf, "%.4e", or_kei(k / 100.0));
+        kei[k] = (double)strtof(buf, NULL);
+    }
+}
+
+/* get_kei_value, kelvin_function.f90:96-131.  A radius of exactly 10 L_w
+ * makes the reference's search run past the table (k = n); the last entry is
+ * returned there. */
+static double kei_value(double r, double L_w, const double *rn, const double *kv) {
+    const int n = 1001;
+    const double x = r / L_w;
+    if (x > rn[n - 1]) return kv[n - 1];
+    int k;
+    for (k = 0; k < n - 1; k++)
+        if (x >= rn[k] && x < rn[k + 1]) break;
+    if (k == n - 1) return kv[n - 1];
+    return kv[k] + (x - rn[k]) / (rn[k + 1] - rn[k]) * (kv[k + 1] - kv[k]);
+}
+
+/* calc_kei_filter_2D (kelvin_function.f90:22-94) on the domain coordinates
+ * of init_dims (isos_utils.f90:221-240), then calc_viscous_green
+ * (green_functions.f90:28-41): GV = -L_w^2 / (2 pi D) * kei * (dx dy) */
+void or_green_viscous(int n, double dx, double L_w, double D_lith, double *gv) {
+    double rn[1001], kv[1001];
+    or_kei_table(rn, kv);
+    const double c = L_w * L_w / (2.0 * PI_REF * D_lith);
+    for (int j = 0; j < n; j++)
+        for (int i = 0; i < n; i++) {
+            const double x = (double)(i - n / 2) * dx, y = (double)(j - n / 2) * dx;
+            const double kei = kei_value(sqrt(x * x + y * y), L_w, rn, kv);
+            gv[(size_t)j * n + i] = -(c * kei * (dx * dx));
+        }
+}
+
+/* ------------------------------------------------------------------------ */
 /* precompute_kernel / precomputed_fftconvolution, convolutions.f90:29-108 */
 typedef struct {
     int n, m;
@@ -487,7 +552,7 @@ void or_bsl_destroy(or_bsl *b) { if (b) { free(b->bsl_vec); free(b->A_vec); free
 typedef struct {
     double bsl, V_af, V_pov, V_den, deltaV_bsl, dbsl_total;
     double *z_bed, *dwdt, *w, *we, *Haf, *Hice, *z_ss, *dz_ss, *canom_ice, *canom_seawater,
-           *canom_litho, *canom_mantle, *canom_load, *canom_full, *mass_anom;
+           *canom_litho, *canom_mantle, *canom_load, *canom_full, *mass_anom, *w_eq;
     unsigned char *maskocean, *maskgrounded, *maskcontinent;
 } state_t;
 
@@ -498,7 +563,8 @@ struct or_isos {
     double time_diagnostics, time_prognostics;
     int ref_was_set;
     double *K, *dxm, *A, *kappa, *R, *maskactive, *He_lith, *D_lith, *eta_eff;
-    cplx *FGE, *FGN;
+    cplx *FGE, *FGN, *FGV;
+    double tau, L_w;            /* domain%tau (uniform), par%L_w */
     state_t now, ref;
     double ode_t, ode_dt, *ode_x, *ode_x1, *ode_x2, *k;
     long attempts, accepts;
@@ -508,7 +574,8 @@ struct or_isos {
 
 static void state_alloc(state_t *s, size_t nn) {
     double **f[] = {&s->z_bed, &s->dwdt, &s->w, &s->we, &s->Haf, &s->Hice, &s->z_ss, &s->dz_ss, &s->canom_ice,
-                    &s->canom_seawater, &s->canom_litho, &s->canom_mantle, &s->canom_load, &s->canom_full, &s->mass_anom};
+                    &s->canom_seawater, &s->canom_litho, &s->canom_mantle, &s->canom_load, &s->canom_full, &s->mass_anom,
+                    &s->w_eq};
     for (size_t i = 0; i < sizeof f / sizeof f[0]; i++) *f[i] = (double *)calloc(nn, sizeof(double));
     s->maskocean = (unsigned char *)calloc(nn, 1);
     s->maskgrounded = (unsigned char *)calloc(nn, 1);
@@ -516,7 +583,8 @@ static void state_alloc(state_t *s, size_t nn) {
 }
 static void state_free(state_t *s) {
     double *f[] = {s->z_bed, s->dwdt, s->w, s->we, s->Haf, s->Hice, s->z_ss, s->dz_ss, s->canom_ice,
-                   s->canom_seawater, s->canom_litho, s->canom_mantle, s->canom_load, s->canom_full, s->mass_anom};
+                   s->canom_seawater, s->canom_litho, s->canom_mantle, s->canom_load, s->canom_full, s->mass_anom,
+                   s->w_eq};
     for (size_t i = 0; i < sizeof f / sizeof f[0]; i++) free(f[i]);
     free(s->maskocean); free(s->maskgrounded); free(s->maskcontinent);
 }
@@ -568,7 +636,7 @@ static void gaussian_shape(int n, const double *xc, const double *yc, double *f)
 or_isos *or_isos_create(const or_params *p, int nx_ice, int ny_ice, double dx, double dy,
                         const double *K, const double *rho_ice,
                         const double *he_field, const double *eta_field) {
-    if (p->method != 3 && p->method != 0) { set_err("only method 0 and 3 (LV-ELVA) are supported"); return NULL; }
+    if (p->method < 0 || p->method > 3) { set_err("method must be 0 (none), 1 (LLRA), 2 (LV-ELRA) or 3 (LV-ELVA)"); return NULL; }
     if (p->nl < 1 || p->nl > OR_MAXL) { set_err("nl out of range"); return NULL; }
     or_isos *s = (or_isos *)calloc(1, sizeof *s);
     s->par = *p;
@@ -614,6 +682,18 @@ or_isos *or_isos_create(const or_params *p, int nx_ice, int ny_ice, double dx, d
         free(gk); free(xc); or_isos_destroy(s); return NULL;
     }
 
+    /* fastisostasy.f90:216-236,444-450 */
+    if (s->par.method == 2) {
+        s->L_w = pow(D0 / (s->par.rho_uppermantle * s->par.g), 0.25);   /* calc_flexural_lengthscale, isos_utils.f90:66-73 */
+        or_green_viscous(n, dx, s->L_w, D0, gk);
+        s->FGV = (cplx *)malloc(sizeof(cplx) * mm);
+        conv_spectrum(s->conv, gk, s->FGV);
+        s->tau = s->par.tau;
+    } else if (s->par.method != 3) {
+        s->tau = s->par.tau;
+        for (size_t c = 0; c < nn; c++) s->maskactive[c] = 1;
+        s->L_w = 0.0;
+    }
     if (s->par.method == 3) {
         or_kappa(n, dx, s->kappa);
         switch (s->par.lithosphere) {
@@ -697,7 +777,7 @@ void or_isos_destroy(or_isos *s) {
     double *f[] = {s->K, s->dxm, s->A, s->kappa, s->R, s->maskactive, s->He_lith, s->D_lith, s->eta_eff,
                    s->ode_x, s->ode_x1, s->ode_x2, s->k};
     for (size_t i = 0; i < sizeof f / sizeof f[0]; i++) free(f[i]);
-    free(s->FGE); free(s->FGN);
+    free(s->FGE); free(s->FGN); free(s->FGV);
     if (s->now.z_bed) state_free(&s->now);
     if (s->ref.z_bed) state_free(&s->ref);
     rhs_destroy(s->rhs); conv_destroy(s->conv);
@@ -792,7 +872,27 @@ static void sl_contributions(or_isos *s) {
 /* get_dwdt, fastisostasy.f90:677-729 (the w argument is ignored, now%w is used) */
 static void get_dwdt(or_isos *s) {
     const size_t nn = (size_t)s->n * s->n;
-    if (s->par.method == 0) { memset(s->now.dwdt, 0, sizeof(double) * nn); return; }
+    if (s->par.method == 0) {
+        memset(s->now.w_eq, 0, sizeof(double) * nn);
+        memset(s->now.dwdt, 0, sizeof(double) * nn);
+        return;
+    }
+    if (s->par.method == 1 || s->par.method == 2) {
+        if (s->par.method == 1) {
+            /* calc_llra_equilibrium, lv_xlra.f90:29-38 */
+            for (size_t c = 0; c < nn; c++) s->now.w_eq[c] = -s->now.canom_load[c] / s->par.rho_uppermantle;
+        } else {
+            /* fastisostasy.f90:702-712: convolution of -canom_load g K^2 with GV, corner BC */
+            double *in = (double *)calloc(nn, sizeof(double));
+            for (size_t c = 0; c < nn; c++) in[c] = -s->now.canom_load[c] * s->par.g * pow(s->K[c], 2.0);
+            conv_apply(s->conv, s->FGV, in, s->now.w_eq);
+            free(in);
+            zerobc(s->n, s->now.w_eq);
+        }
+        /* calc_asthenosphere_relax, lv_xlra.f90:17-26 */
+        for (size_t c = 0; c < nn; c++) s->now.dwdt[c] = (s->now.w_eq[c] - s->now.w[c]) / s->tau;
+        return;
+    }
     canom_full(s);
     rhs_eval(s->rhs, s->now.w, s->now.canom_full, s->maskactive, s->par.g, s->par.nu, s->D_lith, s->eta_eff,
              s->R, s->kappa, s->dxm, s->sec_per_year, s->now.dwdt);
@@ -973,6 +1073,7 @@ static const double *field_ptr(const or_isos *s, int f, int *is_mask, const unsi
     case OR_W_REF: return s->ref.w;
     case OR_HAF_REF: return s->ref.Haf;
     case OR_Z_SS_REF: return s->ref.z_ss;
+    case OR_W_EQUILIBRIUM: return s->now.w_eq;
     default: return NULL;
     }
 }

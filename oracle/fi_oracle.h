@@ -25,7 +25,7 @@ extern "C" {
 typedef struct {
     int heterogeneous_ssh, interactive_sealevel, include_elastic;
     int correct_compression, correct_distortion;
-    int method;                 /* 0 none, 3 LV-ELVA */
+    int method;                 /* 0 none, 1 LLRA, 2 LV-ELRA, 3 LV-ELVA */
     double dt_diagnostics;      /* [yr] */
     double min_pad;             /* [m] */
     int mantle;                 /* 0 uniform, 1 gaussian_plus, 2 gaussian_minus, 3 fields */
@@ -44,6 +44,7 @@ typedef struct {
     double nu, g, r_earth, m_earth;
     int dt_method;              /* 0 euler, 1 rk4, 2 bs32, 3 tsit54 */
     double atol, rtol, dt_min, dt_max, dt_init, q;
+    double tau;                 /* [yr] asthenosphere relaxation time (methods 0-2) */
 } or_params;
 
 typedef struct or_isos or_isos;
@@ -57,7 +58,7 @@ enum {
     OR_CANOM_LOAD, OR_CANOM_MANTLE, OR_MASS_ANOM, OR_HAF, OR_ETA_EFF, OR_D_LITH,
     OR_KAPPA, OR_R, OR_MASKACTIVE, OR_HE_LITH, OR_ODE_X, OR_CANOM_SEAWATER,
     OR_CANOM_ICE, OR_CANOM_LITHO, OR_Z_BED_REF, OR_W_REF, OR_HAF_REF, OR_Z_SS_REF,
-    OR_NFIELDS
+    OR_W_EQUILIBRIUM, OR_NFIELDS
 };
 /* scalar ids */
 enum {
@@ -102,6 +103,14 @@ void or_lumping(int n, int nl, int lumping, const double *eta, const double *bnd
                 const double *kappa, double *eta_eff, double *R);
 void or_green_elastic(int n, double dx, double *ge);
 void or_green_ssh(int n, double dx, double r_earth, double m_earth, double *gn);
+/* Kelvin function kei_0(x) (series of Abramowitz & Stegun 9.9.12) and the
+ * tabulation kei(0.00 .. 10.00 step 0.01) of load_kei_values
+ * (kelvin_function.f90:167-1190): 5 significant digits, single precision */
+double or_kei(double x);
+void or_kei_table(double *rn, double *kei);
+/* calc_kei_filter_2D + calc_viscous_green (kelvin_function.f90:22-94,
+ * green_functions.f90:28-41) on the n x n domain grid */
+void or_green_viscous(int n, double dx, double L_w, double D_lith, double *gv);
 void or_convolve(int n, const double *kernel, const double *in, double *out);
 void or_lvelva(int n, const double *w, const double *canom_full, const double *maskactive,
                double g, double nu, const double *D, const double *eta, const double *R,

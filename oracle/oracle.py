@@ -22,7 +22,7 @@ FIELDS = ["z_bed", "z_ss", "dz_ss", "w", "we", "dwdt", "canom_full", "Hice",
           "maskocean", "maskgrounded", "maskcontinent",
           "canom_load", "canom_mantle", "mass_anom", "Haf", "eta_eff", "D_lith",
           "kappa", "R", "maskactive", "He_lith", "ode_x", "canom_seawater",
-          "canom_ice", "canom_litho", "z_bed_ref", "w_ref", "Haf_ref", "z_ss_ref"]
+          "canom_ice", "canom_litho", "z_bed_ref", "w_ref", "Haf_ref", "z_ss_ref", "w_equilibrium"]
 FIELD_ID = {name: i for i, name in enumerate(FIELDS)}
 SCALARS = ["n", "t", "dt", "bsl", "deltaV", "V_af", "V_den", "attempts", "accepts",
            "D_lith11", "icrop1", "jcrop1", "nx_ice", "ny_ice"]
@@ -54,7 +54,7 @@ class OrParams(ctypes.Structure):
         ("g", ctypes.c_double), ("r_earth", ctypes.c_double), ("m_earth", ctypes.c_double),
         ("dt_method", ctypes.c_int), ("atol", ctypes.c_double), ("rtol", ctypes.c_double),
         ("dt_min", ctypes.c_double), ("dt_max", ctypes.c_double),
-        ("dt_init", ctypes.c_double), ("q", ctypes.c_double),
+        ("dt_init", ctypes.c_double), ("q", ctypes.c_double), ("tau", ctypes.c_double),
     ]
 
 
@@ -102,6 +102,10 @@ def lib():
         L.or_lumping.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp, dp, dp, dp]
         L.or_green_elastic.argtypes = [ctypes.c_int, ctypes.c_double, dp]
         L.or_green_ssh.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, dp]
+        L.or_kei.argtypes = [ctypes.c_double]
+        L.or_kei.restype = ctypes.c_double
+        L.or_kei_table.argtypes = [dp, dp]
+        L.or_green_viscous.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, dp]
         L.or_convolve.argtypes = [ctypes.c_int, dp, dp, dp]
         L.or_lvelva.argtypes = [ctypes.c_int, dp, dp, dp, ctypes.c_double, ctypes.c_double,
                                 dp, dp, dp, dp, dp, ctypes.c_double, dp]
@@ -190,6 +194,22 @@ def green_elastic(n, dx):
 def green_ssh(n, dx, r_earth, m_earth):
     out = np.empty((n, n))
     lib().or_green_ssh(n, dx, r_earth, m_earth, _p(out))
+    return out
+
+
+def kei(x):
+    return lib().or_kei(float(x))
+
+
+def kei_table():
+    rn, kv = np.empty(1001), np.empty(1001)
+    lib().or_kei_table(_p(rn), _p(kv))
+    return rn, kv
+
+
+def green_viscous(n, dx, L_w, D_lith):
+    out = np.empty((n, n))
+    lib().or_green_viscous(n, dx, L_w, D_lith, _p(out))
     return out
 
 

@@ -343,3 +343,59 @@ def test_attempt_graphs_match_plain_launches(prec, monkeypatch):
     for (wa, da, na), (wb, db, nb) in zip(a, b):
         assert na == nb
         assert np.array_equal(wa, wb) and np.array_equal(da, db)
+
+
+# ---------------------------------------------------------------- methods 0-2
+@pytest.mark.parametrize("method,prec", [(0, "fp64"), (1, "fp64"), (2, "fp64"), (1, "mixed"), (2, "mixed")])
+def test_relaxation_methods_match_oracle(method, prec):
+    """No displacement, LLRA and LV-ELRA (the reference's test1a-c setups) over
+    400 yr of 10-yr coupling steps: displacement within 1e-8 m (FP64) or 1 mm
+    (mixed: the viscous convolution runs in FP32), w_equilibrium alike, masks
+    bit-exact."""
+    wl = W.relax_disc(method)
+    times = [0.0] + [10.0 * k for k in range(1, 41)]
+    for o, g in run_pair(wl, times, precision=prec):
+        pass
+    print(f"method {method} {prec}: attempts {g.scalar('attempts'):.0f} / {o.scalar('attempts'):.0f}", flush=True)
+    fp64 = prec == "fp64"
+    compare(o, g, wtol=1e-8 if fp64 else 1e-3, rel=None if fp64 else 1e-5)
+    a, b = o.get_full("w_equilibrium"), g.full("w_equilibrium")
+    assert np.abs(a - b).max() <= (1e-8 if fp64 else 1e-3) * max(1.0, np.abs(a).max() / 100), np.abs(a - b).max()
+    if method == 0:
+        assert np.all(g.out("w") == 0) and np.all(b == 0)
+    else:
+        assert g.out("w").min() < -10.0   # the domain is relaxing under the load
+    if method == 1:
+        assert np.array_equal(a, b)       # pointwise equilibrium: same bits
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_relaxation_batch_matches_single(prec):
+    """LV-ELRA domains with different loads: each member of a batch gives the
+    same bits as the same domain updated alone."""
+    wls = []
+    for k, scale in enumerate((1.0, 0.6, 1.4)):
+        wl = W.relax_disc(2)
+        wl["H_ice"] = wl["H_ice"] * scale
+        wls.append(wl)
+    singles, gs, bsls = [], [], []
+    for wl in wls:
+        p = G.make_params(precision=prec, **wl["params"])
+        a = G.IsosClass(p, wl["nx"], wl["nx"], wl["dx"])
+        b = G.IsosClass(p, wl["nx"], wl["nx"], wl["dx"])
+        ba, bb = G.bsl_init(), G.bsl_init()
+        a.init_state(wl["z_bed"], np.zeros_like(wl["H_ice"]), 0.0, ba)
+        b.init_state(wl["z_bed"], np.zeros_like(wl["H_ice"]), 0.0, bb)
+        singles.append((a, ba))
+        gs.append(b)
+        bsls.append(bb)
+    batch = G.Batch(gs)
+    for k in range(1, 11):
+        t = 10.0 * k
+        for (a, ba), wl in zip(singles, wls):
+            a.update(wl["H_ice"], t, ba)
+        batch.update(t, bsls, H_ice=[wl["H_ice"] for wl in wls])
+    for (a, _), b in zip(singles, gs):
+        for f in ("w", "z_bed", "dwdt"):
+            assert np.array_equal(a.out(f), b.out(f)), f
+        assert np.array_equal(a.full("w_equilibrium"), b.full("w_equilibrium"))

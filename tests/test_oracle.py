@@ -11,7 +11,12 @@ term, wrong sign, wrong index or period, transposed operand) fails it:
   * direct linear convolution, the corner correction and the +1-cell crop,
   * the degenerate Tsitouras controller replayed from its definition,
   * masks and the barystatic update on hand-built fields,
-  * the analytical ELVA disc (tests/golden/elva_disc_analytic.json).
+  * the analytical ELVA disc (tests/golden/elva_disc_analytic.json),
+  * the Kelvin-function table of LV-ELRA against tests/golden/kei_table.json
+    (kei_0 by mpmath, identical to the reference's table), the series against
+    scipy, and the viscous Green's function against the mass a flexed plate
+    displaces (rho_m g sum GV = -int_0^10 x kei(x) dx inside 10 L_w),
+  * LLRA relaxation against its closed form.
 """
 import json
 import os
@@ -299,3 +304,85 @@ def test_elva_disc_against_analytical_solution():
             o.update(wl["H_ice"], t, b)
         wc = o.get("w").min()
         assert abs(wc - want[target]) / abs(want[target]) < 0.08, (target, wc, want[target])
+
+
+# ---------------------------------------------------------------- LLRA / LV-ELRA
+def test_kei_table_matches_golden():
+    """load_kei_values (kelvin_function.f90:167-1190): 1001 nodes, five
+    significant digits, single-precision literals -- all entries identical."""
+    gold = json.load(open(os.path.join(HERE, "golden", "kei_table.json")))
+    rn, kv = O.kei_table()
+    want_x = np.array([np.float32(v) for v in gold["x"]], dtype=np.float64)
+    want_k = np.array([np.float32(v) for v in gold["kei"]], dtype=np.float64)
+    assert np.array_equal(rn, want_x)
+    assert np.array_equal(kv, want_k), np.flatnonzero(kv != want_k)[:10]
+
+
+def test_kei_series_against_mpmath_and_scipy():
+    import mpmath
+    from scipy import special
+    xs = np.concatenate([np.linspace(1e-3, 10.0, 200), [0.5, 1.0, 2.5, 7.25]])
+    got = np.array([O.kei(x) for x in xs])
+    want = np.array([float(mpmath.kei(0, x)) for x in xs])
+    assert np.all(np.abs(got - want) <= 1e-15 + 1e-11 * np.abs(want)), np.abs(got - want).max()
+    assert np.allclose(got, special.kei(xs), rtol=1e-8, atol=1e-13)
+    assert O.kei(0.0) == -np.pi / 4
+
+
+def test_viscous_green_displaces_load_mass():
+    """A thin elastic plate on a fluid displaces the mass of its load:
+    rho_m g sum(GV) dx^2-weighted equals -int_0^R x kei(x) dx (R = 10 L_w, the
+    end of the table; beyond it the reference holds kei at its last value)."""
+    from scipy import integrate, special
+    D, rho_m, g = 4.07e24, 3400.0, 9.81
+    L = (D / (rho_m * g)) ** 0.25
+    dx, n = L / 10, 200
+    gv = O.green_viscous(n, dx, L, D)
+    x = (np.arange(n) - n // 2) * dx
+    X, Y = np.meshgrid(x, x)
+    inside = np.sqrt(X * X + Y * Y) <= 10 * L
+    want = -integrate.quad(lambda s: s * special.kei(s), 0, 10, limit=400)[0]
+    assert abs(rho_m * g * gv[inside].sum() - want) < 5e-5
+    # centre at (n/2, n/2) and the flat tail beyond 10 L_w
+    assert gv[n // 2, n // 2] == gv.max()
+    assert gv[0, 0] == gv[0, 1] == gv[-1, 0]
+
+
+def test_viscous_green_on_table_nodes():
+    """With L_w = 2 dx the cells on the axes sit on table nodes (r / L_w =
+    m / 2), where get_kei_value returns the table entry itself."""
+    gold = json.load(open(os.path.join(HERE, "golden", "kei_table.json")))
+    kv = np.array([np.float32(v) for v in gold["kei"]], dtype=np.float64)
+    D, dx, n = 1.0e24, 10e3, 64
+    L = 2 * dx
+    gv = O.green_viscous(n, dx, L, D)
+    c = L * L / (2.0 * PI_F * D)
+    for m in range(0, 20):
+        want = -(c * kv[50 * m] * (dx * dx))
+        assert gv[n // 2, n // 2 + m] == want and gv[n // 2 + m, n // 2] == want
+
+
+def test_llra_relaxation_closed_form():
+    """LLRA on the disc: w_eq = -rho_i H / rho_m exactly; the explicit
+    relaxation w' = (w_eq - w) / tau leaves (w_eq - w) / w_eq the same in every
+    loaded cell, between exp(-t / tau) and the Euler factor of 10-yr steps."""
+    wl = W.relax_disc(1)
+    p = wl["params"]
+    o = O.OracleIsos(O.make_params(**p), wl["nx"], wl["nx"], wl["dx"])
+    b = O.OracleBSL("const", 0.0)
+    o.init_state(wl["z_bed"], np.zeros_like(wl["H_ice"]), 0.0, b)
+    t = 0.0
+    while t < 3000.0:
+        t += 10.0
+        o.update(wl["H_ice"], t, b)
+    weq = o.get_full("w_equilibrium")
+    H = o.get_full("Hice")
+    loaded = H > 0
+    assert np.array_equal(weq[loaded], -(p["rho_ice"] * H[loaded]) / p["rho_uppermantle"])
+    assert np.all(weq[~loaded] == 0)
+    w = o.get_full("w")
+    # the load of the first update acts from the start of its step (t = 0)
+    frac = (weq[loaded] - w[loaded]) / weq[loaded]
+    assert np.ptp(frac) < 1e-12
+    tau, T = p["tau"], 3000.0
+    assert (1 - 10.0 / tau) ** (T / 10.0) <= frac[0] <= np.exp(-T / tau) * (1 + 1e-12)
