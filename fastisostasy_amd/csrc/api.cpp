@@ -239,6 +239,7 @@ struct SumBuf {
 struct fi_bsl {
     BslDev* d = nullptr;
     BslDev h{};
+    std::vector<double> st, sv;   // sea-level series of the methods file and mixed (time yr, bsl m)
 };
 
 struct fi_isos {
@@ -635,12 +636,49 @@ int fi_sync(void) {
     return 0;
 }
 
-int fi_bsl_create(int method, double bsl_init, double A_ocean_pd, int nvec, const double* bsl_vec,
-                  const double* A_vec, double year_bp, fi_bsl** out) {
+namespace {
+
+// interp_0d (isos_utils.f90:423-457) and interp_linear (barysealevel.f90:370-400):
+// the same rule -- clamp outside [x(1), x(n)], else the first x(j) >= xout
+double interp_first_ge(const double* x, const double* y, int n, double xout) {
+    if (xout < x[0]) return y[0];
+    if (xout > x[n - 1]) return y[n - 1];
+    int j = 0;
+    while (j < n && !(x[j] >= xout)) ++j;
+    if (j == 0) return y[0];
+    if (j == n) return y[n - 1];
+    const double al = (xout - x[j - 1]) / (x[j] - x[j - 1]);
+    return y[j - 1] + al * (y[j] - y[j - 1]);
+}
+
+// bsl_init_fastiso (barysealevel.f90:155-193): bsl_init from the parameters,
+// the A_ocean(bsl) table corrected for its present-day bias at bsl_init
+void bsl_init_fastiso(BslDev& d, double bsl_init, int nvec, const double* bsl_vec, const double* A_vec) {
+    d.bsl_init = bsl_init;
+    if (nvec == 0) {
+        d.constant_ocean_surface = 1;
+        d.A_ocean_now = d.A_ocean_pd;
+        return;
+    }
+    d.nvec = nvec;
+    for (int i = 0; i < nvec; ++i) { d.bsl_vec[i] = bsl_vec[i]; d.A_vec[i] = A_vec[i]; }
+    const double biased = interp_first_ge(d.bsl_vec, d.A_vec, nvec, d.bsl_init);
+    for (int i = 0; i < nvec; ++i) d.A_vec[i] = d.A_vec[i] * d.A_ocean_pd / biased;
+    d.A_ocean_now = interp_first_ge(d.bsl_vec, d.A_vec, nvec, d.bsl_init);
+}
+
+int bsl_create_impl(int method, double bsl_init, double A_ocean_pd, int nvec, const double* bsl_vec,
+                    const double* A_vec, int nseries, const double* st, const double* sv, double year_bp,
+                    fi_bsl** out) {
     int rc = ensure_device();
     if (rc) return rc;
-    if (method != 0 && method != 1) return fail(FI_E_UNSUPPORTED, "bsl method must be const (0) or fastiso (1)");
+    if (!out) return fail(FI_E_ARG, "NULL argument");
+    if (method < 0 || method > 3)
+        return fail(FI_E_ARG, "bsl_init:: Error: method not recognized (0 const, 1 fastiso, 2 file, 3 mixed)");
     if (nvec < 0 || nvec > 64) return fail(FI_E_ARG, "A_ocean table must have 0..64 entries");
+    if (nvec > 0 && (!bsl_vec || !A_vec)) return fail(FI_E_ARG, "A_ocean table is NULL");
+    const bool series = method == 2 || method == 3;
+    if (series && (nseries < 1 || !st || !sv)) return fail(FI_E_ARG, "bsl methods file and mixed need a sea-level series");
     fi_bsl* b = new fi_bsl();
     BslDev& d = b->h;
     d.method = method;
@@ -648,39 +686,77 @@ int fi_bsl_create(int method, double bsl_init, double A_ocean_pd, int nvec, cons
     d.A_ocean_pd = A_ocean_pd;
     d.A_ocean_now = A_ocean_pd;
     d.bsl_init = bsl_init;
-    if (method == 1) {
-        if (nvec == 0) {
-            d.constant_ocean_surface = 1;
-        } else {
-            // bsl_init_fastiso (barysealevel.f90:176-188): correct the pd bias of the table
-            d.nvec = nvec;
-            for (int i = 0; i < nvec; ++i) { d.bsl_vec[i] = bsl_vec[i]; d.A_vec[i] = A_vec[i]; }
-            auto interp = [&](double x) {
-                const int n = d.nvec;
-                if (x < d.bsl_vec[0]) return d.A_vec[0];
-                if (x > d.bsl_vec[n - 1]) return d.A_vec[n - 1];
-                int j;
-                for (j = 0; j < n; ++j) if (d.bsl_vec[j] >= x) break;
-                if (j == 0) return d.A_vec[0];
-                if (j == n) return d.A_vec[n - 1];
-                const double al = (x - d.bsl_vec[j - 1]) / (d.bsl_vec[j] - d.bsl_vec[j - 1]);
-                return d.A_vec[j - 1] + al * (d.A_vec[j] - d.A_vec[j - 1]);
-            };
-            const double biased = interp(d.bsl_init);
-            for (int i = 0; i < nvec; ++i) d.A_vec[i] = d.A_vec[i] * d.A_ocean_pd / biased;
-            d.A_ocean_now = interp(d.bsl_init);
-        }
+    if (series) {
+        // bsl_init_file (barysealevel.f90:108-153): bsl_init = the series at year_bp
+        b->st.assign(st, st + nseries);
+        b->sv.assign(sv, sv + nseries);
+        d.bsl_now = interp_first_ge(b->st.data(), b->sv.data(), nseries, year_bp);
+        d.bsl_init = d.bsl_now;
     }
-    d.bsl_now = d.bsl_init;
+    // mixed re-reads bsl_init from the parameters in bsl_init_fastiso
+    if (method == 1 || method == 3) bsl_init_fastiso(d, bsl_init, nvec, bsl_vec, A_vec);
+    d.bsl_now = d.bsl_init;   // barysealevel.f90:100
     if (hipMalloc(&b->d, sizeof(BslDev)) != hipSuccess) { delete b; return fail(FI_E_HIP, "hipMalloc bsl"); }
     HIPCHK(hipMemcpy(b->d, &d, sizeof(BslDev), hipMemcpyHostToDevice));
     *out = b;
     return 0;
 }
 
+}  // namespace
+
+int fi_bsl_create(int method, double bsl_init, double A_ocean_pd, int nvec, const double* bsl_vec,
+                  const double* A_vec, double year_bp, fi_bsl** out) {
+    if (method == 2 || method == 3)
+        return fail(FI_E_ARG, "bsl methods file and mixed take a series: use fi_bsl_create_series");
+    return bsl_create_impl(method, bsl_init, A_ocean_pd, nvec, bsl_vec, A_vec, 0, nullptr, nullptr, year_bp, out);
+}
+
+int fi_bsl_create_series(int method, double bsl_init, double A_ocean_pd, int nvec, const double* bsl_vec,
+                         const double* A_vec, int nseries, const double* series_time, const double* series_bsl,
+                         double year_bp, fi_bsl** out) {
+    return bsl_create_impl(method, bsl_init, A_ocean_pd, nvec, bsl_vec, A_vec, nseries, series_time, series_bsl,
+                           year_bp, out);
+}
+
+// read_series (barysealevel.f90:254-315): the first line is a header; data
+// lines hold "time value"; lines starting with '#' or '!' and blank lines are
+// skipped (the reference counts them as entries it never fills).
+int fi_bsl_read_series(const char* path, int max_n, double* time, double* var, int* n_out) {
+    if (!path || !n_out || max_n < 0 || (max_n > 0 && (!time || !var))) return fail(FI_E_ARG, "NULL argument");
+    FILE* f = std::fopen(path, "r");
+    if (!f) return fail(FI_E_ARG, "read_series:: cannot open %s", path);
+    char line[4096];
+    int n = 0;
+    bool header = true;
+    while (std::fgets(line, sizeof line, f)) {
+        if (header) { header = false; continue; }
+        const char* p = line;
+        while (*p == ' ' || *p == '\t') ++p;
+        if (*p == '\0' || *p == '\n' || *p == '\r' || *p == '#' || *p == '!') continue;
+        double a, v;
+        if (std::sscanf(p, "%lf%*[ ,\t]%lf", &a, &v) != 2) {
+            std::fclose(f);
+            return fail(FI_E_ARG, "read_series:: bad line %d in %s", n + 2, path);
+        }
+        if (n >= max_n) {
+            std::fclose(f);
+            return fail(FI_E_ARG, "read_series:: more than %d entries in %s", max_n, path);
+        }
+        time[n] = a;
+        var[n] = v;
+        ++n;
+    }
+    std::fclose(f);
+    *n_out = n;
+    return 0;
+}
+
 int fi_bsl_update(fi_bsl* b, double year_bp) {
     if (!b) return fail(FI_E_ARG, "bsl is NULL");
-    launch_bsl_update(b->d, year_bp, g_stream);
+    // bsl_update_file (barysealevel.f90:230-238): the series at the caller's time
+    const double file_value =
+        b->st.empty() ? 0.0 : interp_first_ge(b->st.data(), b->sv.data(), (int)b->st.size(), year_bp);
+    launch_bsl_update(b->d, year_bp, file_value, g_stream);
     HIPCHK(hipGetLastError());
     return 0;
 }

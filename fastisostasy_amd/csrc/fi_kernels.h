@@ -70,7 +70,7 @@ void launch_pre_fields(const Member* d_mem, int nmem, int n, int phase, bool dia
 void launch_begin(const Member* d_mem, int nmem, double tf, int stepping, bool per_member, hipStream_t st);
 void launch_finalize(const Member* d_mem, int nmem, int n, hipStream_t st);
 void launch_gather(const Member* d_mem, int nmem, CtlSummary* d_out, hipStream_t st);
-void launch_bsl_update(BslDev* b, double year_bp, hipStream_t st);
+void launch_bsl_update(BslDev* b, double year_bp, double file_value, hipStream_t st);
 // methods 0-2: w_equilibrium of the update (lv_xlra.f90, fastisostasy.f90:686-714);
 // method 2 first stages -canom_load g K^2 (launch_weq_input), then convolves it
 void launch_weq_input(const Member* d_mem, int nmem, int n, hipStream_t st);

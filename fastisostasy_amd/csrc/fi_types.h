@@ -49,10 +49,11 @@ struct CtlSummary {
 };
 
 // Barystatic sea level, resident in device memory (bsl_class,
-// barysealevel.f90:16-29; methods const and fastiso).
+// barysealevel.f90:16-29).  The sea-level series of the methods file and
+// mixed stays on the host, which interpolates it at each bsl_update.
 struct BslDev {
     double bsl_init, bsl_now, time, A_ocean_pd, A_ocean_now;
-    int method;                  // 0 const, 1 fastiso
+    int method;                  // 0 const, 1 fastiso, 2 file, 3 mixed
     int constant_ocean_surface;
     int nvec;
     int pad_;

@@ -109,6 +109,11 @@ __global__ void __launch_bounds__(64) k_pre_bsl(const Member* __restrict__ mem, 
             c.deltaV = (c.V_af + c.V_den + c.V_pov) - V_old;
             BslDev& b = *M.bsl;
             if (b.method == 1) b.bsl_now = b.bsl_now - c.deltaV / b.A_ocean_now;
+            if (b.method == 3) {
+                // mixed: the domain's cumulative contribution comes off the series value (:771-775)
+                c.dbsl_total = c.dbsl_total + c.deltaV / b.A_ocean_now;
+                b.bsl_now = b.bsl_now - c.dbsl_total;
+            }
             c.bsl = b.bsl_now;
         }
         __syncthreads();
@@ -371,10 +376,13 @@ void launch_gather(const Member* d_mem, int nmem, CtlSummary* d_out, hipStream_t
     hipLaunchKernelGGL(k_gather, dim3((nmem + 63) / 64), dim3(64), 0, st, d_mem, nmem, d_out);
 }
 
-// barysealevel.f90:195-252 (const / fastiso), one thread
-__global__ void k_bsl_update(BslDev* b, double year_bp) {
+// barysealevel.f90:195-252 (const / fastiso / file / mixed), one thread
+__global__ void k_bsl_update(BslDev* b, double year_bp, double file_value) {
     b->time = year_bp;
     if (b->method == 0) { b->bsl_now = b->bsl_init; return; }
+    // file and mixed: the series value, interpolated by the host (bsl_update_file, :230-238)
+    if (b->method == 2 || b->method == 3) b->bsl_now = file_value;
+    if (b->method == 2) return;
     if (b->constant_ocean_surface) { b->A_ocean_now = b->A_ocean_pd; return; }
     // interp_0d (isos_utils.f90:423-457)
     const int n = b->nvec;
@@ -441,8 +449,8 @@ void launch_finalize(const Member* d_mem, int nmem, int n, hipStream_t st) {
     hipLaunchKernelGGL(k_finalize, dim3(pre_blocks(n), nmem), dim3(PRE_THREADS), 0, st, d_mem);
     hipLaunchKernelGGL(k_fin_check, dim3(nmem), dim3(64), 0, st, d_mem, pre_blocks(n));
 }
-void launch_bsl_update(BslDev* b, double year_bp, hipStream_t st) {
-    hipLaunchKernelGGL(k_bsl_update, dim3(1), dim3(1), 0, st, b, year_bp);
+void launch_bsl_update(BslDev* b, double year_bp, double file_value, hipStream_t st) {
+    hipLaunchKernelGGL(k_bsl_update, dim3(1), dim3(1), 0, st, b, year_bp, file_value);
 }
 void launch_init_state_fields(const Member* d_mem, int nmem, int n, hipStream_t st) {
     hipLaunchKernelGGL(k_init_state, dim3(pre_blocks(n), nmem), dim3(PRE_THREADS), 0, st, d_mem);

@@ -176,6 +176,24 @@ module fastisostasy
             type(c_ptr), intent(out) :: out
             integer(c_int) :: rc
         end function
+        function c_fi_bsl_create_series(method, bsl_init, A_pd, nvec, bvec, avec, ns, st, sv, year_bp, out) &
+                bind(C, name="fi_bsl_create_series") result(rc)
+            import :: c_int, c_ptr, c_double
+            integer(c_int), value :: method, nvec, ns
+            real(c_double), value :: bsl_init, A_pd, year_bp
+            type(c_ptr), value :: bvec, avec
+            real(c_double), intent(in) :: st(*), sv(*)
+            type(c_ptr), intent(out) :: out
+            integer(c_int) :: rc
+        end function
+        function c_fi_bsl_read_series(path, max_n, t, v, n) bind(C, name="fi_bsl_read_series") result(rc)
+            import :: c_int, c_double, c_char
+            character(kind=c_char), intent(in) :: path(*)
+            integer(c_int), value :: max_n
+            real(c_double), intent(out) :: t(*), v(*)
+            integer(c_int), intent(out) :: n
+            integer(c_int) :: rc
+        end function
         function c_fi_bsl_update(b, year_bp) bind(C, name="fi_bsl_update") result(rc)
             import :: c_int, c_ptr, c_double
             type(c_ptr), value :: b
@@ -484,10 +502,10 @@ contains
         if (allocated(isos%domain%D_lith)) deallocate(isos%domain%D_lith)
     end subroutine isos_end
 
-    ! bsl_init (src/barysealevel.f90:40-106) for the methods const and fastiso
-    subroutine read_barysealevel_nml(filename, method, bsl0, A_pd)
+    ! bsl_init (src/barysealevel.f90:40-153): methods const, fastiso, file, mixed
+    subroutine read_barysealevel_nml(filename, method, bsl0, A_pd, series_path)
         character(len=*), intent(IN) :: filename
-        character(len=256), intent(OUT) :: method
+        character(len=256), intent(OUT) :: method, series_path
         real(dp), intent(OUT) :: bsl0, A_pd
         character(len=256) :: sl_path, sl_name, A_ocean_path, restart
         real(dp) :: bsl_init, A_ocean_pd
@@ -504,28 +522,47 @@ contains
             error stop "fastisostasy: A_ocean_path tables are not supported by the namelist shim"
         bsl0 = bsl_init
         A_pd = A_ocean_pd
+        series_path = sl_path
     end subroutine read_barysealevel_nml
 
     subroutine bsl_init(bsl, filename, year_bp)
         type(bsl_class), intent(INOUT) :: bsl
         character(len=*), intent(IN) :: filename
         real(wp), intent(IN) :: year_bp
-        character(len=256) :: method
+        character(len=256) :: method, series_path
         real(dp) :: bsl0, A_pd
-        integer :: m
-        call read_barysealevel_nml(filename, method, bsl0, A_pd)
+        integer, parameter :: nmax = 10000
+        real(c_double), allocatable :: st(:), sv(:)
+        integer(c_int) :: ns
+        integer :: m, k
+        call read_barysealevel_nml(filename, method, bsl0, A_pd, series_path)
         select case (trim(method))
         case ("const"); m = 0
         case ("fastiso"); m = 1
-        case default; error stop "fastisostasy: bsl method must be const or fastiso"
+        case ("file"); m = 2
+        case ("mixed"); m = 3
+        case default; error stop "bsl_init:: Error: method not recognized."
         end select
-        call check(c_fi_bsl_create(int(m, c_int), real(bsl0, c_double), real(A_pd, c_double), 0_c_int, &
-            c_null_ptr, c_null_ptr, real(year_bp, c_double), bsl%h), "bsl_init")
+        if (m <= 1) then
+            call check(c_fi_bsl_create(int(m, c_int), real(bsl0, c_double), real(A_pd, c_double), 0_c_int, &
+                c_null_ptr, c_null_ptr, real(year_bp, c_double), bsl%h), "bsl_init")
+        else
+            ! read_series (barysealevel.f90:254-315); NetCDF series need the netCDF library
+            k = len_trim(series_path)
+            if (k >= 2) then
+                if (series_path(k-1:k) == "nc") error stop "fastisostasy: NetCDF sea-level series are not supported"
+            end if
+            allocate(st(nmax), sv(nmax))
+            call check(c_fi_bsl_read_series(trim(series_path)//c_null_char, int(nmax, c_int), st, sv, ns), &
+                "read_series")
+            call check(c_fi_bsl_create_series(int(m, c_int), real(bsl0, c_double), real(A_pd, c_double), 0_c_int, &
+                c_null_ptr, c_null_ptr, ns, st, sv, real(year_bp, c_double), bsl%h), "bsl_init")
+        end if
         bsl%method = method
-        bsl%bsl_init = bsl0
         bsl%A_ocean_pd = A_pd
         bsl%time = year_bp
         call bsl_refresh(bsl)
+        bsl%bsl_init = bsl%bsl_now   ! bsl_now = bsl_init after bsl_init (barysealevel.f90:100)
     end subroutine bsl_init
 
     ! bsl_update (src/barysealevel.f90:195-228)

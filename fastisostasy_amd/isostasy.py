@@ -29,7 +29,7 @@ LUMPING = {"bottom": 0, "min": 1, "max": 2, "mean": 3, "logmean": 4, "freqdomain
 VSCALING = {"none": 0, "min": 1, "scale": 2, "max": 3, "stddev": 4}
 DT_METHOD = {"euler": 0, "rk4": 1, "bs32": 2, "tsit54": 3}
 PRECISION = {"fp64": 0, "mixed": 1}
-BSL_METHOD = {"const": 0, "fastiso": 1}
+BSL_METHOD = {"const": 0, "fastiso": 1, "file": 2, "mixed": 3}
 
 _ENUMS = {"mantle": MANTLE, "lithosphere": LITHOSPHERE, "layering": LAYERING, "lumping": LUMPING,
           "viscosity_scaling_method": VSCALING, "dt_method": DT_METHOD, "precision": PRECISION}
@@ -72,16 +72,23 @@ def padded_size(nx_ice, ny_ice, dx, dy, min_pad):
 
 
 class BslClass:
-    """bsl_class (src/barysealevel.f90:16-29); methods const and fastiso."""
+    """bsl_class (src/barysealevel.f90:16-29): methods const, fastiso, file and
+    mixed; `series` = (time_yr, bsl_m) is the sea-level series of sl_path
+    (read_series gives it from an ASCII file)."""
 
     def __init__(self, method="const", bsl_init=0.0, A_ocean_pd=3.625e14, bsl_vec=None, A_vec=None,
-                 year_bp=0.0):
+                 year_bp=0.0, series=None):
         n = 0 if bsl_vec is None else len(bsl_vec)
         self._bv, self._av = _c(bsl_vec), _c(A_vec)
         h = ctypes.c_void_p()
         m = BSL_METHOD[method] if isinstance(method, str) else int(method)
-        check(load().fi_bsl_create(m, bsl_init, A_ocean_pd, n, _dp(self._bv), _dp(self._av), year_bp,
-                                   ctypes.byref(h)))
+        if series is None:
+            check(load().fi_bsl_create(m, bsl_init, A_ocean_pd, n, _dp(self._bv), _dp(self._av), year_bp,
+                                       ctypes.byref(h)))
+        else:
+            st, sv = _c(series[0]), _c(series[1])
+            check(load().fi_bsl_create_series(m, bsl_init, A_ocean_pd, n, _dp(self._bv), _dp(self._av),
+                                              len(st), _dp(st), _dp(sv), year_bp, ctypes.byref(h)))
         self.h = h
 
     @property
@@ -191,8 +198,17 @@ class Batch:
 
 
 # ---- reference-named procedures (src/fastisostasy.f90:40-50, barysealevel.f90:32-33)
-def bsl_init(method="const", bsl_init=0.0, A_ocean_pd=3.625e14, year_bp=0.0, bsl_vec=None, A_vec=None):
-    return BslClass(method, bsl_init, A_ocean_pd, bsl_vec, A_vec, year_bp)
+def bsl_init(method="const", bsl_init=0.0, A_ocean_pd=3.625e14, year_bp=0.0, bsl_vec=None, A_vec=None,
+             series=None):
+    return BslClass(method, bsl_init, A_ocean_pd, bsl_vec, A_vec, year_bp, series)
+
+
+def read_series(path, max_n=10000):
+    """read_series (src/barysealevel.f90:254-315): (time, value) arrays of an
+    ASCII series file."""
+    t, v, n = np.empty(max_n), np.empty(max_n), ctypes.c_int()
+    check(load().fi_bsl_read_series(str(path).encode(), max_n, _dp(t), _dp(v), ctypes.byref(n)))
+    return t[:n.value].copy(), v[:n.value].copy()
 
 
 def bsl_update(bsl, year_bp):

@@ -132,6 +132,18 @@ int fi_isos_destroy(fi_isos* h);
  * A_ocean_path (otherwise the ocean surface is constant). */
 int fi_bsl_create(int method, double bsl_init, double A_ocean_pd, int nvec, const double* bsl_vec,
                   const double* A_vec, double year_bp, fi_bsl** out);
+/* bsl_init for the series methods 2 = file and 3 = mixed
+ * (src/barysealevel.f90:40-153): series_time (yr) / series_bsl (m) is the
+ * sea-level series of sl_path; mixed also takes the fastiso arguments.
+ * fi_bsl_update then interpolates the series at year_bp (interp_linear,
+ * :370-400); mixed subtracts each domain's cumulative contribution in
+ * isos_update (src/fastisostasy.f90:771-775). */
+int fi_bsl_create_series(int method, double bsl_init, double A_ocean_pd, int nvec, const double* bsl_vec,
+                         const double* A_vec, int nseries, const double* series_time,
+                         const double* series_bsl, double year_bp, fi_bsl** out);
+/* read_series (src/barysealevel.f90:254-315): ASCII [time, value] columns
+ * after one header line; '#' / '!' comment lines skipped.  Host only. */
+int fi_bsl_read_series(const char* path, int max_n, double* time, double* var, int* n_out);
 int fi_bsl_update(fi_bsl* b, double year_bp);
 int fi_bsl_get(fi_bsl* b, double* bsl_now, double* A_ocean_now);
 int fi_bsl_destroy(fi_bsl* b);

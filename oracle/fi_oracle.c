@@ -506,46 +506,76 @@ double or_interp_0d(int n, const double *x, const double *y, double xout) {
 }
 
 /* ------------------------------------------------------------------------ */
-/* bsl_class, barysealevel.f90:16-29, 40-106, 155-252 (methods const / fastiso) */
+/* bsl_class, barysealevel.f90:16-29, 40-252 (methods const, fastiso, file, mixed) */
 struct or_bsl {
-    int method;                 /* 0 const, 1 fastiso */
+    int method;                 /* 0 const, 1 fastiso, 2 file, 3 mixed */
     double bsl_init, bsl_now, time, A_ocean_pd, A_ocean_now;
-    int constant_ocean_surface, nvec;
+    int constant_ocean_surface, nvec, ns;
     double *bsl_vec, *A_vec;
+    double *st, *sv;            /* series%time, series%var */
 };
 
-or_bsl *or_bsl_create(int method, double bsl_init, double A_ocean_pd, int nvec,
-                      const double *bsl_vec, const double *A_vec, double year_bp) {
-    if (method != 0 && method != 1) { set_err("bsl: method not supported"); return NULL; }
+/* bsl_init_fastiso, barysealevel.f90:155-193 */
+static void bsl_init_fastiso(or_bsl *b, double bsl_init, int nvec, const double *bsl_vec, const double *A_vec) {
+    b->bsl_init = bsl_init;
+    if (nvec <= 0) {
+        b->constant_ocean_surface = 1; b->A_ocean_now = b->A_ocean_pd;
+        return;
+    }
+    b->constant_ocean_surface = 0;
+    b->nvec = nvec;
+    b->bsl_vec = (double *)malloc(sizeof(double) * (size_t)nvec);
+    b->A_vec = (double *)malloc(sizeof(double) * (size_t)nvec);
+    memcpy(b->bsl_vec, bsl_vec, sizeof(double) * (size_t)nvec);
+    memcpy(b->A_vec, A_vec, sizeof(double) * (size_t)nvec);
+    const double biased = or_interp_0d(nvec, b->bsl_vec, b->A_vec, b->bsl_init);
+    for (int i = 0; i < nvec; i++) b->A_vec[i] = b->A_vec[i] * b->A_ocean_pd / biased;
+    b->A_ocean_now = or_interp_0d(nvec, b->bsl_vec, b->A_vec, b->bsl_init);
+}
+
+/* series_interp / interp_linear, barysealevel.f90:353-400 (the interp_0d rule) */
+static double series_interp(const or_bsl *b, double year_bp) { return or_interp_0d(b->ns, b->st, b->sv, year_bp); }
+
+/* bsl_init, barysealevel.f90:40-106; bsl_init_file :108-153 */
+or_bsl *or_bsl_create_series(int method, double bsl_init, double A_ocean_pd, int nvec, const double *bsl_vec,
+                             const double *A_vec, int ns, const double *st, const double *sv, double year_bp) {
+    if (method < 0 || method > 3) { set_err("bsl: method not recognized"); return NULL; }
+    if ((method == 2 || method == 3) && ns < 1) { set_err("bsl: file and mixed need a series"); return NULL; }
     or_bsl *b = (or_bsl *)calloc(1, sizeof *b);
     b->method = method; b->time = year_bp; b->A_ocean_pd = A_ocean_pd; b->A_ocean_now = A_ocean_pd;
     b->bsl_init = bsl_init;
-    if (method == 1) {
-        if (nvec <= 0) {
-            b->constant_ocean_surface = 1; b->A_ocean_now = A_ocean_pd;
-        } else {
-            b->nvec = nvec;
-            b->bsl_vec = (double *)malloc(sizeof(double) * (size_t)nvec);
-            b->A_vec = (double *)malloc(sizeof(double) * (size_t)nvec);
-            memcpy(b->bsl_vec, bsl_vec, sizeof(double) * (size_t)nvec);
-            memcpy(b->A_vec, A_vec, sizeof(double) * (size_t)nvec);
-            const double biased = or_interp_0d(nvec, b->bsl_vec, b->A_vec, b->bsl_init);
-            for (int i = 0; i < nvec; i++) b->A_vec[i] = b->A_vec[i] * b->A_ocean_pd / biased;
-            b->A_ocean_now = or_interp_0d(nvec, b->bsl_vec, b->A_vec, b->bsl_init);
-        }
+    if (method == 2 || method == 3) {
+        b->ns = ns;
+        b->st = (double *)malloc(sizeof(double) * (size_t)ns);
+        b->sv = (double *)malloc(sizeof(double) * (size_t)ns);
+        memcpy(b->st, st, sizeof(double) * (size_t)ns);
+        memcpy(b->sv, sv, sizeof(double) * (size_t)ns);
+        b->bsl_now = series_interp(b, year_bp);
+        b->bsl_init = b->bsl_now;
     }
+    if (method == 1 || method == 3) bsl_init_fastiso(b, bsl_init, nvec, bsl_vec, A_vec);
     b->bsl_now = b->bsl_init;
     return b;
 }
+
+or_bsl *or_bsl_create(int method, double bsl_init, double A_ocean_pd, int nvec,
+                      const double *bsl_vec, const double *A_vec, double year_bp) {
+    if (method != 0 && method != 1) { set_err("bsl: file and mixed need a series"); return NULL; }
+    return or_bsl_create_series(method, bsl_init, A_ocean_pd, nvec, bsl_vec, A_vec, 0, NULL, NULL, year_bp);
+}
+
+/* bsl_update, barysealevel.f90:195-252 */
 void or_bsl_update(or_bsl *b, double year_bp) {
     b->time = year_bp;
     if (b->method == 0) { b->bsl_now = b->bsl_init; return; }
-    if (b->constant_ocean_surface) b->A_ocean_now = b->A_ocean_pd;
+    if (b->method == 2 || b->method == 3) b->bsl_now = series_interp(b, year_bp);   /* bsl_update_file */
+    if (b->method == 2) return;
+    if (b->constant_ocean_surface) b->A_ocean_now = b->A_ocean_pd;                   /* bsl_update_fastiso */
     else b->A_ocean_now = or_interp_0d(b->nvec, b->bsl_vec, b->A_vec, b->bsl_now);
 }
 double or_bsl_now(const or_bsl *b) { return b->bsl_now; }
 double or_bsl_A_ocean(const or_bsl *b) { return b->A_ocean_now; }
-void or_bsl_destroy(or_bsl *b) { if (b) { free(b->bsl_vec); free(b->A_vec); free(b); } }
+void or_bsl_destroy(or_bsl *b) { if (b) { free(b->bsl_vec); free(b->A_vec); free(b->st); free(b->sv); free(b); } }
 
 /* ------------------------------------------------------------------------ */
 /* isos_class (isostasy_defs.f90:202-209) */
@@ -982,6 +1012,10 @@ static int isos_update_impl(or_isos *s, const double *H_ice, double time, or_bsl
     calc_Haf(s, &s->now);
     sl_contributions(s);
     if (b->method == 1) b->bsl_now = b->bsl_now - s->now.deltaV_bsl / b->A_ocean_now;
+    if (b->method == 3) {       /* fastisostasy.f90:771-775 */
+        s->now.dbsl_total = s->now.dbsl_total + s->now.deltaV_bsl / b->A_ocean_now;
+        b->bsl_now = b->bsl_now - s->now.dbsl_total;
+    }
     s->now.bsl = b->bsl_now;
     calc_z_ss(nn, s->now.z_ss, s->now.bsl, s->ref.z_ss, s->now.dz_ss);
     int diag = 0;

@@ -86,6 +86,8 @@ const char *or_last_error(void);
 or_bsl *or_bsl_create(int method, double bsl_init, double A_ocean_pd,
                       int nvec, const double *bsl_vec, const double *A_vec,
                       double year_bp);
+or_bsl *or_bsl_create_series(int method, double bsl_init, double A_ocean_pd, int nvec, const double *bsl_vec,
+                             const double *A_vec, int ns, const double *st, const double *sv, double year_bp);
 void   or_bsl_update(or_bsl *b, double year_bp);
 double or_bsl_now(const or_bsl *b);
 double or_bsl_A_ocean(const or_bsl *b);

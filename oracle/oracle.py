@@ -34,7 +34,7 @@ LAYERING = {"parallel": 0, "equalized": 1}
 LUMPING = {"bottom": 0, "min": 1, "max": 2, "mean": 3, "logmean": 4, "freqdomain": 5}
 VSCALING = {"none": 0, "min": 1, "scale": 2, "max": 3, "stddev": 4}
 DT_METHOD = {"euler": 0, "rk4": 1, "bs32": 2, "tsit54": 3}
-BSL_METHOD = {"const": 0, "fastiso": 1}
+BSL_METHOD = {"const": 0, "fastiso": 1, "file": 2, "mixed": 3}
 
 
 class OrParams(ctypes.Structure):
@@ -88,6 +88,9 @@ def lib():
         L.or_bsl_create.restype = vp
         L.or_bsl_create.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                     dp, dp, ctypes.c_double]
+        L.or_bsl_create_series.restype = vp
+        L.or_bsl_create_series.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                           dp, dp, ctypes.c_int, dp, dp, ctypes.c_double]
         L.or_bsl_update.argtypes = [vp, ctypes.c_double]
         L.or_bsl_now.argtypes = [vp]
         L.or_bsl_now.restype = ctypes.c_double
@@ -237,11 +240,16 @@ def interp_0d(x, y, xout):
 # ---- isos_class / bsl_class mirrors -------------------------------------
 class OracleBSL:
     def __init__(self, method="const", bsl_init=0.0, A_ocean_pd=3.625e14, bsl_vec=None,
-                 A_vec=None, year_bp=0.0):
+                 A_vec=None, year_bp=0.0, series=None):
         n = 0 if bsl_vec is None else len(bsl_vec)
         bv = None if bsl_vec is None else _c(bsl_vec)
         av = None if A_vec is None else _c(A_vec)
-        self.h = lib().or_bsl_create(BSL_METHOD[method], bsl_init, A_ocean_pd, n, _p(bv), _p(av), year_bp)
+        if series is None:
+            self.h = lib().or_bsl_create(BSL_METHOD[method], bsl_init, A_ocean_pd, n, _p(bv), _p(av), year_bp)
+        else:
+            st, sv = _c(series[0]), _c(series[1])
+            self.h = lib().or_bsl_create_series(BSL_METHOD[method], bsl_init, A_ocean_pd, n, _p(bv), _p(av),
+                                                len(st), _p(st), _p(sv), year_bp)
         if not self.h:
             raise RuntimeError(lib().or_last_error().decode())
 

@@ -89,3 +89,17 @@ def test_server_fails_loudly_without_gpu():
     h = ctypes.c_void_p()
     rc = load().fi_server_create(1e9, 8, ctypes.byref(h))
     assert rc == -3 and b"no HIP device" in load().fi_last_error()
+
+
+def test_series_file_reader(tmp_path):
+    """read_series (barysealevel.f90:254-315) through the library's host-only
+    entry point: header line skipped, comment and blank lines skipped, comma or
+    blank separators."""
+    from fastisostasy_amd import isostasy as G
+    from fastisostasy_amd._lib import FiError
+    f = tmp_path / "sl.dat"
+    f.write_text("# time [yr]  bsl [m]\n-20000  -120.5\n# comment\n\n-10000, -60.25\n  0 0\n")
+    t, v = G.read_series(f)
+    assert t.tolist() == [-20000.0, -10000.0, 0.0] and v.tolist() == [-120.5, -60.25, 0.0]
+    with pytest.raises(FiError):
+        G.read_series(tmp_path / "missing.dat")

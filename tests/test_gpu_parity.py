@@ -17,7 +17,7 @@ OUT = ["z_bed", "z_ss", "dz_ss", "w", "we", "dwdt", "canom_full", "Hice"]
 MASKS = ["maskocean", "maskgrounded", "maskcontinent"]
 
 
-def run_pair(wl, times, precision="fp64", bsl_method="const", H_of_t=None):
+def run_pair(wl, times, precision="fp64", bsl_method="const", H_of_t=None, bsl_kw=None):
     """Run the workload through oracle and GPU on the same inputs; yield after each update."""
     p = dict(wl["params"])
     op = O.make_params(**p)
@@ -25,8 +25,10 @@ def run_pair(wl, times, precision="fp64", bsl_method="const", H_of_t=None):
     kw = dict(he_field=wl.get("he_field"), eta_field=wl.get("eta_field"))
     o = O.OracleIsos(op, wl["nx"], wl["nx"], wl["dx"], **kw)
     g = G.IsosClass(gp, wl["nx"], wl["nx"], wl["dx"], **kw)
-    ob = O.OracleBSL(bsl_method, 0.0, year_bp=times[0])
-    gb = G.bsl_init(bsl_method, 0.0, year_bp=times[0])
+    bkw = {"bsl_init": 0.0}
+    bkw.update(bsl_kw or {})
+    ob = O.OracleBSL(bsl_method, year_bp=times[0], **bkw)
+    gb = G.bsl_init(bsl_method, year_bp=times[0], **bkw)
     H = (lambda t: W.step_on_load(wl["H_ice"], t)) if H_of_t is None else H_of_t
     o.init_state(wl["z_bed"], H(times[0]), times[0], ob)
     g.init_state(wl["z_bed"], H(times[0]), times[0], gb)
@@ -145,6 +147,25 @@ def small_sealevel():
     return wl
 
 
+SERIES = (np.array([0.0, 100.0, 250.0]), np.array([0.0, -5.0, -2.0]))
+
+
+@pytest.mark.parametrize("bsl_method", ["file", "mixed"])
+def test_sealevel_series_bsl_fp64(bsl_method):
+    """BSL from a sea-level series (file), and the series minus each domain's
+    cumulative contribution (mixed), against the oracle: bsl bit-for-bit
+    after every update, fields as in the fastiso case."""
+    wl = small_sealevel()
+    Hmax = wl["H_max"]
+    times = [0.0] + [10.0 * k for k in range(1, 21)]
+    H_of_t = lambda t: Hmax * min(1.0, t / 150.0)
+    for o, g in run_pair(wl, times, bsl_method=bsl_method, H_of_t=H_of_t, bsl_kw=dict(bsl_init=1.0, series=SERIES)):
+        assert abs(o.scalar("bsl") - g.scalar("bsl")) <= 1e-12 * max(1.0, abs(o.scalar("bsl"))), \
+            (o.scalar("bsl"), g.scalar("bsl"))
+    assert o.scalar("bsl") != 0.0
+    compare(o, g, wtol=1e-6)
+
+
 def test_sealevel_masks_bsl_fp64():
     wl = small_sealevel()
     Hmax = wl["H_max"]
@@ -206,11 +227,13 @@ def test_fused_stencil_matches_separate_kernels(case, prec, threads, monkeypatch
     assert np.abs(a["w"]).max() > 0
 
 
-def test_batch_shared_bsl_matches_consecutive_updates():
+@pytest.mark.parametrize("method", ["fastiso", "mixed"])
+def test_batch_shared_bsl_matches_consecutive_updates(method):
     """Domains sharing one barystatic sea-level object: a batch update applies
     their volume changes to the shared bsl in member order, exactly as
     consecutive isos_update calls do (the last two members share one object,
     the first has its own)."""
+    kw = dict(series=SERIES) if method == "mixed" else {}
     wl = small_sealevel()
     Hmax = wl["H_max"]
     scales = (1.0, 0.6, 0.8)
@@ -219,8 +242,8 @@ def test_batch_shared_bsl_matches_consecutive_updates():
     for _ in scales:
         singles.append(G.IsosClass(p, wl["nx"], wl["nx"], wl["dx"]))
         gs.append(G.IsosClass(p, wl["nx"], wl["nx"], wl["dx"]))
-    sb = [G.bsl_init("fastiso", 0.0), G.bsl_init("fastiso", 0.0)]
-    bb = [G.bsl_init("fastiso", 0.0), G.bsl_init("fastiso", 0.0)]
+    sb = [G.bsl_init(method, 0.0, **kw), G.bsl_init(method, 0.0, **kw)]
+    bb = [G.bsl_init(method, 0.0, **kw), G.bsl_init(method, 0.0, **kw)]
     s_of = [sb[0], sb[1], sb[1]]
     b_of = [bb[0], bb[1], bb[1]]
     for a, b, sa, ba in zip(singles, gs, s_of, b_of):

@@ -386,3 +386,46 @@ def test_llra_relaxation_closed_form():
     assert np.ptp(frac) < 1e-12
     tau, T = p["tau"], 3000.0
     assert (1 - 10.0 / tau) ** (T / 10.0) <= frac[0] <= np.exp(-T / tau) * (1 + 1e-12)
+
+
+# ---------------------------------------------------------------- BSL series methods
+SERIES = (np.array([0.0, 1000.0, 3000.0]), np.array([0.0, -10.0, -50.0]))
+
+
+def test_bsl_file_interpolates_series():
+    """bsl_init_file / bsl_update_file (barysealevel.f90:108-153, 230-238):
+    interp_linear of the series, clamped outside it; bsl_init = the series at
+    the initial time."""
+    b = O.OracleBSL("file", bsl_init=7.0, series=SERIES, year_bp=500.0)
+    assert b.bsl_now == -5.0
+    for t, want in ((2000.0, -30.0), (-100.0, 0.0), (5000.0, -50.0), (1000.0, -10.0), (250.0, -2.5)):
+        b.update(t)
+        assert b.bsl_now == want, (t, b.bsl_now)
+
+
+def test_bsl_mixed_subtracts_cumulative_domain_contribution():
+    """mixed (fastisostasy.f90:771-775): each isos_update adds deltaV / A_ocean
+    to the domain's dbsl_total and sets bsl = series(t) - dbsl_total.  The
+    initial bsl_now is the parameter bsl_init, which bsl_init_fastiso re-reads
+    after bsl_init_file (barysealevel.f90:92-94, 159)."""
+    wl = W.sealevel_cycle(nx=32, dx=50e3, pad=400e3)
+    p = dict(wl["params"], include_elastic=0, heterogeneous_ssh=0)
+    A_pd = 3.625e14
+    b = O.OracleBSL("mixed", bsl_init=2.0, A_ocean_pd=A_pd, series=SERIES, year_bp=0.0)
+    assert b.bsl_now == 2.0 and b.A_ocean_now == A_pd
+    o = O.OracleIsos(O.make_params(**p), wl["nx"], wl["nx"], wl["dx"])
+    Hmax = wl["H_max"]
+    o.init_state(wl["z_bed"], np.zeros_like(Hmax), 0.0, b)
+    total = o.scalar("deltaV") / A_pd
+    assert b.bsl_now == 2.0 - total
+    moved = 0
+    for k in range(1, 8):
+        t = 100.0 * k
+        b.update(t)
+        o.update(Hmax * min(1.0, t / 400.0), t, b)
+        total = total + o.scalar("deltaV") / A_pd
+        file_value = float(np.interp(t, *SERIES))
+        assert b.bsl_now == file_value - total and o.scalar("bsl") == b.bsl_now
+        moved += o.scalar("deltaV") != 0.0
+    assert moved >= 3
+
