@@ -26,7 +26,8 @@ SCALAR_ID = {n: i for i, n in enumerate(SCALARS)}
 # every entry point of include/fastiso.h
 EXPORTS = ["fi_last_error", "fi_version", "fi_padded_size", "fi_isos_create", "fi_isos_init_ref",
            "fi_isos_init_state", "fi_isos_update", "fi_isos_get", "fi_isos_get_mask", "fi_isos_get_full",
-           "fi_isos_get_scalar", "fi_isos_destroy", "fi_bsl_create", "fi_bsl_create_series",
+           "fi_isos_get_scalar", "fi_isos_restart_write", "fi_isos_restart_read", "fi_bsl_restart_write",
+           "fi_bsl_restart_read", "fi_isos_destroy", "fi_bsl_create", "fi_bsl_create_series",
            "fi_bsl_read_series", "fi_bsl_update", "fi_bsl_get",
            "fi_bsl_destroy", "fi_batch_create", "fi_batch_update", "fi_batch_destroy", "fi_sync",
            "fi_prof_select", "fi_prof_collect", "fi_prof_fused", "fi_bench_copy",
@@ -84,6 +85,10 @@ def load():
     L.fi_isos_get_full.argtypes = [vp, i, dp]
     L.fi_isos_get_scalar.argtypes = [vp, i, dp]
     L.fi_isos_destroy.argtypes = [vp]
+    L.fi_isos_restart_write.argtypes = [vp, ctypes.c_char_p, d]
+    L.fi_isos_restart_read.argtypes = [vp, ctypes.c_char_p, d]
+    L.fi_bsl_restart_write.argtypes = [vp, ctypes.c_char_p, d]
+    L.fi_bsl_restart_read.argtypes = [ctypes.c_char_p, dp, dp]
     L.fi_bsl_create.argtypes = [i, d, d, i, dp, dp, d, ctypes.POINTER(vp)]
     L.fi_bsl_create_series.argtypes = [i, d, d, i, dp, dp, i, dp, dp, d, ctypes.POINTER(vp)]
     L.fi_bsl_read_series.argtypes = [ctypes.c_char_p, i, dp, dp, ctypes.POINTER(i)]

@@ -248,6 +248,7 @@ struct fi_isos {
     double dx = 0;
     bool fp64 = true, uniform = true;
     bool ref_was_set = false, state_init = false;
+    bool use_restart = false;    // a restart was read: the next init_state takes the restart branch
     double time_diag = 0, time_prog = 0;
     double last_t = 0, last_dt = 0;  // ode%t and ode%dt after the last update (host copy; 0 = unknown)
     char* arena = nullptr;
@@ -1046,11 +1047,22 @@ int fi_isos_init_state(fi_isos* h, const double* z_bed, const double* H_ice, dou
     rc = read_ctl(M, c);
     if (rc) return rc;
     c.bsl = b->h.bsl_now;
-    // out2in of z_bed and Hice into the zero-initialised state (:618-619)
-    rc = upload_embedded(h, M.z_bed, z_bed, 1e6);
-    if (rc) return rc;
-    rc = upload_embedded(h, M.Hice, H_ice, 0.0);
-    if (rc) return rc;
+    if (h->use_restart) {
+        // restart branch (:609-615): ref and now were read from the file;
+        // extendice2isostasy (isos_utils.f90:407-418) zeroes Hice, then embeds
+        // z_bed and H_ice into the ice-grid window only
+        h->ref_was_set = true;
+        rc = upload_embedded(h, M.Hice, H_ice, 0.0);
+        if (rc) return rc;
+        HIPCHK(hipMemcpy2D(M.z_bed + (size_t)h->jcrop1 * h->n + h->icrop1, (size_t)h->n * 8, z_bed,
+                           (size_t)h->nx_ice * 8, (size_t)h->nx_ice * 8, h->ny_ice, hipMemcpyHostToDevice));
+    } else {
+        // out2in of z_bed and Hice into the zero-initialised state (:618-619)
+        rc = upload_embedded(h, M.z_bed, z_bed, 1e6);
+        if (rc) return rc;
+        rc = upload_embedded(h, M.Hice, H_ice, 0.0);
+        if (rc) return rc;
+    }
     if (!h->ref_was_set) {
         // isos_init_ref(z_bed, H_ice, now%bsl, now%dz_ss) (:621-624)
         rc = upload_embedded(h, M.z_bed_ref, z_bed, 1e6);
@@ -1083,6 +1095,7 @@ int fi_isos_init_state(fi_isos* h, const double* z_bed, const double* H_ice, dou
     rc = write_ctl(M, c);
     if (rc) return rc;
     h->state_init = true;
+    h->use_restart = false;
     return 0;
 }
 
@@ -1126,6 +1139,172 @@ static const double* field_ptr(fi_isos* h, int f, bool& is_mask, int& bit) {
     case FI_W_EQUILIBRIUM: return M.weq;
     default: return nullptr;
     }
+}
+
+// ------------------------------------------------------------------ restart files
+// The variables, units and grid of the reference's NetCDF restarts
+// (isos_restart_write / _read, isostasy_io.f90:56-194; bsl_restart_write,
+// barysealevel.f90:426-450) in a binary container:
+//   "FIRESTRT", int32 version 1, int32 nrec, double time,
+//   nrec x { char name[32], char units[16], int32 nx, int32 ny, double v[nx*ny] }
+// little-endian, fields in Fortran (nx, ny) order.
+namespace {
+
+constexpr char kRestartMagic[9] = "FIRESTRT";
+
+struct RestartRec {
+    std::string name, units;
+    int nx, ny;
+    std::vector<double> v;
+};
+
+int restart_write_file(const char* path, double time, const std::vector<RestartRec>& recs) {
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return fail(FI_E_ARG, "restart: cannot create %s", path);
+    const int32_t head[2] = {1, (int32_t)recs.size()};
+    std::fwrite(kRestartMagic, 1, 8, f);
+    std::fwrite(head, 4, 2, f);
+    std::fwrite(&time, 8, 1, f);
+    for (const RestartRec& r : recs) {
+        char name[32] = {}, units[16] = {};
+        std::strncpy(name, r.name.c_str(), sizeof name - 1);
+        std::strncpy(units, r.units.c_str(), sizeof units - 1);
+        const int32_t dims[2] = {r.nx, r.ny};
+        std::fwrite(name, 1, 32, f);
+        std::fwrite(units, 1, 16, f);
+        std::fwrite(dims, 4, 2, f);
+        std::fwrite(r.v.data(), 8, r.v.size(), f);
+    }
+    const bool bad = std::ferror(f) != 0;
+    std::fclose(f);
+    if (bad) return fail(FI_E_ARG, "restart: write to %s failed", path);
+    return 0;
+}
+
+int restart_read_file(const char* path, std::vector<RestartRec>& recs, double* time) {
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return fail(FI_E_ARG, "restart: cannot open %s", path);
+    char magic[8];
+    int32_t head[2];
+    double t;
+    if (std::fread(magic, 1, 8, f) != 8 || std::memcmp(magic, kRestartMagic, 8) != 0 || std::fread(head, 4, 2, f) != 2 ||
+        std::fread(&t, 8, 1, f) != 1 || head[0] != 1 || head[1] < 0) {
+        std::fclose(f);
+        return fail(FI_E_ARG, "restart: %s is not a restart file", path);
+    }
+    recs.clear();
+    for (int k = 0; k < head[1]; ++k) {
+        char name[33] = {}, units[17] = {};
+        int32_t dims[2];
+        if (std::fread(name, 1, 32, f) != 32 || std::fread(units, 1, 16, f) != 16 || std::fread(dims, 4, 2, f) != 2 ||
+            dims[0] < 0 || dims[1] < 0) {
+            std::fclose(f);
+            return fail(FI_E_ARG, "restart: %s is truncated", path);
+        }
+        RestartRec r{name, units, dims[0], dims[1], std::vector<double>((size_t)dims[0] * dims[1])};
+        if (std::fread(r.v.data(), 8, r.v.size(), f) != r.v.size()) {
+            std::fclose(f);
+            return fail(FI_E_ARG, "restart: %s is truncated", path);
+        }
+        recs.push_back(std::move(r));
+    }
+    std::fclose(f);
+    if (time) *time = t;
+    return 0;
+}
+
+int download(const double* d, size_t count, std::vector<double>& out) {
+    out.resize(count);
+    HIPCHK(hipMemcpy(out.data(), d, count * 8, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+}  // namespace
+
+int fi_isos_restart_write(fi_isos* h, const char* path, double time) {
+    if (!h || !path) return fail(FI_E_ARG, "NULL argument");
+    HIPCHK(hipStreamSynchronize(g_stream));
+    const Member& M = h->hm;
+    const int n = h->n;
+    const size_t nn = (size_t)n * n;
+    struct { const char* name; const char* units; const double* d; } f64[] = {
+        {"z_bed", "m", M.z_bed}, {"z_ss", "m", M.z_ss}, {"dz_ss", "m", M.dz_ss}, {"w", "m", M.W}, {"we", "m", M.we},
+        {"dz_ss_ref", "m", M.dz_ss_ref}, {"H_ice_ref", "m", M.Hice_ref}, {"z_bed_ref", "m", M.z_bed_ref},
+        {"w_ref", "m", M.w_ref}, {"we_ref", "m", M.we_ref}};
+    std::vector<RestartRec> recs;
+    for (const auto& v : f64) {
+        RestartRec r{v.name, v.units, n, n, {}};
+        int rc = download(v.d, nn, r.v);
+        if (rc) return rc;
+        recs.push_back(std::move(r));
+    }
+    RestartRec he{"He_lith", "km", n, n, {}};
+    int rc = download(h->He, nn, he.v);
+    if (rc) return rc;
+    for (double& x : he.v) x = x * (double)1e-3f;   // He_lith * 1e-3 (single-precision literal)
+    recs.push_back(std::move(he));
+    std::vector<uint8_t> mk(nn);
+    HIPCHK(hipMemcpy(mk.data(), M.mask, nn, hipMemcpyDeviceToHost));
+    RestartRec ma{"mask_active", "1", n, n, std::vector<double>(nn)};
+    for (size_t c = 0; c < nn; ++c) ma.v[c] = (mk[c] & 8) ? 1.0 : 0.0;
+    recs.push_back(std::move(ma));
+    if (h->par.method == 3) {
+        RestartRec lg{"log10_eta_eff", "Pa s", n, n, {}};
+        rc = download(h->eta_eff, nn, lg.v);
+        if (rc) return rc;
+        for (double& x : lg.v) x = std::log10(x);
+        recs.push_back(std::move(lg));
+    }
+    return restart_write_file(path, time, recs);
+}
+
+int fi_isos_restart_read(fi_isos* h, const char* path, double time) {
+    (void)time;
+    if (!h || !path) return fail(FI_E_ARG, "NULL argument");
+    std::vector<RestartRec> recs;
+    int rc = restart_read_file(path, recs, nullptr);
+    if (rc) return rc;
+    const Member& M = h->hm;
+    struct { const char* name; double* d; } want[] = {
+        {"z_bed_ref", M.z_bed_ref}, {"H_ice_ref", M.Hice_ref}, {"dz_ss_ref", M.dz_ss_ref}, {"w_ref", M.w_ref},
+        {"we_ref", M.we_ref}, {"z_bed", M.z_bed}, {"z_ss", M.z_ss}, {"dz_ss", M.dz_ss}, {"w", M.W}, {"we", M.we}};
+    HIPCHK(hipStreamSynchronize(g_stream));
+    for (const auto& w : want) {
+        const RestartRec* r = nullptr;
+        for (const RestartRec& q : recs)
+            if (q.name == w.name) r = &q;
+        if (!r) return fail(FI_E_ARG, "restart: %s has no variable %s", path, w.name);
+        if (r->nx != h->n || r->ny != h->n)
+            return fail(FI_E_ARG, "restart: %s is %d x %d, the domain is %d x %d", w.name, r->nx, r->ny, h->n, h->n);
+        HIPCHK(hipMemcpy(w.d, r->v.data(), r->v.size() * 8, hipMemcpyHostToDevice));
+    }
+    h->use_restart = true;
+    return 0;
+}
+
+int fi_bsl_restart_write(fi_bsl* b, const char* path, double time) {
+    if (!b || !path) return fail(FI_E_ARG, "NULL argument");
+    int rc = fi_bsl_get(b, nullptr, nullptr);
+    if (rc) return rc;
+    std::vector<RestartRec> recs;
+    recs.push_back({"bsl", "m", 1, 1, {b->h.bsl_now}});
+    recs.push_back({"A_ocean", "m^2", 1, 1, {b->h.A_ocean_now}});
+    return restart_write_file(path, time, recs);
+}
+
+int fi_bsl_restart_read(const char* path, double* bsl, double* A_ocean) {
+    if (!path) return fail(FI_E_ARG, "NULL argument");
+    std::vector<RestartRec> recs;
+    int rc = restart_read_file(path, recs, nullptr);
+    if (rc) return rc;
+    bool got_bsl = false;
+    for (const RestartRec& r : recs) {
+        if (r.v.size() != 1) continue;
+        if (r.name == "bsl") { if (bsl) *bsl = r.v[0]; got_bsl = true; }
+        if (r.name == "A_ocean" && A_ocean) *A_ocean = r.v[0];
+    }
+    if (!got_bsl) return fail(FI_E_ARG, "restart: %s has no variable bsl", path);
+    return 0;
 }
 
 int fi_isos_get_full(fi_isos* h, int field, double* dst) {

@@ -54,6 +54,8 @@ module fastisostasy
         real(wp) :: rho_ice, rho_uppermantle, rho_seawater, rho_water, g, nu, E
         character(len=56) :: dt_method
         integer :: precision = 0
+        character(len=256) :: restart = "None"
+        logical :: use_restart = .false.
     end type
 
     type, public :: isos_domain_class
@@ -92,7 +94,8 @@ module fastisostasy
     end type
 
     public :: isos_init, isos_init_ref, isos_init_state, isos_update, isos_end
-    public :: bsl_init, bsl_update, bsl_end
+    public :: isos_restart_write, isos_restart_read
+    public :: bsl_init, bsl_update, bsl_end, bsl_restart_write
     public :: fi_select_precision
 
     integer, parameter :: FI_Z_BED = 0, FI_Z_SS = 1, FI_DZ_SS = 2, FI_W = 3, FI_WE = 4, FI_DWDT = 5
@@ -194,6 +197,33 @@ module fastisostasy
             integer(c_int), intent(out) :: n
             integer(c_int) :: rc
         end function
+        function c_fi_isos_restart_write(h, path, time) bind(C, name="fi_isos_restart_write") result(rc)
+            import :: c_int, c_ptr, c_double, c_char
+            type(c_ptr), value :: h
+            character(kind=c_char), intent(in) :: path(*)
+            real(c_double), value :: time
+            integer(c_int) :: rc
+        end function
+        function c_fi_isos_restart_read(h, path, time) bind(C, name="fi_isos_restart_read") result(rc)
+            import :: c_int, c_ptr, c_double, c_char
+            type(c_ptr), value :: h
+            character(kind=c_char), intent(in) :: path(*)
+            real(c_double), value :: time
+            integer(c_int) :: rc
+        end function
+        function c_fi_bsl_restart_write(b, path, time) bind(C, name="fi_bsl_restart_write") result(rc)
+            import :: c_int, c_ptr, c_double, c_char
+            type(c_ptr), value :: b
+            character(kind=c_char), intent(in) :: path(*)
+            real(c_double), value :: time
+            integer(c_int) :: rc
+        end function
+        function c_fi_bsl_restart_read(path, b, a) bind(C, name="fi_bsl_restart_read") result(rc)
+            import :: c_int, c_double, c_char
+            character(kind=c_char), intent(in) :: path(*)
+            real(c_double), intent(out) :: b, a
+            integer(c_int) :: rc
+        end function
         function c_fi_bsl_update(b, year_bp) bind(C, name="fi_bsl_update") result(rc)
             import :: c_int, c_ptr, c_double
             type(c_ptr), value :: b
@@ -260,9 +290,10 @@ contains
     end function pick
 
     ! isos_par_load (src/fastisostasy.f90:925-1015) with NAMELIST input
-    subroutine read_isostasy_nml(filename, group, p)
+    subroutine read_isostasy_nml(filename, group, p, restart_path)
         character(len=*), intent(IN) :: filename, group
         type(fi_params), intent(OUT) :: p
+        character(len=256), intent(OUT) :: restart_path
         logical :: heterogeneous_ssh, interactive_sealevel, include_elastic
         logical :: correct_compression, correct_distortion
         integer :: method, nl, u, ios
@@ -296,8 +327,7 @@ contains
         read(u, nml=isostasy, iostat=ios)
         close(u)
         if (ios /= 0) error stop "fastisostasy: cannot read the &isostasy namelist"
-        if (trim(restart) /= "None" .and. trim(restart) /= "none" .and. trim(restart) /= "no") &
-            error stop "fastisostasy: restart files are not supported"
+        restart_path = restart
         if (trim(mask_file) /= "None" .and. trim(mask_file) /= "none" .and. trim(mask_file) /= "no") &
             error stop "fastisostasy: mask files are not supported"
         if (nl < 1 .or. nl > FI_MAXL) error stop "fastisostasy: nl out of range"
@@ -351,7 +381,10 @@ contains
         real(c_double) :: v
         integer :: n
 
-        call read_isostasy_nml(filename, group, p)
+        call read_isostasy_nml(filename, group, p, isos%par%restart)
+        ! par%use_restart (fastisostasy.f90:997-1004)
+        isos%par%use_restart = .not. (trim(isos%par%restart) == "None" .or. trim(isos%par%restart) == "none" &
+            .or. trim(isos%par%restart) == "no")
         prho = c_null_ptr
         pK = c_null_ptr
         if (present(rho_ice)) then
@@ -474,11 +507,41 @@ contains
         real(wp), intent(IN) :: z_bed(:, :), H_ice(:, :)
         real(wp), intent(IN) :: time
         type(bsl_class), intent(INOUT) :: bsl
+        if (isos%par%use_restart) call isos_restart_read(isos, trim(isos%par%restart), time)
         call check(c_fi_isos_init_state(isos%h, real(z_bed, c_double), real(H_ice, c_double), &
             real(time, c_double), bsl%h), "isos_init_state")
         call refresh(isos)
         call bsl_refresh(bsl)
     end subroutine isos_init_state
+
+    ! isos_restart_write / isos_restart_read (src/isostasy_io.f90:56-194): binary
+    ! container of the same variables (the library's restart format)
+    subroutine isos_restart_write(isos, filename, time, init)
+        type(isos_class), intent(IN) :: isos
+        character(len=*), intent(IN) :: filename
+        real(wp), intent(IN) :: time
+        logical, optional, intent(IN) :: init
+        call check(c_fi_isos_restart_write(isos%h, trim(filename)//c_null_char, real(time, c_double)), &
+            "isos_restart_write")
+    end subroutine isos_restart_write
+
+    subroutine isos_restart_read(isos, filename, time)
+        type(isos_class), intent(INOUT) :: isos
+        character(len=*), intent(IN) :: filename
+        real(wp), intent(IN) :: time
+        call check(c_fi_isos_restart_read(isos%h, trim(filename)//c_null_char, real(time, c_double)), &
+            "isos_restart_read")
+    end subroutine isos_restart_read
+
+    ! bsl_restart_write (src/barysealevel.f90:426-450)
+    subroutine bsl_restart_write(bsl, filename, time, init)
+        type(bsl_class), intent(IN) :: bsl
+        character(len=*), intent(IN) :: filename
+        real(wp), intent(IN) :: time
+        logical, optional, intent(IN) :: init
+        call check(c_fi_bsl_restart_write(bsl%h, trim(filename)//c_null_char, real(time, c_double)), &
+            "bsl_restart_write")
+    end subroutine bsl_restart_write
 
     ! isos_update (src/fastisostasy.f90:731-872); dwdt_corr is unused, as in the reference
     subroutine isos_update(isos, H_ice, time, bsl, dwdt_corr)
@@ -510,6 +573,7 @@ contains
         character(len=256) :: sl_path, sl_name, A_ocean_path, restart
         real(dp) :: bsl_init, A_ocean_pd
         integer :: u, ios
+        real(c_double) :: rb, ra
         namelist /barysealevel/ method, bsl_init, sl_path, sl_name, A_ocean_pd, A_ocean_path, restart
         method = "const"; bsl_init = 0.0_dp; A_ocean_pd = 3.625e14_dp
         sl_path = "None"; sl_name = "None"; A_ocean_path = "None"; restart = "None"
@@ -520,6 +584,12 @@ contains
         if (ios /= 0) error stop "fastisostasy: cannot read the &barysealevel namelist"
         if (trim(A_ocean_path) /= "None" .and. trim(A_ocean_path) /= "none" .and. trim(A_ocean_path) /= "no") &
             error stop "fastisostasy: A_ocean_path tables are not supported by the namelist shim"
+        ! const with a restart file takes bsl_init from it (barysealevel.f90:57-73)
+        if (trim(method) == "const" .and. .not. (trim(restart) == "None" .or. trim(restart) == "none" .or. &
+                trim(restart) == "no")) then
+            call check(c_fi_bsl_restart_read(trim(restart)//c_null_char, rb, ra), "bsl_init")
+            bsl_init = rb
+        end if
         bsl0 = bsl_init
         A_pd = A_ocean_pd
         series_path = sl_path

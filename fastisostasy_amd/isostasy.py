@@ -103,6 +103,10 @@ class BslClass:
         check(load().fi_bsl_get(self.h, None, ctypes.byref(v)))
         return v.value
 
+    def restart_write(self, path, time):
+        """bsl_restart_write (src/barysealevel.f90:426-450)."""
+        check(load().fi_bsl_restart_write(self.h, str(path).encode(), float(time)))
+
     def close(self):
         if getattr(self, "h", None):
             load().fi_bsl_destroy(self.h)
@@ -150,6 +154,15 @@ class IsosClass:
         a = np.empty((self.ny_ice, self.nx_ice))
         check(load().fi_isos_get(self.h, _lib.FIELD_ID[name], _dp(a)))
         return a
+
+    def restart_write(self, path, time):
+        """isos_restart_write (src/isostasy_io.f90:56-139)."""
+        check(load().fi_isos_restart_write(self.h, str(path).encode(), float(time)))
+
+    def restart_read(self, path, time):
+        """isos_restart_read (src/isostasy_io.f90:141-194); the next init_state
+        takes the restart branch."""
+        check(load().fi_isos_restart_read(self.h, str(path).encode(), float(time)))
 
     def full(self, name):
         """Padded-grid (N x N) state field."""
@@ -201,6 +214,14 @@ class Batch:
 def bsl_init(method="const", bsl_init=0.0, A_ocean_pd=3.625e14, year_bp=0.0, bsl_vec=None, A_vec=None,
              series=None):
     return BslClass(method, bsl_init, A_ocean_pd, bsl_vec, A_vec, year_bp, series)
+
+
+def bsl_restart_read(path):
+    """(bsl, A_ocean) of a BSL restart file; bsl_init of the const method takes
+    bsl from it (src/barysealevel.f90:72-73)."""
+    b, a = ctypes.c_double(), ctypes.c_double()
+    check(load().fi_bsl_restart_read(str(path).encode(), ctypes.byref(b), ctypes.byref(a)))
+    return b.value, a.value
 
 
 def read_series(path, max_n=10000):

@@ -124,6 +124,20 @@ int fi_isos_get_mask(fi_isos* h, int field, int8_t* dst);
 /* padded-grid state (N x N) */
 int fi_isos_get_full(fi_isos* h, int field, double* dst);
 int fi_isos_get_scalar(fi_isos* h, int which, double* out);
+/* isos_restart_write / isos_restart_read (src/isostasy_io.f90:56-194): the
+ * reference's restart variables and units (now and ref states, He_lith in km,
+ * mask_active, log10_eta_eff) on the padded grid, in a binary container
+ * ("FIRESTRT", int32 version 1, int32 nrec, double time, then per record
+ * char name[32], char units[16], int32 nx, int32 ny, double values, x fastest).
+ * After a read, the next fi_isos_init_state takes the reference's restart
+ * branch (src/fastisostasy.f90:609-615). */
+int fi_isos_restart_write(fi_isos* h, const char* path, double time);
+int fi_isos_restart_read(fi_isos* h, const char* path, double time);
+/* bsl_restart_write (src/barysealevel.f90:426-450): bsl (m), A_ocean (m^2).
+ * fi_bsl_restart_read (host only) gives the value bsl_init takes from a
+ * restart file (:72-73). */
+int fi_bsl_restart_write(fi_bsl* b, const char* path, double time);
+int fi_bsl_restart_read(const char* path, double* bsl, double* A_ocean);
 /* isos_end (src/fastisostasy.f90:911-923) */
 int fi_isos_destroy(fi_isos* h);
 

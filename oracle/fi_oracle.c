@@ -26,6 +26,7 @@
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
+#include <stdint.h>
 #include <string.h>
 
 typedef double complex cplx;
@@ -591,7 +592,7 @@ struct or_isos {
     int n, nx_ice, ny_ice, icrop1, icrop2, jcrop1, jcrop2;  /* 1-based crop indices */
     double dx, dy, sec_per_year, compressibility_correction, Vden_factor;
     double time_diagnostics, time_prognostics;
-    int ref_was_set;
+    int ref_was_set, use_restart;
     double *K, *dxm, *A, *kappa, *R, *maskactive, *He_lith, *D_lith, *eta_eff;
     cplx *FGE, *FGN, *FGV;
     double tau, L_w;            /* domain%tau (uniform), par%L_w */
@@ -1056,12 +1057,21 @@ int or_isos_update(or_isos *s, const double *H_ice, double time, or_bsl *b) {
 }
 
 /* isos_init_state, fastisostasy.f90:596-660 */
+/* isos_init_state, fastisostasy.f90:596-660 (restart branch :609-615) */
 int or_isos_init_state(or_isos *s, const double *z_bed, const double *H_ice, double time, or_bsl *b) {
     const size_t nn = (size_t)s->n * s->n;
     s->now.bsl = b->bsl_now;
-    out2in(s, s->now.z_bed, z_bed);
-    out2in(s, s->now.Hice, H_ice);
-    if (!s->ref_was_set) or_isos_init_ref(s, z_bed, H_ice, &s->now.bsl, s->now.dz_ss);
+    if (s->use_restart) {
+        /* the restart read set ref and now; extendice2isostasy, isos_utils.f90:407-418 */
+        s->ref_was_set = 1;
+        memset(s->now.Hice, 0, sizeof(double) * nn);
+        out2in(s, s->now.z_bed, z_bed);
+        out2in(s, s->now.Hice, H_ice);
+    } else {
+        out2in(s, s->now.z_bed, z_bed);
+        out2in(s, s->now.Hice, H_ice);
+        if (!s->ref_was_set) or_isos_init_ref(s, z_bed, H_ice, &s->now.bsl, s->now.dz_ss);
+    }
     memset(s->ref.z_ss, 0, sizeof(double) * nn);
     calc_z_ss(nn, s->ref.z_ss, s->ref.bsl, s->ref.z_ss, s->ref.dz_ss);
     calc_z_ss(nn, s->now.z_ss, s->now.bsl, s->now.z_ss, s->now.dz_ss);
@@ -1073,6 +1083,110 @@ int or_isos_init_state(or_isos *s, const double *z_bed, const double *H_ice, dou
     s->ode_t = time;
     memcpy(s->ode_x, s->now.w, sizeof(double) * nn);
     return rc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Restart files.  The variables, units and grid of the NetCDF restarts
+ * (isos_restart_write / _read, isostasy_io.f90:56-194; bsl_restart_write,
+ * barysealevel.f90:426-450) in a binary container:
+ *   "FIRESTRT", int32 version = 1, int32 nrec, double time,
+ *   nrec x { char name[32], char units[16], int32 nx, int32 ny, double v[nx*ny] }
+ * little-endian, fields in Fortran (nx, ny) order (x fastest). */
+#define RST_MAGIC "FIRESTRT"
+
+typedef struct { const char *name, *units; const double *v; int nx, ny; } rst_rec;
+
+static int rst_write(const char *path, double time, const rst_rec *r, int nrec) {
+    FILE *f = fopen(path, "wb");
+    if (!f) { set_err("restart: cannot create file"); return -1; }
+    const int32_t ver = 1, nr = nrec;
+    fwrite(RST_MAGIC, 1, 8, f); fwrite(&ver, 4, 1, f); fwrite(&nr, 4, 1, f); fwrite(&time, 8, 1, f);
+    for (int k = 0; k < nrec; k++) {
+        char name[32] = {0}, units[16] = {0};
+        strncpy(name, r[k].name, sizeof name - 1); strncpy(units, r[k].units, sizeof units - 1);
+        const int32_t nx = r[k].nx, ny = r[k].ny;
+        fwrite(name, 1, 32, f); fwrite(units, 1, 16, f); fwrite(&nx, 4, 1, f); fwrite(&ny, 4, 1, f);
+        fwrite(r[k].v, 8, (size_t)nx * ny, f);
+    }
+    const int bad = ferror(f);
+    fclose(f);
+    if (bad) { set_err("restart: write failed"); return -1; }
+    return 0;
+}
+
+/* the record `name` of shape nx x ny into out (time of the file into *time) */
+static int rst_read(const char *path, const char *name, int nx, int ny, double *out, double *time) {
+    FILE *f = fopen(path, "rb");
+    if (!f) { set_err("restart: cannot open file"); return -1; }
+    char magic[8]; int32_t ver, nr; double t;
+    if (fread(magic, 1, 8, f) != 8 || memcmp(magic, RST_MAGIC, 8) || fread(&ver, 4, 1, f) != 1 ||
+        fread(&nr, 4, 1, f) != 1 || fread(&t, 8, 1, f) != 1 || ver != 1) {
+        fclose(f); set_err("restart: not a restart file"); return -1;
+    }
+    if (time) *time = t;
+    for (int k = 0; k < nr; k++) {
+        char nm[32], un[16]; int32_t rx, ry;
+        if (fread(nm, 1, 32, f) != 32 || fread(un, 1, 16, f) != 16 || fread(&rx, 4, 1, f) != 1 ||
+            fread(&ry, 4, 1, f) != 1) break;
+        const size_t cnt = (size_t)rx * ry;
+        if (strncmp(nm, name, 32) == 0) {
+            if (rx != nx || ry != ny) { fclose(f); set_err("restart: grid size differs"); return -1; }
+            const size_t got = fread(out, 8, cnt, f);
+            fclose(f);
+            if (got != cnt) { set_err("restart: truncated file"); return -1; }
+            return 0;
+        }
+        if (fseek(f, (long)(cnt * 8), SEEK_CUR)) break;
+    }
+    fclose(f);
+    set_err("restart: variable missing");
+    return -1;
+}
+
+int or_isos_restart_write(const or_isos *s, const char *path, double time) {
+    const int n = s->n;
+    const size_t nn = (size_t)n * n;
+    double *he_km = (double *)malloc(sizeof(double) * nn), *lg = NULL;
+    for (size_t c = 0; c < nn; c++) he_km[c] = s->He_lith[c] * (double)1e-3f;
+    rst_rec r[13] = {
+        {"z_bed", "m", s->now.z_bed, n, n}, {"z_ss", "m", s->now.z_ss, n, n}, {"dz_ss", "m", s->now.dz_ss, n, n},
+        {"w", "m", s->now.w, n, n}, {"we", "m", s->now.we, n, n},
+        {"dz_ss_ref", "m", s->ref.dz_ss, n, n}, {"H_ice_ref", "m", s->ref.Hice, n, n},
+        {"z_bed_ref", "m", s->ref.z_bed, n, n}, {"w_ref", "m", s->ref.w, n, n}, {"we_ref", "m", s->ref.we, n, n},
+        {"He_lith", "km", he_km, n, n}, {"mask_active", "1", s->maskactive, n, n}, {NULL, NULL, NULL, 0, 0}};
+    int nrec = 12;
+    if (s->par.method == 3) {
+        lg = (double *)malloc(sizeof(double) * nn);
+        for (size_t c = 0; c < nn; c++) lg[c] = log10(s->eta_eff[c]);
+        r[12] = (rst_rec){"log10_eta_eff", "Pa s", lg, n, n};
+        nrec = 13;
+    }
+    const int rc = rst_write(path, time, r, nrec);
+    free(he_km); free(lg);
+    return rc;
+}
+
+int or_isos_restart_read(or_isos *s, const char *path, double time) {
+    (void)time;
+    const int n = s->n;
+    struct { const char *name; double *dst; } v[10] = {
+        {"z_bed_ref", s->ref.z_bed}, {"H_ice_ref", s->ref.Hice}, {"dz_ss_ref", s->ref.dz_ss},
+        {"w_ref", s->ref.w}, {"we_ref", s->ref.we}, {"z_bed", s->now.z_bed}, {"z_ss", s->now.z_ss},
+        {"dz_ss", s->now.dz_ss}, {"w", s->now.w}, {"we", s->now.we}};
+    for (int k = 0; k < 10; k++)
+        if (rst_read(path, v[k].name, n, n, v[k].dst, NULL)) return -1;
+    s->use_restart = 1;
+    return 0;
+}
+
+int or_bsl_restart_write(const or_bsl *b, const char *path, double time) {
+    rst_rec r[2] = {{"bsl", "m", &b->bsl_now, 1, 1}, {"A_ocean", "m^2", &b->A_ocean_now, 1, 1}};
+    return rst_write(path, time, r, 2);
+}
+
+int or_bsl_restart_read(const char *path, double *bsl, double *A_ocean) {
+    if (rst_read(path, "bsl", 1, 1, bsl, NULL)) return -1;
+    return rst_read(path, "A_ocean", 1, 1, A_ocean, NULL);
 }
 
 static const double *field_ptr(const or_isos *s, int f, int *is_mask, const unsigned char **mask) {

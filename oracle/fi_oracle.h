@@ -79,6 +79,13 @@ int  or_isos_update(or_isos *s, const double *H_ice, double time, or_bsl *b);
 int  or_isos_get(const or_isos *s, int field, double *dst);      /* ice grid */
 int  or_isos_get_full(const or_isos *s, int field, double *dst); /* padded grid */
 double or_isos_scalar(const or_isos *s, int which);
+/* isos_restart_write / isos_restart_read (isostasy_io.f90:56-194): binary
+ * container of the same variables; a read makes the next init_state take
+ * the restart branch (fastisostasy.f90:609-615) */
+int or_isos_restart_write(const or_isos *s, const char *path, double time);
+int or_isos_restart_read(or_isos *s, const char *path, double time);
+int or_bsl_restart_write(const or_bsl *b, const char *path, double time);
+int or_bsl_restart_read(const char *path, double *bsl, double *A_ocean);
 void or_isos_destroy(or_isos *s);
 const char *or_last_error(void);
 

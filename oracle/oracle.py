@@ -92,6 +92,10 @@ def lib():
         L.or_bsl_create_series.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                            dp, dp, ctypes.c_int, dp, dp, ctypes.c_double]
         L.or_bsl_update.argtypes = [vp, ctypes.c_double]
+        L.or_isos_restart_write.argtypes = [vp, ctypes.c_char_p, ctypes.c_double]
+        L.or_isos_restart_read.argtypes = [vp, ctypes.c_char_p, ctypes.c_double]
+        L.or_bsl_restart_write.argtypes = [vp, ctypes.c_char_p, ctypes.c_double]
+        L.or_bsl_restart_read.argtypes = [ctypes.c_char_p, dp, dp]
         L.or_bsl_now.argtypes = [vp]
         L.or_bsl_now.restype = ctypes.c_double
         L.or_bsl_A_ocean.argtypes = [vp]
@@ -256,6 +260,10 @@ class OracleBSL:
     def update(self, year_bp):
         lib().or_bsl_update(self.h, year_bp)
 
+    def restart_write(self, path, time):
+        if lib().or_bsl_restart_write(self.h, str(path).encode(), time):
+            raise RuntimeError(lib().or_last_error().decode())
+
     @property
     def bsl_now(self):
         return lib().or_bsl_now(self.h)
@@ -300,6 +308,12 @@ class OracleIsos:
     def update(self, H_ice, time, bsl):
         self._check(lib().or_isos_update(self.h, _p(_c(H_ice)), time, bsl.h))
 
+    def restart_write(self, path, time):
+        self._check(lib().or_isos_restart_write(self.h, str(path).encode(), time))
+
+    def restart_read(self, path, time):
+        self._check(lib().or_isos_restart_read(self.h, str(path).encode(), time))
+
     def get(self, name):
         out = np.empty((self.ny_ice, self.nx_ice))
         self._check(lib().or_isos_get(self.h, FIELD_ID[name], _p(out)))
@@ -317,3 +331,30 @@ class OracleIsos:
         if getattr(self, "h", None):
             lib().or_isos_destroy(self.h)
             self.h = None
+
+
+def bsl_restart_read(path):
+    """(bsl, A_ocean) of a BSL restart file."""
+    b, a = ctypes.c_double(), ctypes.c_double()
+    if lib().or_bsl_restart_read(str(path).encode(), ctypes.byref(b), ctypes.byref(a)):
+        raise RuntimeError(lib().or_last_error().decode())
+    return b.value, a.value
+
+
+def read_restart_file(path):
+    """All records of a restart container: {name: (units, array[ny, nx])}, time."""
+    raw = open(path, "rb").read()
+    assert raw[:8] == b"FIRESTRT"
+    ver, nrec = np.frombuffer(raw, dtype="<i4", count=2, offset=8)
+    time = float(np.frombuffer(raw, dtype="<f8", count=1, offset=16)[0])
+    assert ver == 1
+    out, off = {}, 24
+    for _ in range(nrec):
+        name = raw[off:off + 32].split(b"\0")[0].decode()
+        units = raw[off + 32:off + 48].split(b"\0")[0].decode()
+        nx, ny = np.frombuffer(raw, dtype="<i4", count=2, offset=off + 48)
+        off += 56
+        v = np.frombuffer(raw, dtype="<f8", count=int(nx) * int(ny), offset=off).reshape(int(ny), int(nx))
+        off += 8 * int(nx) * int(ny)
+        out[name] = (units, v.copy())
+    return out, time

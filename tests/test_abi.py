@@ -103,3 +103,15 @@ def test_series_file_reader(tmp_path):
     assert t.tolist() == [-20000.0, -10000.0, 0.0] and v.tolist() == [-120.5, -60.25, 0.0]
     with pytest.raises(FiError):
         G.read_series(tmp_path / "missing.dat")
+
+
+def test_bsl_restart_file_read_by_library(tmp_path):
+    """The library's host-only restart reader understands the oracle's file:
+    both implement the same container independently."""
+    b = O.OracleBSL("fastiso", -3.5, A_ocean_pd=3.6e14)
+    path = tmp_path / "bsl_restart.bin"
+    b.restart_write(path, 42.0)
+    assert G.bsl_restart_read(path) == (-3.5, 3.6e14)
+    from fastisostasy_amd._lib import FiError
+    with pytest.raises(FiError):
+        G.bsl_restart_read(tmp_path / "nothing.bin")

@@ -422,3 +422,63 @@ def test_relaxation_batch_matches_single(prec):
         for f in ("w", "z_bed", "dwdt"):
             assert np.array_equal(a.out(f), b.out(f)), f
         assert np.array_equal(a.full("w_equilibrium"), b.full("w_equilibrium"))
+
+
+# ---------------------------------------------------------------- restart files
+def test_restart_matches_oracle(tmp_path):
+    """isos_restart_write / isos_restart_read through the library against the
+    oracle's own implementation of the container: the files hold the same
+    records (names, units; values within the FP64 difference of the two runs),
+    the runs continued from them agree, and the library reads the oracle's
+    file to the same result."""
+    wl = small_sealevel()
+    Hmax = wl["H_max"]
+    H = lambda t: Hmax * min(1.0, t / 150.0)
+    times = [0.0] + [10.0 * k for k in range(1, 11)]
+    for o, g in run_pair(wl, times, bsl_method="fastiso", H_of_t=H):
+        pass
+    po, pg = tmp_path / "oracle.bin", tmp_path / "gpu.bin"
+    o.restart_write(po, 100.0)
+    g.restart_write(pg, 100.0)
+    ro, to = O.read_restart_file(po)
+    rg, tg = O.read_restart_file(pg)
+    assert to == tg == 100.0 and list(ro) == list(rg)
+    for k in ro:
+        assert ro[k][0] == rg[k][0], k
+        a, b = ro[k][1], rg[k][1]
+        assert np.abs(a - b).max() <= 1e-9 * max(1.0, np.abs(a).max()), (k, np.abs(a - b).max())
+    p = wl["params"]
+    bsl0 = o.scalar("bsl")
+    o2 = O.OracleIsos(O.make_params(**p), wl["nx"], wl["nx"], wl["dx"])
+    g2 = G.IsosClass(G.make_params(**p), wl["nx"], wl["nx"], wl["dx"])
+    g3 = G.IsosClass(G.make_params(**p), wl["nx"], wl["nx"], wl["dx"])
+    o2.restart_read(po, 100.0)
+    g2.restart_read(pg, 100.0)
+    g3.restart_read(po, 100.0)
+    ob = O.OracleBSL("fastiso", bsl0)
+    gb2, gb3 = G.bsl_init("fastiso", bsl0), G.bsl_init("fastiso", bsl0)
+    z_bed = o.get("z_bed")
+    o2.init_state(z_bed, H(100.0), 100.0, ob)
+    g2.init_state(z_bed, H(100.0), 100.0, gb2)
+    g3.init_state(z_bed, H(100.0), 100.0, gb3)
+    for k in range(11, 21):
+        t = 10.0 * k
+        ob.update(t)
+        G.bsl_update(gb2, t)
+        G.bsl_update(gb3, t)
+        o2.update(H(t), t, ob)
+        g2.update(H(t), t, gb2)
+        g3.update(H(t), t, gb3)
+    compare(o2, g2, wtol=1e-6)
+    compare(o2, g3, wtol=1e-6)
+    assert abs(o2.scalar("bsl") - g2.scalar("bsl")) < 1e-9
+
+
+def test_bsl_restart_file_read_by_oracle(tmp_path):
+    b = G.bsl_init("fastiso", -1.25, A_ocean_pd=3.5e14)
+    G.bsl_update(b, 10.0)
+    path = tmp_path / "bsl.bin"
+    b.restart_write(path, 10.0)
+    assert O.bsl_restart_read(path) == (b.bsl_now, b.A_ocean_now) == (-1.25, 3.5e14)
+    recs, t = O.read_restart_file(path)
+    assert t == 10.0 and {k: v[0] for k, v in recs.items()} == {"bsl": "m", "A_ocean": "m^2"}

@@ -429,3 +429,70 @@ def test_bsl_mixed_subtracts_cumulative_domain_contribution():
         moved += o.scalar("deltaV") != 0.0
     assert moved >= 3
 
+
+
+# ---------------------------------------------------------------- restart files
+RESTART_VARS = {"z_bed": "m", "z_ss": "m", "dz_ss": "m", "w": "m", "we": "m", "dz_ss_ref": "m",
+                "H_ice_ref": "m", "z_bed_ref": "m", "w_ref": "m", "we_ref": "m", "He_lith": "km",
+                "mask_active": "1", "log10_eta_eff": "Pa s"}   # isostasy_io.f90:105-136
+
+
+def _sealevel_run(t_end, step=10.0):
+    wl = W.sealevel_cycle(nx=32, dx=50e3, pad=400e3)
+    o = O.OracleIsos(O.make_params(**wl["params"]), wl["nx"], wl["nx"], wl["dx"])
+    b = O.OracleBSL("fastiso", 0.0)
+    H = lambda t: wl["H_max"] * min(1.0, t / 150.0)
+    o.init_state(wl["z_bed"], H(0.0), 0.0, b)
+    t = 0.0
+    while t < t_end:
+        t += step
+        b.update(t)
+        o.update(H(t), t, b)
+    return wl, o, b, H
+
+
+def test_restart_file_holds_reference_variables(tmp_path):
+    wl, o, b, H = _sealevel_run(100.0)
+    path = tmp_path / "isos_restart.bin"
+    o.restart_write(path, 100.0)
+    recs, t = O.read_restart_file(path)
+    assert t == 100.0
+    assert {k: v[0] for k, v in recs.items()} == RESTART_VARS
+    for name, field in (("z_bed", "z_bed"), ("w", "w"), ("we", "we"), ("dz_ss", "dz_ss"), ("z_ss", "z_ss"),
+                        ("z_bed_ref", "z_bed_ref"), ("w_ref", "w_ref")):
+        assert np.array_equal(recs[name][1], o.get_full(field)), name
+    assert np.array_equal(recs["He_lith"][1], o.get_full("He_lith") * float(np.float32(1e-3)))
+    assert np.array_equal(recs["mask_active"][1], o.get_full("maskactive"))
+    assert np.array_equal(recs["log10_eta_eff"][1], np.log10(o.get_full("eta_eff")))
+    assert np.abs(recs["we"][1]).max() > 0 and np.abs(recs["dz_ss"][1]).max() > 0
+
+
+def test_restart_continues_the_run(tmp_path):
+    """isos_init_state after isos_restart_read (fastisostasy.f90:609-615):
+    the state is restored (ref and now), z_bed / H_ice re-embedded, the ODE
+    restarts at dt_init.  The continued run stays within the integrator's
+    tolerance of the uninterrupted one."""
+    wl, o, b, H = _sealevel_run(100.0)
+    path = tmp_path / "isos_restart.bin"
+    o.restart_write(path, 100.0)
+    bpath = tmp_path / "bsl_restart.bin"
+    b.restart_write(bpath, 100.0)
+    bsl0, A0 = O.bsl_restart_read(bpath)
+    assert bsl0 == b.bsl_now and A0 == b.A_ocean_now
+    o2 = O.OracleIsos(O.make_params(**wl["params"]), wl["nx"], wl["nx"], wl["dx"])
+    o2.restart_read(path, 100.0)
+    for name in ("z_bed_ref", "w_ref"):
+        assert np.array_equal(o2.get_full(name), o.get_full(name))
+    b2 = O.OracleBSL("fastiso", bsl0)
+    o2.init_state(o.get("z_bed"), H(100.0), 100.0, b2)
+    assert np.array_equal(o2.get_full("w"), o.get_full("w") - 0.25 * (
+        o.get_full("w")[0, 0] + o.get_full("w")[-1, -1] + o.get_full("w")[-1, 0] + o.get_full("w")[0, -1]))
+    t = 100.0
+    while t < 200.0:
+        t += 10.0
+        b.update(t); b2.update(t)
+        o.update(H(t), t, b)
+        o2.update(H(t), t, b2)
+    w1, w2 = o.get("w"), o2.get("w")
+    assert np.abs(w1).max() > 1.0
+    assert np.abs(w1 - w2).max() < 1e-2 * np.abs(w1).max(), np.abs(w1 - w2).max()
