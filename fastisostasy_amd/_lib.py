@@ -24,7 +24,8 @@ SCALARS = ["n", "t", "dt", "bsl", "deltaV", "V_af", "V_den", "attempts", "accept
 SCALAR_ID = {n: i for i, n in enumerate(SCALARS)}
 
 # every entry point of include/fastiso.h
-EXPORTS = ["fi_last_error", "fi_version", "fi_padded_size", "fi_isos_create", "fi_isos_init_ref",
+EXPORTS = ["fi_last_error", "fi_version", "fi_padded_size", "fi_isos_create", "fi_isos_create_gridded",
+           "fi_isos_init_ref",
            "fi_isos_init_state", "fi_isos_update", "fi_isos_get", "fi_isos_get_mask", "fi_isos_get_full",
            "fi_isos_get_scalar", "fi_isos_restart_write", "fi_isos_restart_read", "fi_bsl_restart_write",
            "fi_bsl_restart_read", "fi_isos_destroy", "fi_bsl_create", "fi_bsl_create_series",
@@ -57,6 +58,21 @@ class FiParams(ctypes.Structure):
     ]
 
 
+class FiRheologyGrid(ctypes.Structure):
+    """fi_rheology_grid: the arrays of the rheology_file / mask_file inputs."""
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int),
+        ("xc", ctypes.POINTER(ctypes.c_double)), ("yc", ctypes.POINTER(ctypes.c_double)),
+        ("litho_thickness", ctypes.POINTER(ctypes.c_double)), ("litho_thickness_scaling", ctypes.c_double),
+        ("nz", ctypes.c_int), ("zc", ctypes.POINTER(ctypes.c_double)),
+        ("log10_eta", ctypes.POINTER(ctypes.c_double)), ("stddev_log10_eta", ctypes.POINTER(ctypes.c_double)),
+        ("rheo_smoothing_radius", ctypes.c_double),
+        ("mask_nx", ctypes.c_int), ("mask_ny", ctypes.c_int),
+        ("mask_xc", ctypes.POINTER(ctypes.c_double)), ("mask_yc", ctypes.POINTER(ctypes.c_double)),
+        ("mask", ctypes.POINTER(ctypes.c_double)),
+    ]
+
+
 _lib = None
 
 
@@ -77,6 +93,8 @@ def load():
     L.fi_version.restype = i
     L.fi_padded_size.argtypes = [i, i, d, d, d, ctypes.POINTER(i)]
     L.fi_isos_create.argtypes = [ctypes.POINTER(FiParams), i, i, d, d, dp, dp, dp, dp, ctypes.POINTER(vp)]
+    L.fi_isos_create_gridded.argtypes = [ctypes.POINTER(FiParams), i, i, d, d, dp, dp,
+                                         ctypes.POINTER(FiRheologyGrid), ctypes.POINTER(vp)]
     L.fi_isos_init_ref.argtypes = [vp, dp, dp, dp, dp]
     L.fi_isos_init_state.argtypes = [vp, dp, dp, d, vp]
     L.fi_isos_update.argtypes = [vp, dp, d, vp]

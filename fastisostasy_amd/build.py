@@ -9,7 +9,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfastiso.so")
-SOURCES = ["rhs.hip", "rhs_fast.hip", "sealevel.hip", "init.hip", "conv.hip", "kelvin.cpp", "api.cpp"]
+SOURCES = ["rhs.hip", "rhs_fast.hip", "sealevel.hip", "init.hip", "conv.hip", "kelvin.cpp", "rheogrid.cpp", "api.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",

@@ -779,11 +779,33 @@ int fi_bsl_destroy(fi_bsl* b) {
     return 0;
 }
 
+static int create_impl(const fi_params* p, int nx_ice, int ny_ice, double dx, double dy, const double* K,
+                       const double* rho_ice, const double* he_field, const double* eta_field,
+                       const fi_rheology_grid* grid, fi_isos** out);
+
 int fi_isos_create(const fi_params* p, int nx_ice, int ny_ice, double dx, double dy, const double* K,
                    const double* rho_ice, const double* he_field, const double* eta_field, fi_isos** out) {
+    return create_impl(p, nx_ice, ny_ice, dx, dy, K, rho_ice, he_field, eta_field, nullptr, out);
+}
+
+int fi_isos_create_gridded(const fi_params* p, int nx_ice, int ny_ice, double dx, double dy, const double* K,
+                           const double* rho_ice, const fi_rheology_grid* grid, fi_isos** out) {
+    if (!grid) return fail(FI_E_ARG, "NULL rheology grid");
+    return create_impl(p, nx_ice, ny_ice, dx, dy, K, rho_ice, nullptr, nullptr, grid, out);
+}
+
+static int create_impl(const fi_params* p, int nx_ice, int ny_ice, double dx, double dy, const double* K,
+                       const double* rho_ice, const double* he_field, const double* eta_field,
+                       const fi_rheology_grid* grid, fi_isos** out) {
     int rc = ensure_device();
     if (rc) return rc;
     if (!p || !out) return fail(FI_E_ARG, "NULL argument");
+    if (p->lithosphere == 4 && !(grid && grid->litho_thickness && grid->xc && grid->yc && grid->nx > 1 && grid->ny > 1))
+        return fail(FI_E_ARG, "lithosphere=rheology_file needs the gridded thickness (fi_isos_create_gridded)");
+    if (p->mantle == 4 && !(grid && grid->log10_eta && grid->zc && grid->nz >= 1 && p->lithosphere == 4))
+        return fail(FI_E_ARG, "mantle=rheology_file needs the gridded log10 viscosity and lithosphere=rheology_file");
+    if (grid && grid->mask && !(grid->mask_xc && grid->mask_yc && grid->mask_nx > 1 && grid->mask_ny > 1))
+        return fail(FI_E_ARG, "mask_file grid is incomplete");
     if (p->method < 0 || p->method > 3)
         return fail(FI_E_ARG, "method must be 0 (none), 1 (LLRA), 2 (LV-ELRA) or 3 (LV-ELVA)");
     if (p->nl < 1 || p->nl > FI_MAX_LAYERS) return fail(FI_E_ARG, "nl out of range");
@@ -797,7 +819,8 @@ int fi_isos_create(const fi_params* p, int nx_ice, int ny_ice, double dx, double
     if (rho_ice) h->par.rho_ice = *rho_ice;
     rc = domain_size(nx_ice, ny_ice, dx, dy, p->min_pad, h->n, h->icrop1, h->jcrop1);
     if (rc) { delete h; return rc; }
-    if (p->interactive_sealevel && !(p->min_pad > (double)100.e3f)) {
+    const bool mask_file = grid && grid->mask;
+    if (p->interactive_sealevel && !mask_file && !(p->min_pad > (double)100.e3f)) {
         delete h;
         return fail(FI_E_ARG, "interactive_sealevel=true requires a mask file or at least 100km of padding.");
     }
@@ -909,14 +932,17 @@ int fi_isos_create(const fi_params* p, int nx_ice, int ny_ice, double dx, double
         HIPCHK(hipMemcpy(M.K2, K2.data(), nn * 8, hipMemcpyHostToDevice));
     }
 
-    // ---- maskactive (fastisostasy.f90:185-198) in bit 3 of both masks
+    // ---- maskactive (fastisostasy.f90:185-213) in bit 3 of both masks
     {
-        std::vector<uint8_t> mk(nn, 0);
+        std::vector<uint8_t> mk(nn, 0), from_file;
+        if (mask_file && P.method >= 2) grid_mask(*grid, n, dx, from_file);
         for (int j = 0; j < n; ++j)
             for (int i = 0; i < n; ++i) {
                 // methods 0 and 1 set maskactive everywhere (fastisostasy.f90:444-447)
                 bool act = true;
-                if (P.interactive_sealevel && P.method >= 2)
+                if (!from_file.empty())
+                    act = from_file[(size_t)j * n + i] != 0;
+                else if (P.interactive_sealevel && P.method >= 2)
                     act = (i >= h->icrop1 && i < h->icrop1 + nx_ice && j >= h->jcrop1 && j < h->jcrop1 + ny_ice);
                 mk[(size_t)j * n + i] = act ? 8 : 0;
             }
@@ -926,8 +952,18 @@ int fi_isos_create(const fi_params* p, int nx_ice, int ny_ice, double dx, double
 
     // ---- rheology (fastisostasy.f90:166-172, 240-442)
     if (P.method == 3) {
+        // rheology_file: interpolated on the host, then used as field inputs
+        std::vector<double> he_grid, eta_grid;
+        if (P.lithosphere == 4 || P.mantle == 4) {
+            rc = grid_rheology(P, *grid, n, dx, he_grid, eta_grid);
+            if (rc) return rc;
+            if (P.lithosphere == 4) he_field = he_grid.data();
+            if (P.mantle == 4) eta_field = eta_grid.data();
+        }
         InitArgs ia{};
-        ia.n = n; ia.dx = dx; ia.nl = P.nl; ia.lithosphere = P.lithosphere; ia.mantle = P.mantle;
+        ia.n = n; ia.dx = dx; ia.nl = P.nl;
+        ia.lithosphere = P.lithosphere == 4 ? 3 : P.lithosphere;
+        ia.mantle = P.mantle == 4 ? 3 : P.mantle;
         ia.layering = P.layering; ia.lumping = P.lumping; ia.vscale_method = P.viscosity_scaling_method;
         ia.vscale = P.viscosity_scaling;
         for (int k = 0; k < FI_MAX_LAYERS; ++k) { ia.zl[k] = P.zl[k]; ia.visc[k] = P.viscosities[k]; }
@@ -935,16 +971,24 @@ int fi_isos_create(const fi_params* p, int nx_ice, int ny_ice, double dx, double
         ia.comp_corr = P.correct_compression ? 1.5 / (1 + P.nu) : 1.0;
         ia.r_earth = P.r_earth; ia.m_earth = P.m_earth;
         double *d_he = nullptr, *d_eta = nullptr;
-        if (P.lithosphere == 3) {
+        if (ia.lithosphere == 3) {
             HIPCHK(hipMalloc(&d_he, nn * 8));
             HIPCHK(hipMemcpy(d_he, he_field, nn * 8, hipMemcpyHostToDevice));
         }
-        if (P.mantle == 3) {
+        if (ia.mantle == 3) {
             HIPCHK(hipMalloc(&d_eta, nn * 8 * P.nl));
             HIPCHK(hipMemcpy(d_eta, eta_field, nn * 8 * P.nl, hipMemcpyHostToDevice));
         }
         launch_init_rheology(h->d_mem, ia, d_he, d_eta, h->eta_eff, h->kappa, h->R, h->He, h->Dd, scratch, h->fp64,
                              g_stream);
+        if (P.mantle == 4 && grid->rheo_smoothing_radius > 0.0) {
+            // smooth_gauss_2D of eta_eff before the scaling tail (fastisostasy.f90:404-407)
+            std::vector<double> e(nn);
+            HIPCHK(hipStreamSynchronize(g_stream));
+            HIPCHK(hipMemcpy(e.data(), h->eta_eff, nn * 8, hipMemcpyDeviceToHost));
+            grid_smooth(n, e.data(), dx, grid->rheo_smoothing_radius);
+            HIPCHK(hipMemcpy(h->eta_eff, e.data(), nn * 8, hipMemcpyHostToDevice));
+        }
         launch_init_finalize(ia, scratch, h->eta_eff, h->kappa, h->R, h->Dd, M.D, M.eta, M.kR, h->fp64, g_stream);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(g_stream));

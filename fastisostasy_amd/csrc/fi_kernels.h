@@ -3,6 +3,9 @@
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
+#include "../../include/fastiso.h"
 #include "fi_types.h"
 
 namespace fi {
@@ -101,6 +104,11 @@ void launch_init_finalize(const InitArgs& a, const double* d_scratch, double* d_
                           const double* d_R, const double* d_Dd, void* D_t, void* eta_t, void* kR_t,
                           bool fp64, hipStream_t st);
 void launch_green(int which, int n, double dx, double r_earth, double m_earth, double* d_out, hipStream_t st);
+// rheology_file / mask_file inputs on the host (rheogrid.cpp)
+void grid_smooth(int n, double* var, double dx, double sigma);
+int grid_rheology(const fi_params& P, const fi_rheology_grid& g, int n, double dx, std::vector<double>& he,
+                  std::vector<double>& eta);
+void grid_mask(const fi_rheology_grid& g, int n, double dx, std::vector<uint8_t>& active);
 // LV-ELRA viscous Green's function on the host (kelvin.cpp)
 void green_viscous_host(int n, double dx, double L_w, double D_lith, double* out);
 double kelvin_kei_value(double x);

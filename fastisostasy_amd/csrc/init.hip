@@ -201,16 +201,18 @@ void launch_init_rheology(const Member* d_mem, const InitArgs& a, const double* 
     }
     hipLaunchKernelGGL(k_rheo, cells_grid(n), dim3(256), 0, st, a, (const double*)d_kappa, (const double*)gshape,
                        (const double*)gmax, d_he, d_eta, d_etaeff, d_R, d_He, d_Dd);
-    if (a.vscale_method == 1 || a.vscale_method == 3)
-        hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, st, (const double*)d_etaeff, nn,
-                           a.vscale_method == 1 ? 1 : 2, red);
+    (void)red;
 }
 
 void launch_init_finalize(const InitArgs& a, const double* d_scratch, double* d_etaeff, const double* d_kappa,
                           const double* d_R, const double* d_Dd, void* D_t, void* eta_t, void* kR_t,
                           bool fp64, hipStream_t st) {
     const size_t nn = (size_t)a.n * a.n;
-    const double* red = d_scratch + nn + 1;
+    double* red = const_cast<double*>(d_scratch) + nn + 1;
+    // min / max scaling of the (possibly smoothed) eta_eff (fastisostasy.f90:413-436)
+    if (a.vscale_method == 1 || a.vscale_method == 3)
+        hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, st, (const double*)d_etaeff, nn,
+                           a.vscale_method == 1 ? 1 : 2, red);
     if (fp64)
         hipLaunchKernelGGL(k_vfinal<double>, cells_grid(a.n), dim3(256), 0, st, a, red, d_etaeff, d_kappa, d_R,
                            d_Dd, (double*)D_t, (double*)eta_t, (double*)kR_t);

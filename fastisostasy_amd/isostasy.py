@@ -22,7 +22,7 @@ import numpy as np
 from . import _lib
 from ._lib import FiParams, check, load
 
-MANTLE = {"uniform": 0, "gaussian_plus": 1, "gaussian_minus": 2, "fields": 3}
+MANTLE = {"uniform": 0, "gaussian_plus": 1, "gaussian_minus": 2, "fields": 3, "rheology_file": 4}
 LITHOSPHERE = MANTLE
 LAYERING = {"parallel": 0, "equalized": 1}
 LUMPING = {"bottom": 0, "min": 1, "max": 2, "mean": 3, "logmean": 4, "freqdomain": 5}
@@ -69,6 +69,25 @@ def padded_size(nx_ice, ny_ice, dx, dy, min_pad):
     n = ctypes.c_int()
     check(load().fi_padded_size(nx_ice, ny_ice, dx, dy, min_pad, ctypes.byref(n)))
     return n.value
+
+
+def rheology_grid(xc=None, yc=None, litho_thickness=None, litho_thickness_scaling=1.0, zc=None,
+                  log10_eta=None, stddev_log10_eta=None, rheo_smoothing_radius=0.0, mask_xc=None,
+                  mask_yc=None, mask=None):
+    """fi_rheology_grid from arrays ([ny, nx] / [nz, ny, nx], coordinates in m);
+    returns (struct, arrays that must outlive it)."""
+    keep = {k: _c(v) for k, v in dict(xc=xc, yc=yc, litho_thickness=litho_thickness, zc=zc,
+                                      log10_eta=log10_eta, stddev_log10_eta=stddev_log10_eta,
+                                      mask_xc=mask_xc, mask_yc=mask_yc, mask=mask).items()}
+    g = _lib.FiRheologyGrid()
+    for k, v in keep.items():
+        setattr(g, k, _dp(v))
+    g.nx, g.ny = (0 if xc is None else len(xc)), (0 if yc is None else len(yc))
+    g.nz = 0 if zc is None else len(zc)
+    g.mask_nx, g.mask_ny = (0 if mask_xc is None else len(mask_xc)), (0 if mask_yc is None else len(mask_yc))
+    g.litho_thickness_scaling = litho_thickness_scaling
+    g.rheo_smoothing_radius = rheo_smoothing_radius
+    return g, keep
 
 
 class BslClass:
@@ -120,7 +139,9 @@ class IsosClass:
     """isos_class (src/isostasy_defs.f90:202-209) held on the GPU."""
 
     def __init__(self, params, nx_ice, ny_ice, dx, dy=None, K=None, rho_ice=None, he_field=None,
-                 eta_field=None):
+                 eta_field=None, grid=None):
+        """grid: the rheology_file / mask_file arrays as keywords of
+        rheology_grid() (xc, yc, litho_thickness, zc, log10_eta, ...)."""
         dy = dx if dy is None else dy
         if isinstance(params, dict):
             params = make_params(**params)
@@ -128,8 +149,13 @@ class IsosClass:
         keep = [_c(K), _c(he_field), _c(eta_field)]
         rho = None if rho_ice is None else np.array([rho_ice], dtype=np.float64)
         h = ctypes.c_void_p()
-        check(load().fi_isos_create(ctypes.byref(params), nx_ice, ny_ice, dx, dy, _dp(keep[0]), _dp(rho),
-                                    _dp(keep[1]), _dp(keep[2]), ctypes.byref(h)))
+        if grid is not None:
+            self._grid = rheology_grid(**grid)
+            check(load().fi_isos_create_gridded(ctypes.byref(params), nx_ice, ny_ice, dx, dy, _dp(keep[0]),
+                                                _dp(rho), ctypes.byref(self._grid[0]), ctypes.byref(h)))
+        else:
+            check(load().fi_isos_create(ctypes.byref(params), nx_ice, ny_ice, dx, dy, _dp(keep[0]), _dp(rho),
+                                        _dp(keep[1]), _dp(keep[2]), ctypes.byref(h)))
         self.h = h
         self.nx_ice, self.ny_ice = nx_ice, ny_ice
         self.n = int(self.scalar("n"))

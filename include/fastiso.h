@@ -54,8 +54,8 @@ typedef struct {
     int method;                   /* 0 no displacement, 1 LLRA, 2 LV-ELRA, 3 LV-ELVA */
     double dt_diagnostics;        /* [yr] */
     double min_pad;               /* [m]  namelist "pad" */
-    int mantle;                   /* 0 uniform, 1 gaussian_plus, 2 gaussian_minus, 3 fields */
-    int lithosphere;              /* 0 uniform, 1 gaussian_plus, 2 gaussian_minus, 3 fields */
+    int mantle;                   /* 0 uniform, 1 gaussian_plus, 2 gaussian_minus, 3 fields, 4 rheology_file */
+    int lithosphere;              /* 0 uniform, 1 gaussian_plus, 2 gaussian_minus, 3 fields, 4 rheology_file */
     int layering;                 /* 0 parallel, 1 equalized */
     int lumping;                  /* 0 bottom, 1 min, 2 max, 3 mean, 4 logmean, 5 freqdomain */
     int viscosity_scaling_method; /* 0 as-is, 1 min, 2 scale, 3 max, 4 stddev (no-op) */
@@ -108,6 +108,37 @@ int fi_padded_size(int nx_ice, int ny_ice, double dx, double dy, double min_pad,
 int fi_isos_create(const fi_params* p, int nx_ice, int ny_ice, double dx, double dy,
                    const double* K, const double* rho_ice, const double* he_field,
                    const double* eta_field, fi_isos** out);
+/* The contents of the rheology_file and mask_file NetCDF inputs of isos_init
+ * (src/fastisostasy.f90:199-213, 262-290, 336-411), read by the caller:
+ * coordinates in metres (as nc_read_and_scale_vec leaves them), fields x
+ * fastest (Fortran (nx, ny[, nz]) order).  lithosphere = 4 interpolates
+ * litho_thickness (m) x litho_thickness_scaling onto the domain; mantle = 4
+ * (needs lithosphere = 4, as in the reference) interpolates log10_eta per
+ * level, adds viscosity_scaling x stddev_log10_eta when
+ * viscosity_scaling_method is stddev, clamps to [16, 23] and interpolates to
+ * the layer boundaries; both smooth with rheo_smoothing_radius > 0.  mask
+ * (or NULL) gives maskactive = interpolated mask > 0.5. */
+typedef struct {
+    int nx, ny;
+    const double* xc;
+    const double* yc;
+    const double* litho_thickness;
+    double litho_thickness_scaling;
+    int nz;
+    const double* zc;
+    const double* log10_eta;
+    const double* stddev_log10_eta;
+    double rheo_smoothing_radius;
+    int mask_nx, mask_ny;
+    const double* mask_xc;
+    const double* mask_yc;
+    const double* mask;
+} fi_rheology_grid;
+
+/* isos_init with the rheology / mask file branches (src/fastisostasy.f90:64-473) */
+int fi_isos_create_gridded(const fi_params* p, int nx_ice, int ny_ice, double dx, double dy,
+                           const double* K, const double* rho_ice, const fi_rheology_grid* grid,
+                           fi_isos** out);
 /* isos_init_ref (src/fastisostasy.f90:550-575); bsl and dz_ss (N x N) may be NULL */
 int fi_isos_init_ref(fi_isos* h, const double* z_bed, const double* H_ice,
                      const double* bsl, const double* dz_ss);

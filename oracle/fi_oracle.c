@@ -664,9 +664,97 @@ static void gaussian_shape(int n, const double *xc, const double *yc, double *f)
         }
 }
 
+/* bilinear_interpolation (isos_utils.f90:495-558): flat (nearest edge value in
+ * both directions) when either coordinate lies within 1 m of or beyond the
+ * grid ends; z is x fastest */
+static double bilinear_point(int nxr, int nyr, const double *x, const double *y, const double *z,
+                             double xo, double yo) {
+    const double tol = 1.0;
+    int i = 0, j = 0, flat = 0;
+    double tx = 0.0, ty = 0.0;
+    if (xo <= x[0] + tol) { i = 0; flat = 1; }
+    else if (xo >= x[nxr - 1] - tol) { i = nxr - 1; flat = 1; }
+    else {
+        for (i = 0; i < nxr - 1; i++) if (xo > x[i] && xo <= x[i + 1]) break;
+        tx = (xo - x[i]) / (x[i + 1] - x[i]);
+    }
+    if (yo <= y[0] + tol) { j = 0; flat = 1; }
+    else if (yo >= y[nyr - 1] - tol) { j = nyr - 1; flat = 1; }
+    else {
+        for (j = 0; j < nyr - 1; j++) if (yo > y[j] && yo <= y[j + 1]) break;
+        ty = (yo - y[j]) / (y[j + 1] - y[j]);
+    }
+    if (flat) return z[(size_t)j * nxr + i];
+    const double z11 = z[(size_t)j * nxr + i], z21 = z[(size_t)j * nxr + i + 1];
+    const double z12 = z[(size_t)(j + 1) * nxr + i], z22 = z[(size_t)(j + 1) * nxr + i + 1];
+    return z11 * (1.0 - tx) * (1.0 - ty) + z21 * tx * (1.0 - ty) + z12 * (1.0 - tx) * ty + z22 * tx * ty;
+}
+
+void or_bilinear(int nxr, int nyr, const double *x, const double *y, const double *z,
+                 int nxo, int nyo, const double *xo, const double *yo, double *out) {
+    for (int j = 0; j < nyo; j++)
+        for (int i = 0; i < nxo; i++) out[(size_t)j * nxo + i] = bilinear_point(nxr, nyr, x, y, z, xo[i], yo[j]);
+}
+
+/* smooth_gauss_2D (isos_utils.f90:696-817) with gauss_values (:918-957): a
+ * (2 n2 + 1)^2 Gaussian window, n2 = ceiling(2 sigma / dx), normalised twice,
+ * over the field mirrored at its edges (edge cells repeated) */
+void or_smooth_gauss(int nx, int ny, double *var, double dx, double sigma) {
+    const int n2 = (int)ceil(2.0 * sigma / dx), n = 2 * n2 + 1;
+    double *f = (double *)malloc(sizeof(double) * (size_t)n * n);
+    for (int j = -n2; j <= n2; j++)
+        for (int i = -n2; i <= n2; i++) {
+            const double x = i * dx, y = j * dx;
+            f[(size_t)(j + n2) * n + (i + n2)] =
+                1.0 / (2.0 * PI_REF * (sigma * sigma)) * exp(-(x * x + y * y) / (2 * (sigma * sigma)));
+        }
+    double fs = 0.0;
+    for (size_t q = 0; q < (size_t)n * n; q++) fs += f[q];
+    for (size_t q = 0; q < (size_t)n * n; q++) f[q] = f[q] / fs;
+    fs = 0.0;
+    for (size_t q = 0; q < (size_t)n * n; q++) fs += f[q];
+    for (size_t q = 0; q < (size_t)n * n; q++) f[q] = f[q] / fs;
+    const int ex = nx + 2 * n2, ey = ny + 2 * n2;
+    double *e = (double *)malloc(sizeof(double) * (size_t)ex * ey);
+#define EXT(i, j) e[(size_t)((j) + n2) * ex + ((i) + n2)]   /* 0-based i in [-n2, nx+n2) */
+    for (int j = 0; j < ny; j++) for (int i = 0; i < nx; i++) EXT(i, j) = var[(size_t)j * nx + i];
+    for (int j = 0; j < ny; j++) {
+        for (int i = -1; i >= -n2; i--) EXT(i, j) = EXT(-i - 1, j);
+        for (int i = nx; i < nx + n2; i++) EXT(i, j) = EXT(2 * nx - 1 - i, j);
+    }
+    for (int i = -n2; i < nx + n2; i++) {
+        for (int j = -1; j >= -n2; j--) EXT(i, j) = EXT(i, -j - 1);
+        for (int j = ny; j < ny + n2; j++) EXT(i, j) = EXT(i, 2 * ny - 1 - j);
+    }
+    for (int j = 0; j < ny; j++)
+        for (int i = 0; i < nx; i++) {
+            double acc = 0.0;
+            for (int b = -n2; b <= n2; b++)
+                for (int a = -n2; a <= n2; a++) acc += EXT(i + a, j + b) * f[(size_t)(b + n2) * n + (a + n2)];
+            var[(size_t)j * nx + i] = acc;
+        }
+#undef EXT
+    free(e); free(f);
+}
+
+static or_isos *create_impl(const or_params *p, int nx_ice, int ny_ice, double dx, double dy,
+                            const double *K, const double *rho_ice, const double *he_field,
+                            const double *eta_field, const or_rheology_grid *g);
+
+or_isos *or_isos_create_gridded(const or_params *p, int nx_ice, int ny_ice, double dx, double dy,
+                                const double *K, const double *rho_ice, const or_rheology_grid *g) {
+    return create_impl(p, nx_ice, ny_ice, dx, dy, K, rho_ice, NULL, NULL, g);
+}
+
 or_isos *or_isos_create(const or_params *p, int nx_ice, int ny_ice, double dx, double dy,
                         const double *K, const double *rho_ice,
                         const double *he_field, const double *eta_field) {
+    return create_impl(p, nx_ice, ny_ice, dx, dy, K, rho_ice, he_field, eta_field, NULL);
+}
+
+static or_isos *create_impl(const or_params *p, int nx_ice, int ny_ice, double dx, double dy,
+                            const double *K, const double *rho_ice, const double *he_field,
+                            const double *eta_field, const or_rheology_grid *g) {
     if (p->method < 0 || p->method > 3) { set_err("method must be 0 (none), 1 (LLRA), 2 (LV-ELRA) or 3 (LV-ELVA)"); return NULL; }
     if (p->nl < 1 || p->nl > OR_MAXL) { set_err("nl out of range"); return NULL; }
     or_isos *s = (or_isos *)calloc(1, sizeof *s);
@@ -702,8 +790,13 @@ or_isos *or_isos_create(const or_params *p, int nx_ice, int ny_ice, double dx, d
     or_green_elastic(n, dx, gk); conv_spectrum(s->conv, gk, s->FGE);
     or_green_ssh(n, dx, s->par.r_earth, s->par.m_earth, gk); conv_spectrum(s->conv, gk, s->FGN);
 
-    /* maskactive, fastisostasy.f90:185-198 */
-    if (!s->par.interactive_sealevel) {
+    /* maskactive, fastisostasy.f90:185-213 */
+    if (g && g->mask) {
+        double *mr = (double *)malloc(sizeof(double) * nn);
+        or_bilinear(g->mask_nx, g->mask_ny, g->mask_xc, g->mask_yc, g->mask, n, n, xc, xc, mr);
+        for (size_t c = 0; c < nn; c++) s->maskactive[c] = mr[c] > 0.5 ? 1 : 0;
+        free(mr);
+    } else if (!s->par.interactive_sealevel) {
         for (size_t c = 0; c < nn; c++) s->maskactive[c] = 1;
     } else if (s->par.min_pad > 100.e3f) {
         for (int j = s->jcrop1 - 1; j < s->jcrop2; j++)
@@ -741,6 +834,16 @@ or_isos *or_isos_create(const or_params *p, int nx_ice, int ny_ice, double dx, d
             if (!he_field) { set_err("lithosphere=fields needs he_field"); free(gk); free(xc); or_isos_destroy(s); return NULL; }
             memcpy(s->He_lith, he_field, sizeof(double) * nn);
             break;
+        case 4: {   /* rheology_file, fastisostasy.f90:262-290 */
+            if (!g || !g->litho_thickness) { set_err("lithosphere=rheology_file needs the gridded thickness"); free(gk); free(xc); or_isos_destroy(s); return NULL; }
+            const size_t nr = (size_t)g->nx * g->ny;
+            double *T = (double *)malloc(sizeof(double) * nr);
+            for (size_t q = 0; q < nr; q++) T[q] = g->litho_thickness[q] * g->litho_thickness_scaling;
+            or_bilinear(g->nx, g->ny, g->xc, g->yc, T, n, n, xc, xc, s->He_lith);
+            free(T);
+            if (g->rheo_smoothing_radius > 0.0) or_smooth_gauss(n, n, s->He_lith, dx, g->rheo_smoothing_radius);
+            break;
+        }
         default: set_err("unknown lithosphere"); free(gk); free(xc); or_isos_destroy(s); return NULL;
         }
         if (s->par.lithosphere != 0)
@@ -770,6 +873,36 @@ or_isos *or_isos_create(const or_params *p, int nx_ice, int ny_ice, double dx, d
             memcpy(eta, eta_field, sizeof(double) * nn * (size_t)nl);
             or_lumping(n, nl, s->par.lumping, eta, bnd, s->kappa, s->eta_eff, s->R);
             break;
+        case 4: {   /* rheology_file, fastisostasy.f90:336-411 */
+            if (!g || !g->log10_eta || !g->zc || g->nz < 1) { set_err("mantle=rheology_file needs the gridded viscosity"); free(bnd); free(eta); free(gk); free(xc); or_isos_destroy(s); return NULL; }
+            const int nz = g->nz;
+            const size_t nr = (size_t)g->nx * g->ny;
+            double *depth = (double *)malloc(sizeof(double) * (size_t)nz);
+            for (int i = 0; i < nz; i++) depth[i] = g->zc[nz - 1] - g->zc[nz - 1 - i];
+            double *bil = (double *)malloc(sizeof(double) * nn * (size_t)nz), *tmp = (double *)malloc(sizeof(double) * nn);
+            for (int i = 0; i < nz; i++) {
+                or_bilinear(g->nx, g->ny, g->xc, g->yc, g->log10_eta + (size_t)(nz - 1 - i) * nr, n, n, xc, xc, bil + (size_t)i * nn);
+                if (s->par.viscosity_scaling_method == 4 && g->stddev_log10_eta) {
+                    or_bilinear(g->nx, g->ny, g->xc, g->yc, g->stddev_log10_eta + (size_t)(nz - 1 - i) * nr, n, n, xc, xc, tmp);
+                    for (size_t c = 0; c < nn; c++) bil[(size_t)i * nn + c] = bil[(size_t)i * nn + c] + s->par.viscosity_scaling * tmp[c];
+                }
+            }
+            for (size_t q = 0; q < nn * (size_t)nz; q++) {
+                if (bil[q] < 16) bil[q] = 16;
+                if (bil[q] > 23) bil[q] = 23;
+            }
+            double *col = (double *)malloc(sizeof(double) * (size_t)nz);
+            for (int k = 0; k < nl; k++)
+                for (size_t c = 0; c < nn; c++) {
+                    for (int i = 0; i < nz; i++) col[i] = bil[(size_t)i * nn + c];
+                    eta[(size_t)k * nn + c] = or_interp_0d(nz, depth, col, bnd[(size_t)k * nn + c]);
+                }
+            for (size_t q = 0; q < nn * (size_t)nl; q++) eta[q] = pow(10.0, eta[q]);
+            or_lumping(n, nl, s->par.lumping, eta, bnd, s->kappa, s->eta_eff, s->R);
+            if (g->rheo_smoothing_radius > 0.0) or_smooth_gauss(n, n, s->eta_eff, dx, g->rheo_smoothing_radius);
+            free(depth); free(bil); free(tmp); free(col);
+            break;
+        }
         default: set_err("unknown mantle"); free(bnd); free(eta); free(gk); free(xc); or_isos_destroy(s); return NULL;
         }
         free(bnd); free(eta);

@@ -28,8 +28,8 @@ typedef struct {
     int method;                 /* 0 none, 1 LLRA, 2 LV-ELRA, 3 LV-ELVA */
     double dt_diagnostics;      /* [yr] */
     double min_pad;             /* [m] */
-    int mantle;                 /* 0 uniform, 1 gaussian_plus, 2 gaussian_minus, 3 fields */
-    int lithosphere;            /* 0 uniform, 1 gaussian_plus, 2 gaussian_minus, 3 fields */
+    int mantle;                 /* 0 uniform, 1 gaussian_plus, 2 gaussian_minus, 3 fields, 4 rheology_file */
+    int lithosphere;            /* 0 uniform, 1 gaussian_plus, 2 gaussian_minus, 3 fields, 4 rheology_file */
     int layering;               /* 0 parallel, 1 equalized */
     int lumping;                /* 0 bottom, 1 min, 2 max, 3 mean, 4 logmean, 5 freqdomain */
     int viscosity_scaling_method; /* 0 as-is, 1 min, 2 scale, 3 max, 4 stddev */
@@ -71,6 +71,28 @@ enum {
 or_isos *or_isos_create(const or_params *p, int nx_ice, int ny_ice, double dx, double dy,
                         const double *K, const double *rho_ice,
                         const double *he_field, const double *eta_field);
+/* The arrays of the rheology_file and mask_file NetCDF inputs of isos_init
+ * (fastisostasy.f90:199-213, 262-290, 336-411), coordinates in metres,
+ * fields x fastest (Fortran (nx, ny[, nz]) order). */
+typedef struct {
+    int nx, ny;
+    const double *xc, *yc;
+    const double *litho_thickness;      /* nx*ny [m], lithosphere = 4 */
+    double litho_thickness_scaling;
+    int nz;
+    const double *zc;                   /* nz [m] */
+    const double *log10_eta;            /* nx*ny*nz, mantle = 4 */
+    const double *stddev_log10_eta;     /* nx*ny*nz, viscosity_scaling_method = stddev */
+    double rheo_smoothing_radius;       /* [m] */
+    int mask_nx, mask_ny;
+    const double *mask_xc, *mask_yc, *mask;   /* mask_file, or NULL */
+} or_rheology_grid;
+or_isos *or_isos_create_gridded(const or_params *p, int nx_ice, int ny_ice, double dx, double dy,
+                                const double *K, const double *rho_ice, const or_rheology_grid *g);
+/* bilinear_interpolation_array (isos_utils.f90:495-575), smooth_gauss_2D (:696-817) */
+void or_bilinear(int nxr, int nyr, const double *x, const double *y, const double *z,
+                 int nxo, int nyo, const double *xo, const double *yo, double *out);
+void or_smooth_gauss(int nx, int ny, double *var, double dx, double sigma);
 int  or_isos_init_ref(or_isos *s, const double *z_bed, const double *H_ice,
                       const double *bsl, const double *dz_ss);
 int  or_isos_init_state(or_isos *s, const double *z_bed, const double *H_ice,

@@ -28,7 +28,7 @@ SCALARS = ["n", "t", "dt", "bsl", "deltaV", "V_af", "V_den", "attempts", "accept
            "D_lith11", "icrop1", "jcrop1", "nx_ice", "ny_ice"]
 SCALAR_ID = {name: i for i, name in enumerate(SCALARS)}
 
-MANTLE = {"uniform": 0, "gaussian_plus": 1, "gaussian_minus": 2, "fields": 3}
+MANTLE = {"uniform": 0, "gaussian_plus": 1, "gaussian_minus": 2, "fields": 3, "rheology_file": 4}
 LITHOSPHERE = MANTLE
 LAYERING = {"parallel": 0, "equalized": 1}
 LUMPING = {"bottom": 0, "min": 1, "max": 2, "mean": 3, "logmean": 4, "freqdomain": 5}
@@ -56,6 +56,42 @@ class OrParams(ctypes.Structure):
         ("dt_min", ctypes.c_double), ("dt_max", ctypes.c_double),
         ("dt_init", ctypes.c_double), ("q", ctypes.c_double), ("tau", ctypes.c_double),
     ]
+
+
+class OrRheologyGrid(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int),
+        ("xc", ctypes.POINTER(ctypes.c_double)), ("yc", ctypes.POINTER(ctypes.c_double)),
+        ("litho_thickness", ctypes.POINTER(ctypes.c_double)), ("litho_thickness_scaling", ctypes.c_double),
+        ("nz", ctypes.c_int), ("zc", ctypes.POINTER(ctypes.c_double)),
+        ("log10_eta", ctypes.POINTER(ctypes.c_double)), ("stddev_log10_eta", ctypes.POINTER(ctypes.c_double)),
+        ("rheo_smoothing_radius", ctypes.c_double),
+        ("mask_nx", ctypes.c_int), ("mask_ny", ctypes.c_int),
+        ("mask_xc", ctypes.POINTER(ctypes.c_double)), ("mask_yc", ctypes.POINTER(ctypes.c_double)),
+        ("mask", ctypes.POINTER(ctypes.c_double)),
+    ]
+
+
+def make_grid(struct_type, xc=None, yc=None, litho_thickness=None, litho_thickness_scaling=1.0, zc=None,
+              log10_eta=None, stddev_log10_eta=None, rheo_smoothing_radius=0.0, mask_xc=None, mask_yc=None,
+              mask=None):
+    """Gridded rheology / mask inputs (arrays [ny, nx] or [nz, ny, nx]) in a
+    ctypes struct; returns (struct, arrays to keep alive)."""
+    keep = {k: (None if v is None else np.ascontiguousarray(v, dtype=np.float64)) for k, v in dict(
+        xc=xc, yc=yc, litho_thickness=litho_thickness, zc=zc, log10_eta=log10_eta,
+        stddev_log10_eta=stddev_log10_eta, mask_xc=mask_xc, mask_yc=mask_yc, mask=mask).items()}
+    g = struct_type()
+    ptr = lambda a: None if a is None else a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    for k, v in keep.items():
+        setattr(g, k, ptr(v))
+    g.nx = 0 if xc is None else len(xc)
+    g.ny = 0 if yc is None else len(yc)
+    g.nz = 0 if zc is None else len(zc)
+    g.mask_nx = 0 if mask_xc is None else len(mask_xc)
+    g.mask_ny = 0 if mask_yc is None else len(mask_yc)
+    g.litho_thickness_scaling = litho_thickness_scaling
+    g.rheo_smoothing_radius = rheo_smoothing_radius
+    return g, keep
 
 
 def build():
@@ -109,6 +145,12 @@ def lib():
         L.or_lumping.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp, dp, dp, dp]
         L.or_green_elastic.argtypes = [ctypes.c_int, ctypes.c_double, dp]
         L.or_green_ssh.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, dp]
+        L.or_isos_create_gridded.restype = vp
+        L.or_isos_create_gridded.argtypes = [ctypes.POINTER(OrParams), ctypes.c_int, ctypes.c_int,
+                                             ctypes.c_double, ctypes.c_double, dp, dp,
+                                             ctypes.POINTER(OrRheologyGrid)]
+        L.or_bilinear.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp, dp, ctypes.c_int, ctypes.c_int, dp, dp, dp]
+        L.or_smooth_gauss.argtypes = [ctypes.c_int, ctypes.c_int, dp, ctypes.c_double, ctypes.c_double]
         L.or_kei.argtypes = [ctypes.c_double]
         L.or_kei.restype = ctypes.c_double
         L.or_kei_table.argtypes = [dp, dp]
@@ -204,6 +246,19 @@ def green_ssh(n, dx, r_earth, m_earth):
     return out
 
 
+def bilinear(x, y, z, xo, yo):
+    x, y, z, xo, yo = _c(x), _c(y), _c(z), _c(xo), _c(yo)
+    out = np.empty((len(yo), len(xo)))
+    lib().or_bilinear(len(x), len(y), _p(x), _p(y), _p(z), len(xo), len(yo), _p(xo), _p(yo), _p(out))
+    return out
+
+
+def smooth_gauss(var, dx, sigma):
+    out = _c(var).copy()
+    lib().or_smooth_gauss(out.shape[1], out.shape[0], _p(out), dx, sigma)
+    return out
+
+
 def kei(x):
     return lib().or_kei(float(x))
 
@@ -280,14 +335,20 @@ class OracleBSL:
 
 class OracleIsos:
     def __init__(self, params, nx_ice, ny_ice, dx, dy=None, K=None, rho_ice=None,
-                 he_field=None, eta_field=None):
+                 he_field=None, eta_field=None, grid=None):
+        """grid: keyword arrays of the rheology_file / mask_file inputs (make_grid)."""
         dy = dx if dy is None else dy
         self._keep = [_c(a) if a is not None else None for a in (K, he_field, eta_field)]
         Kc, he, eta = self._keep
         rho = None if rho_ice is None else np.array([rho_ice], dtype=np.float64)
         self._rho = rho
-        self.h = lib().or_isos_create(ctypes.byref(params), nx_ice, ny_ice, dx, dy,
-                                      _p(Kc), _p(rho), _p(he), _p(eta))
+        if grid is not None:
+            self._grid = make_grid(OrRheologyGrid, **grid)
+            self.h = lib().or_isos_create_gridded(ctypes.byref(params), nx_ice, ny_ice, dx, dy, _p(Kc), _p(rho),
+                                                  ctypes.byref(self._grid[0]))
+        else:
+            self.h = lib().or_isos_create(ctypes.byref(params), nx_ice, ny_ice, dx, dy,
+                                          _p(Kc), _p(rho), _p(he), _p(eta))
         if not self.h:
             raise RuntimeError(lib().or_last_error().decode())
         self.n = int(self.scalar("n"))

@@ -482,3 +482,49 @@ def test_bsl_restart_file_read_by_oracle(tmp_path):
     assert O.bsl_restart_read(path) == (b.bsl_now, b.A_ocean_now) == (-1.25, 3.5e14)
     recs, t = O.read_restart_file(path)
     assert t == 10.0 and {k: v[0] for k, v in recs.items()} == {"bsl": "m", "A_ocean": "m^2"}
+
+
+# ---------------------------------------------------------------- rheology / mask files
+def rheology_file_case():
+    """A two-layer domain (24^2 ice cells at 100 km, N = 32) whose lithosphere,
+    mantle viscosity and active mask come from gridded files on their own,
+    coarser grids, smoothed over 150 km."""
+    nx, dx = 24, 100e3
+    p = dict(W.relax_disc(3)["params"], min_pad=400e3, nl=2, zl=[88e3, 250e3], viscosities=[1e21, 1e21],
+             layering="equalized", viscosity_scaling_method="stddev", viscosity_scaling=0.5,
+             lithosphere="rheology_file", mantle="rheology_file", interactive_sealevel=1)
+    xr = np.linspace(-2500e3, 2500e3, 37)
+    X, Y = np.meshgrid(xr, xr)
+    T = 80e3 + 30e3 * np.exp(-(X * X + (Y - 300e3) ** 2) / (2 * 700e3 ** 2))
+    zc = np.array([-400e3, -300e3, -200e3, -100e3, 0.0])
+    lev = np.stack([19.5 + 0.6 * k + 0.4 * np.sin(X / 500e3) * np.cos(Y / 700e3) for k in range(len(zc))])
+    sd = np.stack([0.1 + 0.05 * np.cos(X / 900e3) for _ in zc])
+    xm = np.linspace(-1900e3, 1900e3, 29)
+    XM, YM = np.meshgrid(xm, xm)
+    mask = ((np.abs(XM) < 1000e3) & (YM > -800e3)).astype(float)
+    grid = dict(xc=xr, yc=xr, litho_thickness=T, litho_thickness_scaling=1.05, zc=zc, log10_eta=lev,
+                stddev_log10_eta=sd, rheo_smoothing_radius=150e3, mask_xc=xm, mask_yc=xm, mask=mask)
+    return p, nx, dx, grid
+
+
+def test_rheology_file_init_matches_oracle():
+    p, nx, dx, grid = rheology_file_case()
+    o = O.OracleIsos(O.make_params(**p), nx, nx, dx, grid=grid)
+    g = G.IsosClass(G.make_params(**p), nx, nx, dx, grid=grid)
+    assert g.n == o.n == 32
+    for f in ("He_lith", "D_lith", "kappa", "R", "eta_eff", "maskactive"):
+        a, b = o.get_full(f), g.full(f)
+        assert np.allclose(a, b, rtol=1e-13, atol=0), (f, np.abs(a - b).max())
+    act = g.full("maskactive")
+    assert 0 < act.sum() < act.size
+    H = W.disc_load(nx, dx, r0=600e3)
+    z_bed = np.full((nx, nx), 1e6)
+    ob, gb = O.OracleBSL("const", 0.0), G.bsl_init("const", 0.0)
+    o.init_state(z_bed, np.zeros_like(H), 0.0, ob)
+    g.init_state(z_bed, np.zeros_like(H), 0.0, gb)
+    for k in range(1, 21):
+        t = 10.0 * k
+        o.update(H, t, ob)
+        g.update(H, t, gb)
+    compare(o, g, wtol=1e-6)
+    assert np.abs(g.out("w")).max() > 1e-3

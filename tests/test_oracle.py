@@ -496,3 +496,112 @@ def test_restart_continues_the_run(tmp_path):
     w1, w2 = o.get("w"), o2.get("w")
     assert np.abs(w1).max() > 1.0
     assert np.abs(w1 - w2).max() < 1e-2 * np.abs(w1).max(), np.abs(w1 - w2).max()
+
+
+# ---------------------------------------------------------------- rheology / mask files
+def test_bilinear_reference_rules():
+    """bilinear_interpolation (isos_utils.f90:495-558): exact on bilinear
+    functions inside the grid; within 1 m of an end (or beyond) in either
+    direction the cell value is returned flat in both directions."""
+    x = np.linspace(0.0, 100e3, 11)
+    y = np.linspace(0.0, 50e3, 6)
+    X, Y = np.meshgrid(x, y)
+    z = 3.0 * X + 2.0 * Y + 1e-6 * X * Y + 7.0
+    xo = np.array([0.5, 12.5e3, 55e3, 99999.5])
+    yo = np.array([2.5e3, 31e3, 49999.5])
+    out = O.bilinear(x, y, z, xo, yo)
+    Xo, Yo = np.meshgrid(xo, yo)
+    want = 3.0 * Xo + 2.0 * Yo + 1e-6 * Xo * Yo + 7.0
+    assert np.allclose(out[:2, 1:3], want[:2, 1:3], rtol=1e-14)
+    # x within 1 m of the lower end: z(1, j) of the y cell found, no y interpolation
+    assert out[0, 0] == z[0, 0] and out[1, 0] == z[3, 0]
+    # y within 1 m of the upper end: row ny, x cell index i (lower node), flat
+    assert out[2, 1] == z[-1, 1] and out[2, 2] == z[-1, 5] and out[0, 3] == z[0, -1]
+    out1 = O.bilinear(x, y, z, np.array([1.5]), np.array([2.5e3]))   # 1.5 m is inside: interpolated
+    assert out1[0, 0] != z[0, 0]
+    out2 = O.bilinear(x, y, z, np.array([-5e3, 200e3]), np.array([-1e3]))
+    assert out2[0, 0] == z[0, 0] and out2[0, 1] == z[0, -1]
+
+
+def test_smooth_gauss_kernel_width_and_constants():
+    """smooth_gauss_2D: a constant stays constant (mirrored edges, normalised
+    window); a point load spreads with the variance of a Gaussian of width
+    sigma truncated at the window edge (scipy.stats.truncnorm)."""
+    from scipy import stats
+    assert np.abs(O.smooth_gauss(np.full((20, 30), 5.0), 10e3, 15e3) - 5.0).max() < 1e-13
+    dx, sigma, n = 1.0, 20.0, 101
+    f = np.zeros((n, n))
+    f[n // 2, n // 2] = 1.0
+    s = O.smooth_gauss(f, dx, sigma)
+    x = np.arange(n) - n // 2
+    var_x = (s.sum(axis=0) * x * x).sum() / s.sum()
+    # the window keeps its end samples: a normal truncated at +-(n2 + 1/2) dx, n2 = 2 sigma / dx
+    tn = stats.truncnorm(-40.5 / sigma, 40.5 / sigma, scale=sigma)
+    assert abs(var_x - tn.var()) / tn.var() < 2e-3
+    assert abs(s.sum() - 1.0) < 1e-12 and s[n // 2, n // 2 + 41] == 0.0
+
+
+def test_rheology_file_branch_on_aligned_grid():
+    """With the file grid on the domain's own coordinates bilinear returns the
+    nodes, so the rheology_file branch must equal the field-input path fed with
+    numpy's version of its steps: He = T x scaling; per boundary, linear
+    interpolation (np.interp) in depth zc(nz) - zc of the level-reversed,
+    [16, 23]-clipped log10 viscosity (+ scaling x stddev), then 10**."""
+    wl = W.relax_disc(3)
+    nx, dx = 24, 100e3
+    p = dict(wl["params"], min_pad=400e3, nl=2, zl=[88e3, 250e3], viscosities=[1e21, 1e21],
+             layering="equalized", viscosity_scaling_method="stddev", viscosity_scaling=0.5)
+    n = W.padded_size(nx, dx, p["min_pad"])
+    xc = W.padded_coords(n, dx)
+    X, Y = np.meshgrid(xc, xc)
+    T = 80e3 + 30e3 * np.exp(-(X * X + Y * Y) / (2 * 600e3 ** 2))
+    zc = np.array([-400e3, -300e3, -200e3, -100e3, 0.0])
+    lev = np.stack([20.0 + 0.5 * k + 0.3 * np.sin(X / 400e3) for k in range(len(zc))])
+    lev[-1] += 3.0   # above the clamp at the surface level
+    sd = np.stack([0.2 + 0.0 * X for _ in zc])
+    grid = dict(xc=xc, yc=xc, litho_thickness=T, litho_thickness_scaling=1.1, zc=zc, log10_eta=lev,
+                stddev_log10_eta=sd)
+    pp = dict(p, lithosphere="rheology_file", mantle="rheology_file")
+    og = O.OracleIsos(O.make_params(**pp), nx, nx, dx, grid=grid)
+
+    def on_nodes(a):
+        # interior nodes come back exactly; on an edge row (column) the flat
+        # return takes the lower node of the cell holding the other coordinate
+        b = a.copy()
+        b[0, 1:-1], b[-1, 1:-1] = a[0, :-2], a[-1, :-2]
+        b[1:-1, 0], b[1:-1, -1] = a[:-2, 0], a[:-2, -1]
+        return b
+
+    He = on_nodes(T * 1.1)
+    depth = zc[-1] - zc[::-1]
+    vals = np.clip(np.stack([on_nodes(v) for v in lev[::-1] + 0.5 * sd[::-1]]), 16, 23)
+    bnd = [He, np.full_like(He, 250e3)]
+    eta = np.empty((2, n, n))
+    for k in range(2):
+        for j in range(n):
+            for i in range(n):
+                eta[k, j, i] = 10.0 ** np.interp(bnd[k][j, i], depth, vals[:, j, i])
+    pf = dict(p, lithosphere="fields", mantle="fields")
+    of = O.OracleIsos(O.make_params(**pf), nx, nx, dx, he_field=He, eta_field=eta)
+    assert np.array_equal(og.get_full("He_lith"), He)
+    assert np.array_equal(og.get_full("He_lith")[1:-1, 1:-1], (T * 1.1)[1:-1, 1:-1])
+    assert np.array_equal(og.get_full("D_lith"), of.get_full("D_lith"))
+    # numpy's interp and 10** round differently from the restatement's: ulp-level
+    for f in ("R", "eta_eff"):
+        assert np.allclose(og.get_full(f), of.get_full(f), rtol=1e-12, atol=0), f
+
+
+def test_mask_file_branch():
+    """mask_file (fastisostasy.f90:199-213): maskactive = interpolated mask > 0.5;
+    methods 0 and 1 still set it everywhere."""
+    nx, dx = 24, 100e3
+    p = dict(W.relax_disc(3)["params"], min_pad=400e3, interactive_sealevel=1)
+    n = W.padded_size(nx, dx, p["min_pad"])
+    xc = W.padded_coords(n, dx)
+    X, Y = np.meshgrid(xc, xc)
+    m = (np.abs(X) < 900e3).astype(float) * (Y > -500e3)
+    grid = dict(mask_xc=xc, mask_yc=xc, mask=m)
+    o = O.OracleIsos(O.make_params(**p), nx, nx, dx, grid=grid)
+    assert np.array_equal(o.get_full("maskactive")[1:-1, 1:-1], (m > 0.5).astype(float)[1:-1, 1:-1])
+    o1 = O.OracleIsos(O.make_params(**dict(p, method=1)), nx, nx, dx, grid=grid)
+    assert np.all(o1.get_full("maskactive") == 1)
