@@ -53,3 +53,34 @@ def test_fortran_driver_matches_oracle(tmp_path):
         o.update(wl["H_ice"], t, b)
     assert np.abs(o.get("w") - w_f).max() < 1e-6
     assert np.abs(o.get("z_bed") - zb_f).max() < 1e-6
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("method", [1, 2])
+def test_fortran_driver_relaxation_methods(tmp_path, method):
+    """The namelist path of the module with method = 1 (LLRA) and 2 (LV-ELRA),
+    tau and rho_uppermantle as in the reference's test1b/test1c parameter sets."""
+    if not os.path.exists(EXE):
+        pytest.skip("drive_disc not built")
+    text = open(NML).read()
+    assert "    method = 3\n" in text and "rho_uppermantle = 3300.0" in text
+    nml = tmp_path / f"relax{method}.nml"
+    nml.write_text(text.replace("    method = 3\n", f"    method = {method}\n", 1)
+                   .replace("rho_uppermantle = 3300.0", "rho_uppermantle = 3400.0"))
+    years = 300.0
+    out = tmp_path / "w.bin"
+    r = subprocess.run([EXE, str(nml), "128", "25e3", str(years), str(out)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    raw = np.fromfile(out, dtype=np.float64)
+    w_f = raw[:128 * 128].reshape(128, 128)
+    wl = W.relax_disc(method)
+    o = O.OracleIsos(O.make_params(**wl["params"]), wl["nx"], wl["nx"], wl["dx"])
+    b = O.OracleBSL("const", 0.0)
+    o.init_state(wl["z_bed"], np.zeros_like(wl["H_ice"]), 0.0, b)
+    for k in range(1, int(years / 10) + 1):
+        t = 10.0 * k
+        b.update(t)
+        o.update(wl["H_ice"], t, b)
+    assert np.abs(o.get("w")).max() > 1.0
+    assert np.abs(o.get("w") - w_f).max() < 1e-6
