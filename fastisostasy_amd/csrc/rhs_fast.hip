@@ -67,13 +67,15 @@ __global__ void __launch_bounds__(THREADS) kf_row_fwd(const Member* __restrict__
 // Flexure stencil for uniform spacing (lv_elva.f90:60-108 with the one-sided
 // differences of isos_utils.f90 at the edges) fused with the forward row
 // transform, so f never leaves the CU.  The block owns 2L rows [R0, R0+2L);
-// thread `tid` owns columns tid + THREADS q.  All global inputs of the block
-// are loaded up front (w rows R0-2 .. R0+2L+1, D rows R0-1 .. R0+2L, canom_full
-// and eta rows R0 .. R0+2L-1), then the block marches down the rows:
-// iteration m (local row, -1 .. 2L+1) forms the moments of row m and f of row
-// m - 2.  x-neighbours go through LDS rings (w 3 rows, Mxx 3, Mxy 4); the
-// y-direction (w and Myy of the owned column) stays in registers.  Edge rows
-// pick the shifted windows the clamped centre rows of the reference select.
+// thread `tid` owns columns tid + THREADS q.  Global inputs are loaded
+// SR_LA march steps ahead of their use (w rows R0-2 .. R0+2L+1, D rows
+// R0-1 .. R0+2L, canom_full and eta rows R0 .. R0+2L-1), and the block
+// marches down the rows: step m (local row, -1 .. 2L+1) forms the moments of
+// row m and f of row m - 2, with one barrier per step.  x-neighbours go
+// through LDS rings (w 3 rows, Mxx 3, Mxy 2); each row's x-derivatives of w
+// and Mxy are formed once and kept in registers; the y-direction (w and Myy
+// of the owned column) stays in registers.  Edge rows pick the shifted
+// windows the clamped centre rows of the reference select.
 //
 // Block order is XCD-aware: consecutive workgroup ids go to different XCDs,
 // so the row blocks of one member are mapped to ids that share an XCD and
@@ -81,7 +83,7 @@ __global__ void __launch_bounds__(THREADS) kf_row_fwd(const Member* __restrict__
 template <typename T, int N, int THREADS> struct SRow {
     static constexpr int TPL = N / 8, L = THREADS / TPL, ROWS = 2 * L;
     static constexpr int CPT = N / THREADS;
-    static constexpr size_t ring_bytes = (size_t)(3 + 3 + 4) * N * sizeof(double);
+    static constexpr size_t ring_bytes = (size_t)(3 + 3 + 2) * N * sizeof(double);
     static constexpr size_t stencil_bytes = ring_bytes + (size_t)ROWS * N * sizeof(T);
     static constexpr size_t xch_bytes = (size_t)L * RowXch<T, N>::PITCH * sizeof(cplx<T>);
     static constexpr size_t lds = (size_t)N * sizeof(cplx<T>) + (stencil_bytes > xch_bytes ? stencil_bytes : xch_bytes);
@@ -121,84 +123,111 @@ __device__ __forceinline__ void srow_body(const Member& M, const kptr<PhysDev> p
     cplx<T>* tw_l = (cplx<T>*)smem;
     double* rw = (double*)(tw_l + N);     // w    rows, slot (r + 3) % 3
     double* rxx = rw + 3 * N;             // Mxx  rows, slot (r + 3) % 3
-    double* rxy = rxx + 3 * N;            // Mxy  rows, slot (r + 4) % 4
-    T* sf = (T*)(rxy + 4 * N);            // f    [ROWS][N]
+    double* rxy = rxx + 3 * N;            // Mxy  rows, slot (r + 2) % 2
+    T* sf = (T*)(rxy + 2 * N);            // f    [ROWS][N]
     const double invq = ph->inv_dx2, inv1 = ph->inv_dx, inv2 = ph->inv_2dx, nu = ph->nu, g = ph->g;
-    auto s3 = [](int r) { return (r + 3) % 3; };
-    auto s4 = [](int r) { return (r + 4) % 4; };
+    constexpr auto s3 = [](int r) { return (r + 3) % 3; };
+    constexpr auto s2 = [](int r) { return (r + 2) % 2; };
+    // per owned column: clamped centre, x-neighbours and x-spacing factor
+    int icol[CPT], ic[CPT], ilo[CPT], ihi[CPT];
+    double sx[CPT];
 #pragma unroll
     for (int q = 0; q < CPT; ++q) {
-        rw[s3(-2) * N + tid + THREADS * q] = wv[0][q];
-        rw[s3(-1) * N + tid + THREADS * q] = wv[1][q];
+        const int i = tid + THREADS * q;
+        icol[q] = i;
+        ic[q] = i < 1 ? 1 : (i > N - 2 ? N - 2 : i);
+        ilo[q] = i - 1 < 0 ? 0 : i - 1;
+        ihi[q] = i + 1 > N - 1 ? N - 1 : i + 1;
+        sx[q] = (ihi[q] - ilo[q] == 2) ? inv2 : inv1;
     }
+    // centred / one-sided x-derivative of a ring row at the owned columns
+    auto xder = [&](const double* row, int q) { return (row[ihi[q]] - row[ilo[q]]) * sx[q]; };
+    // x-derivatives of w (rows -2 .. ROWS+1) and of Mxy (rows -1 .. ROWS):
+    // each row's is formed once and kept while the three-row window needs it
+    double wx[ROWS + 4][CPT], mx[ROWS + 2][CPT];
     double my[ROWS + 2][CPT];   // Myy of local rows -1 .. ROWS
+#pragma unroll
+    for (int q = 0; q < CPT; ++q) {
+        rw[s3(-2) * N + icol[q]] = wv[0][q];
+        rw[s3(-1) * N + icol[q]] = wv[1][q];
+        rw[s3(0) * N + icol[q]] = wv[2][q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < CPT; ++q) {
+        wx[0][q] = xder(rw + s3(-2) * N, q);
+        wx[1][q] = xder(rw + s3(-1) * N, q);
+    }
+    // March step m (local row, -1 .. ROWS+1), one barrier per step: the
+    // moments of row m (w rows m, m+1 in the ring), the x-derivative of Mxy
+    // row m-1 and f of row m-2 (Mxx row m-2 in the ring); the step ends by
+    // writing w row m+2 into the slot of row m-1, which no thread reads in
+    // this step.  Row windows at the domain edges follow the clamped
+    // centres of the reference (jlo = max(g-1, 0), jhi = min(g+1, N-1)).
     static_for<-1, ROWS + 2>([&](auto mc) {
         constexpr int m = decltype(mc)::value;
         issue(mc);
+        __syncthreads();
         if constexpr (m <= ROWS) {
-            // moments of local row m (global gm); w rows m-1 .. m+1 in the ring
-            const int gm = R0 + m;
-#pragma unroll
-            for (int q = 0; q < CPT; ++q) rw[s3(m + 1) * N + tid + THREADS * q] = wv[m + 3][q];
-            __syncthreads();
-            const int jlo = (gm - 1 < 0 ? 0 : gm - 1) - R0, jhi = (gm + 1 > N - 1 ? N - 1 : gm + 1) - R0;
-            const double sy = (jhi - jlo == 2) ? inv2 : inv1;
 #pragma unroll
             for (int q = 0; q < CPT; ++q) {
-                const int i = tid + THREADS * q;
-                const int ii = i < 1 ? 1 : (i > N - 2 ? N - 2 : i);
-                const int ilo = i - 1 < 0 ? 0 : i - 1, ihi = i + 1 > N - 1 ? N - 1 : i + 1;
-                const double sx = (ihi - ilo == 2) ? inv2 : inv1;
+                const int i = icol[q], ii = ic[q];
+                wx[m + 3][q] = xder(rw + s3(m + 1) * N, q);
                 const double* wr = rw + s3(m) * N;
                 const double wxx = (wr[ii + 1] - 2 * wr[ii] + wr[ii - 1]) * invq;
                 // centre row of the y-difference: clamp(gm, 1, N-2)
                 double wa = wv[m + 3][q], wb = wv[m + 2][q], wc = wv[m + 1][q];
+                double whi = wx[m + 3][q], wlo = wx[m + 1][q], sy = inv2;
                 if constexpr (m <= 0) {
-                    if (top) { wa = wv[2 + 2][q]; wb = wv[1 + 2][q]; wc = wv[0 + 2][q]; }
+                    if (top) {
+                        wa = wv[2 + 2][q]; wb = wv[1 + 2][q]; wc = wv[0 + 2][q];
+                        if constexpr (m == -1) { whi = wx[2][q]; wlo = wx[2][q]; }
+                        else { wlo = wx[m + 2][q]; }
+                        sy = inv1;
+                    }
                 }
                 if constexpr (m >= ROWS - 1) {
-                    if (bottom) { wa = wv[ROWS - 1 + 2][q]; wb = wv[ROWS - 2 + 2][q]; wc = wv[ROWS - 3 + 2][q]; }
+                    if (bottom) {
+                        wa = wv[ROWS - 1 + 2][q]; wb = wv[ROWS - 2 + 2][q]; wc = wv[ROWS - 3 + 2][q];
+                        if constexpr (m == ROWS - 1) { whi = wx[m + 2][q]; }
+                        else { whi = wx[m + 1][q]; wlo = wx[m + 1][q]; }
+                        sy = inv1;
+                    }
                 }
                 const double wyy = (wa - 2 * wb + wc) * invq;
-                const double* wh = rw + s3(jhi) * N;
-                const double* wl = rw + s3(jlo) * N;
-                const double wx_hi = (wh[ihi] - wh[ilo]) * sx;
-                const double wx_lo = (wl[ihi] - wl[ilo]) * sx;
-                const double wxy = (wx_hi - wx_lo) * sy;
+                const double wxy = (whi - wlo) * sy;
                 const double D = (double)dv[m + 1][q];
                 rxx[s3(m) * N + i] = -(D * (wxx + nu * wyy));
                 my[m + 1][q] = -(D * (wyy + nu * wxx));
-                rxy[s4(m) * N + i] = -(D * (1.0 - nu) * wxy);
+                rxy[s2(m) * N + i] = -(D * (1.0 - nu) * wxy);
             }
-            __syncthreads();
+        }
+        if constexpr (m >= 0) {
+            // x-derivative of Mxy row m-1 (written in the previous step)
+#pragma unroll
+            for (int q = 0; q < CPT; ++q) mx[m][q] = xder(rxy + s2(m - 1) * N, q);
         }
         if constexpr (m - 2 >= 0) {
-            // f of local row j = m - 2: moments of rows j-1 .. j+1 are complete
+            // f of local row j = m - 2
             constexpr int j = m - 2;
-            const int gj = R0 + j;
-            const int jlo = (gj - 1 < 0 ? 0 : gj - 1) - R0, jhi = (gj + 1 > N - 1 ? N - 1 : gj + 1) - R0;
-            const double sy = (jhi - jlo == 2) ? inv2 : inv1;
 #pragma unroll
             for (int q = 0; q < CPT; ++q) {
-                const int i = tid + THREADS * q;
-                const int ii = i < 1 ? 1 : (i > N - 2 ? N - 2 : i);
-                const int ilo = i - 1 < 0 ? 0 : i - 1, ihi = i + 1 > N - 1 ? N - 1 : i + 1;
-                const double sx = (ihi - ilo == 2) ? inv2 : inv1;
+                const int i = icol[q], ii = ic[q];
                 const double* xr = rxx + s3(j) * N;
                 const double mxx = (xr[ii + 1] - 2 * xr[ii] + xr[ii - 1]) * invq;
                 double ya = my[j + 2][q], yb = my[j + 1][q], yc = my[j][q];
+                double hi = mx[j + 2][q], lo = mx[j][q], sy = inv2;
                 if constexpr (j == 0) {
-                    if (top) { ya = my[2 + 1][q]; yb = my[1 + 1][q]; yc = my[0 + 1][q]; }
+                    if (top) { ya = my[2 + 1][q]; yb = my[1 + 1][q]; yc = my[0 + 1][q]; lo = mx[j + 1][q]; sy = inv1; }
                 }
                 if constexpr (j == ROWS - 1) {
-                    if (bottom) { ya = my[ROWS - 1 + 1][q]; yb = my[ROWS - 2 + 1][q]; yc = my[ROWS - 3 + 1][q]; }
+                    if (bottom) {
+                        ya = my[ROWS - 1 + 1][q]; yb = my[ROWS - 2 + 1][q]; yc = my[ROWS - 3 + 1][q];
+                        hi = mx[j + 1][q]; sy = inv1;
+                    }
                 }
                 const double myy = (ya - 2 * yb + yc) * invq;
-                const double* hr = rxy + s4(jhi) * N;
-                const double* lr = rxy + s4(jlo) * N;
-                const double mxyx_hi = (hr[ihi] - hr[ilo]) * sx;
-                const double mxyx_lo = (lr[ihi] - lr[ilo]) * sx;
-                const double mxyxy = (mxyx_hi - mxyx_lo) * sy;
+                const double mxyxy = (hi - lo) * sy;
                 const double p = -(g * (double)cv[j][q]);
                 const double num = p + mxx + 2.0 * mxyxy + myy;
                 T f;
@@ -206,6 +235,10 @@ __device__ __forceinline__ void srow_body(const Member& M, const kptr<PhysDev> p
                 else f = (T)(num * (double)ev[j][q]);   // mixed: eta holds 1/(2 eta)
                 sf[j * N + i] = f;
             }
+        }
+        if constexpr (m + 2 <= ROWS + 1) {
+#pragma unroll
+            for (int q = 0; q < CPT; ++q) rw[s3(m + 2) * N + icol[q]] = wv[m + 4][q];
         }
     });
     __syncthreads();
@@ -291,13 +324,13 @@ __global__ void __launch_bounds__(THREADS) kf_srow_fwd(const Member* __restrict_
     T dv[ROWS + 2][CPT];        // -1 .. ROWS
     T cv[ROWS][CPT], ev[ROWS][CPT];
     // Software-pipelined loads: at march step tau the block issues w row
-    // tau + 1 + SR_LA, D row tau + SR_LA and canom_full / eta row
+    // tau + 2 + SR_LA, D row tau + SR_LA and canom_full / eta row
     // tau - 2 + SR_LA, each SR_LA steps before its first use.  Row bases are
     // uniform (scalar); the lane offset is one 32-bit value for every row.
     gptr<double> w = gin<double>(wpred);
     auto issue = [&](auto tc) {
         constexpr int tau = decltype(tc)::value;
-        constexpr int rw_ = tau + 1 + SR_LA, rd = tau + SR_LA, rc = tau - 2 + SR_LA;
+        constexpr int rw_ = tau + 2 + SR_LA, rd = tau + SR_LA, rc = tau - 2 + SR_LA;
 #pragma unroll
         for (int q = 0; q < CPT; ++q) {
             const unsigned col = tid + THREADS * q;
@@ -311,7 +344,7 @@ __global__ void __launch_bounds__(THREADS) kf_srow_fwd(const Member* __restrict_
         // keep the issue order: each step waits only on loads SR_LA steps old
         __builtin_amdgcn_sched_barrier(0);
     };
-    static_for<-3 - SR_LA, -1>(issue);   // everything due before the first march step
+    static_for<-4 - SR_LA, -1>(issue);   // everything due before the first march step
     {
         // the order of reduce_partials: lane-strided, then butterflies
         double amax = 0.0, s2 = 0.0;
