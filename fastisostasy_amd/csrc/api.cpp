@@ -113,6 +113,7 @@ int make_shape(int n, int nmem, bool fp64, bool uniform, RhsShape& s) {
     s.fp64 = fp64;
     s.uniform = uniform;
     s.method = 3;
+    s.dt_method = -1;
     s.fast = (n & (n - 1)) == 0 && n >= 64 && n <= 1024 && !std::getenv("FI_GENERIC_FFT");
     if (s.fast) {
         // register-FFT kernels: TPL = n/8 threads per line, LDS <= 64 KiB
@@ -371,11 +372,11 @@ int sub_count(int nmem) {
     return std::max(1, std::min({k, MAX_SUB, nmem}));
 }
 
-std::map<std::tuple<int, int, int, int>, RhsShape> g_sub_shapes;
+std::map<std::tuple<int, int, int, int, int>, RhsShape> g_sub_shapes;
 
-int sub_shape(int n, int nsub, bool fp64, bool uniform, const RhsShape*& out) {
+int sub_shape(int n, int nsub, bool fp64, bool uniform, int dt_method, const RhsShape*& out) {
     std::lock_guard<std::mutex> lk(g_mu);
-    auto key = std::make_tuple(n, nsub, (int)fp64, (int)uniform);
+    auto key = std::make_tuple(n, nsub, (int)fp64, (int)uniform, dt_method);
     auto it = g_sub_shapes.find(key);
     if (it == g_sub_shapes.end()) {
         RhsShape s{};
@@ -383,6 +384,7 @@ int sub_shape(int n, int nsub, bool fp64, bool uniform, const RhsShape*& out) {
         int rc = make_shape(n, nsub, fp64, uniform, s);
         g_mu.lock();
         if (rc) return rc;
+        s.dt_method = dt_method;
         it = g_sub_shapes.emplace(key, s).first;
     }
     out = &it->second;
@@ -415,7 +417,7 @@ bool same_shape(const RhsShape& a, const RhsShape& b) {
     return a.n == b.n && a.L == b.L && a.C == b.C && a.threads_row == b.threads_row &&
            a.threads_col == b.threads_col && a.fp64 == b.fp64 && a.uniform == b.uniform && a.fast == b.fast &&
            a.fused == b.fused && a.row_tiles == b.row_tiles && a.tw == b.tw && a.tw64 == b.tw64 &&
-           a.plan.n == b.plan.n && a.plan.nf == b.plan.nf;
+           a.plan.n == b.plan.n && a.plan.nf == b.plan.nf && a.method == b.method && a.dt_method == b.dt_method;
 }
 
 int attempt_graph(const Member* d, int nmem, const RhsShape& s, hipStream_t st, hipGraphExec_t* out) {
@@ -540,7 +542,7 @@ int run_update(std::vector<fi_isos*>& hs, Member* d_mem, const RhsShape& shape, 
             if (rc) return rc;
             for (int k = 0; k <= K; ++k) sb_lo[k] = (int)((long long)nmem * k / K);
             for (int k = 0; k < K; ++k) {
-                rc = sub_shape(n, sb_lo[k + 1] - sb_lo[k], h0.fp64, h0.uniform, sshape[k]);
+                rc = sub_shape(n, sb_lo[k + 1] - sb_lo[k], h0.fp64, h0.uniform, shape.dt_method, sshape[k]);
                 if (rc) return rc;
             }
         }
@@ -834,6 +836,7 @@ static int create_impl(const fi_params* p, int nx_ice, int ny_ice, double dx, do
     rc = make_shape(n, 1, h->fp64, h->uniform, h->shape);
     if (rc) { delete h; return rc; }
     h->shape.method = p->method;
+    h->shape.dt_method = p->dt_method;
     h->has_conv = p->include_elastic || p->heterogeneous_ssh || p->method == 2;
     if (h->has_conv) {
         rc = make_conv_shape(n, h->fp64, h->cshape);
@@ -1455,6 +1458,7 @@ int fi_batch_create(fi_isos* const* members, int nmem, fi_batch** out) {
     int rc = make_shape(members[0]->n, nmem, members[0]->fp64, members[0]->uniform, b->shape);
     if (rc) { delete b; return rc; }
     b->shape.method = members[0]->par.method;
+    b->shape.dt_method = members[0]->par.dt_method;
     if (hipMalloc(&b->d_mem, sizeof(Member) * nmem) != hipSuccess) { delete b; return fail(FI_E_HIP, "hipMalloc"); }
     rc = sum_alloc(b->sb, nmem);
     if (rc) return rc;
@@ -1675,6 +1679,7 @@ int fi_server_run(fi_server* s, int* completed) {
             rc = make_shape(h0->n, nm, h0->fp64, h0->uniform, shp);
             if (rc) return rc;
             shp.method = h0->par.method;
+            shp.dt_method = h0->par.dt_method;
             it = s->shapes.emplace(sk, shp).first;
         }
         // groupings change from round to round: plain launches, no graph captures

@@ -25,6 +25,7 @@ struct RhsShape {
     bool fused;       // fast && uniform: stencil fused into the forward row pass
     int row_tiles;    // row blocks per thread block of the inverse row pass (1 or 2)
     int method;       // 3 LV-ELVA (transform passes); 0-2 pointwise relaxation RHS
+    int dt_method;    // integrator shared by the batch (0-3), -1 if not known
 };
 
 // Geometry of the 2N-point linear convolution.

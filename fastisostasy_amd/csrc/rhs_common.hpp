@@ -46,11 +46,11 @@ struct EpiOut { double x1max, x2max, d2; };
 
 // Stage sums of integrators.f90:12-203 for one cell, in the reference's
 // association order.  Returns x2 (the candidate / new state).
-__device__ __forceinline__ double stage_sums(int method, double x, double k, double dt,
-                                             double& x1) {
-    if (method == 0) { x1 = x + dt * k; return x1; }                       // euler
-    if (method == 1) { x1 = x + (dt / 6) * (((k + 2 * k) + 2 * k) + k); return x1; }  // rk4
-    if (method == 2) {                                                     // bs32
+template <int METHOD>
+__device__ __forceinline__ double stage_sums_m(double x, double k, double dt, double& x1) {
+    if constexpr (METHOD == 0) { x1 = x + dt * k; return x1; }                       // euler
+    if constexpr (METHOD == 1) { x1 = x + (dt / 6) * (((k + 2 * k) + 2 * k) + k); return x1; }  // rk4
+    if constexpr (METHOD == 2) {                                                     // bs32
         x1 = x + dt / 9 * ((2 * k + 3 * k) + 4 * k);
         return x + dt * (((0.0 * k + 0.0 * k) + 0.0 * k) + 3 * k);
     }
@@ -59,6 +59,14 @@ __device__ __forceinline__ double stage_sums(int method, double x, double k, dou
     const double z = dt * 0.0, h6 = dt * (0.2700 - 1 / 5.0);
     x1 = x + b1 * k + z * k + z * k + b4 * k + b5 * k + h6 * k;
     return x + b1 * k + z * k + z * k + b4 * k + b5 * k + b6 * k;
+}
+
+// the integrator as a run-time value
+__device__ __forceinline__ double stage_sums(int method, double x, double k, double dt, double& x1) {
+    if (method == 0) return stage_sums_m<0>(x, k, dt, x1);
+    if (method == 1) return stage_sums_m<1>(x, k, dt, x1);
+    if (method == 2) return stage_sums_m<2>(x, k, dt, x1);
+    return stage_sums_m<3>(x, k, dt, x1);
 }
 
 __device__ __forceinline__ void block_reduce3(double a, double b, double s, double* out) {

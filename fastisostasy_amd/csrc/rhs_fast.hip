@@ -508,7 +508,8 @@ __device__ __forceinline__ void row_reduce_store(double a1a, double a2a, double 
 // the first transform, so the later tiles' data streams in while the first
 // tile is transformed (the grid is otherwise a single occupancy round in
 // which all blocks load, then compute, in lockstep).
-template <typename T, int N, int THREADS, int TILES>
+// METH: the integrator (integrators.f90) compiled in, or -1 for a run-time one
+template <typename T, int N, int THREADS, int TILES, int METH>
 __global__ void __launch_bounds__(THREADS) kf_row_inv(const Member* __restrict__ mem, const void* __restrict__ twv,
                                                       int ncolblk) {
     const Member& M = mem[blockIdx.y];
@@ -535,7 +536,9 @@ __global__ void __launch_bounds__(THREADS) kf_row_inv(const Member* __restrict__
     gmut<double> part = gout<double>(M.partials);
     auto update = [&](size_t cidx, double x, double k, double& a1, double& a2, double& s2) {
         double x1;
-        const double x2 = stage_sums(meth, x, k, dt, x1);
+        double x2;
+        if constexpr (METH >= 0) x2 = stage_sums_m<METH>(x, k, dt, x1);
+        else x2 = stage_sums(meth, x, k, dt, x1);
         Xn[cidx] = x2;
         a1 = fmax(a1, fabs(x1));
         a2 = fmax(a2, fabs(x2));
@@ -670,14 +673,23 @@ static bool srow_fwd(const Member* d, int nmem, hipStream_t st, const void* tw) 
     }
     return false;
 }
-template <typename T, int N, int THREADS>
-static void rows_inv(const Member* d, int nmem, hipStream_t st, const void* tw, int ncolblk, int tiles) {
+template <typename T, int N, int THREADS, int METH>
+static void rows_inv_m(const Member* d, int nmem, hipStream_t st, const void* tw, int ncolblk, int tiles) {
     constexpr int L = THREADS / (N / 8);
     const size_t lds = row_lds<T, N>(THREADS);
     if (tiles == 2 && (N / (2 * L)) % 2 == 0)
-        launch_k(3, kf_row_inv<T, N, THREADS, 2>, dim3(N / (4 * L), nmem), dim3(THREADS), lds, st, d, tw, ncolblk);
+        launch_k(3, kf_row_inv<T, N, THREADS, 2, METH>, dim3(N / (4 * L), nmem), dim3(THREADS), lds, st, d, tw,
+                 ncolblk);
     else
-        launch_k(3, kf_row_inv<T, N, THREADS, 1>, dim3(N / (2 * L), nmem), dim3(THREADS), lds, st, d, tw, ncolblk);
+        launch_k(3, kf_row_inv<T, N, THREADS, 1, METH>, dim3(N / (2 * L), nmem), dim3(THREADS), lds, st, d, tw,
+                 ncolblk);
+}
+// tsit54, the reference's default integrator, is compiled into the epilogue
+template <typename T, int N, int THREADS>
+static void rows_inv(const Member* d, int nmem, hipStream_t st, const void* tw, int ncolblk, int tiles,
+                     int dt_method) {
+    if (dt_method == 3) rows_inv_m<T, N, THREADS, 3>(d, nmem, st, tw, ncolblk, tiles);
+    else rows_inv_m<T, N, THREADS, -1>(d, nmem, st, tw, ncolblk, tiles);
 }
 template <typename T, int N, int C>
 static void cols(const Member* d, int nmem, hipStream_t st, const void* tw, const double2* tw64, bool commit) {
@@ -713,9 +725,9 @@ static bool fast_n(const Member* d, int nmem, const RhsShape& s, hipStream_t st,
         return false;
     }
     switch (s.threads_row) {
-    case 64: if constexpr (N / 8 <= 64) { which == 1 ? rows_fwd<T, N, 64>(d, nmem, st, s.tw) : rows_inv<T, N, 64>(d, nmem, st, s.tw, ncolblk, s.row_tiles); return true; } break;
-    case 128: if constexpr (N / 8 <= 128) { which == 1 ? rows_fwd<T, N, 128>(d, nmem, st, s.tw) : rows_inv<T, N, 128>(d, nmem, st, s.tw, ncolblk, s.row_tiles); return true; } break;
-    case 256: if constexpr (N / 8 <= 256) { which == 1 ? rows_fwd<T, N, 256>(d, nmem, st, s.tw) : rows_inv<T, N, 256>(d, nmem, st, s.tw, ncolblk, s.row_tiles); return true; } break;
+    case 64: if constexpr (N / 8 <= 64) { which == 1 ? rows_fwd<T, N, 64>(d, nmem, st, s.tw) : rows_inv<T, N, 64>(d, nmem, st, s.tw, ncolblk, s.row_tiles, s.dt_method); return true; } break;
+    case 128: if constexpr (N / 8 <= 128) { which == 1 ? rows_fwd<T, N, 128>(d, nmem, st, s.tw) : rows_inv<T, N, 128>(d, nmem, st, s.tw, ncolblk, s.row_tiles, s.dt_method); return true; } break;
+    case 256: if constexpr (N / 8 <= 256) { which == 1 ? rows_fwd<T, N, 256>(d, nmem, st, s.tw) : rows_inv<T, N, 256>(d, nmem, st, s.tw, ncolblk, s.row_tiles, s.dt_method); return true; } break;
     }
     return false;
 }
