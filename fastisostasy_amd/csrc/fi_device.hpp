@@ -30,6 +30,14 @@ template <typename C> __device__ __forceinline__ C csub(C a, C b) { a.x -= b.x; 
 template <typename C> __device__ __forceinline__ C cmul(C a, C b) {
     C r; r.x = a.x * b.x - a.y * b.y; r.y = a.x * b.y + a.y * b.x; return r;
 }
+// FP32 transforms (mixed precision) use fused multiply-adds for the twiddle
+// products; FP64 keeps separate roundings like every other FP64 operation
+__device__ __forceinline__ cplx<float> cmul(cplx<float> a, cplx<float> b) {
+    cplx<float> r;
+    r.x = __builtin_fmaf(a.x, b.x, -(a.y * b.y));
+    r.y = __builtin_fmaf(a.x, b.y, a.y * b.x);
+    return r;
+}
 // multiply by -i
 template <typename C> __device__ __forceinline__ C cmi(C a) { C r; r.x = a.y; r.y = -a.x; return r; }
 // multiply by +i
