@@ -48,6 +48,11 @@ def cases(full=False, lv_years=100.0):
     wl = W.sealevel_cycle()
     out.append(("sealevel", wl, 120e3 if full else 3e3, 5e3 if full else 500.0,
                 lambda t, wl=wl: wl["H_max"] * float(W.sawtooth(t)), "fastiso"))
+    # the reference's test1b / test1c setups: LLRA and LV-ELRA under the disc
+    for method, name in ((1, "llra_disc"), (2, "elra_disc")):
+        wl = W.relax_disc(method)
+        out.append((name, wl, 50e3 if full else 2e3, 1e3 if full else 200.0,
+                    lambda t, wl=wl: W.step_on_load(wl["H_ice"], t), "const"))
     return out
 
 
