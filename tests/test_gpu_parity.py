@@ -528,3 +528,19 @@ def test_rheology_file_init_matches_oracle():
         g.update(H, t, gb)
     compare(o, g, wtol=1e-6)
     assert np.abs(g.out("w")).max() > 1e-3
+
+
+# ---------------------------------------------------------------- integrators
+@pytest.mark.parametrize("case", ["llra", "elva"])
+@pytest.mark.parametrize("dt_method", ["euler", "rk4", "bs32"])
+def test_integrators_match_oracle(case, dt_method):
+    """euler, rk4 and bs32 (integrators.f90:12-129) through the library (the
+    inverse row pass then takes the integrator at run time) against the
+    oracle, on LLRA and on LV-ELVA."""
+    wl = W.relax_disc(1) if case == "llra" else small_disc()
+    wl["params"] = dict(wl["params"], dt_method=dt_method, dt_init=1.0)
+    times = [0.0] + [10.0 * k for k in range(1, 21)]
+    for o, g in run_pair(wl, times):
+        pass
+    assert g.scalar("attempts") == o.scalar("attempts")
+    compare(o, g, wtol=1e-8 if case == "llra" else 1e-6)

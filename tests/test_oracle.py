@@ -605,3 +605,48 @@ def test_mask_file_branch():
     assert np.array_equal(o.get_full("maskactive")[1:-1, 1:-1], (m > 0.5).astype(float)[1:-1, 1:-1])
     o1 = O.OracleIsos(O.make_params(**dict(p, method=1)), nx, nx, dx, grid=grid)
     assert np.all(o1.get_full("maskactive") == 1)
+
+
+# ---------------------------------------------------------------- other integrators
+def _llra_run(dt_method, years, **over):
+    wl = W.relax_disc(1)
+    p = dict(wl["params"], dt_method=dt_method, **over)
+    o = O.OracleIsos(O.make_params(**p), wl["nx"], wl["nx"], wl["dx"])
+    b = O.OracleBSL("const", 0.0)
+    o.init_state(wl["z_bed"], np.zeros_like(wl["H_ice"]), 0.0, b)
+    t = 0.0
+    while t < years:
+        t += 10.0
+        o.update(wl["H_ice"], t, b)
+    weq, w = o.get_full("w_equilibrium"), o.get_full("w")
+    loaded = weq != 0
+    frac = (weq[loaded] - w[loaded]) / weq[loaded]
+    assert np.ptp(frac) < 1e-12
+    return frac[0], o, p
+
+
+@pytest.mark.parametrize("method", ["euler", "rk4"])
+def test_fixed_step_integrators_closed_form(method):
+    """euler (integrators.f90:12-40) keeps dt = min(dt, tf - t), never raising
+    it (dt_init = 1 yr here); rk4 (:42-80) takes h = min(dt, tf - t) without
+    keeping it and, with the RHS fixed during an update, sums (h/6)(k+2k+2k+k).
+    Under LLRA every step maps w_eq - w to (1 - h/tau)(w_eq - w), so after
+    T years of 1-yr steps (w_eq - w) / w_eq = (1 - 1/tau)^T."""
+    T = 300.0
+    frac, o, p = _llra_run(method, T, dt_init=1.0)
+    assert abs(frac - (1 - 1.0 / p["tau"]) ** T) <= 1e-11
+    assert o.scalar("attempts") == o.scalar("accepts") == T
+
+
+def test_bs32_integer_division_weights():
+    """bs32 (integrators.f90:82-129): the weights 7/24, 1/4, 1/3 of x2 are
+    integer divisions (zero), so x2 = x + 3 dt k while x1 = x + dt k; the
+    accepted steps shrink w_eq - w by (1 - 3 dt / tau), dt <= dt_max."""
+    T = 600.0
+    frac, o, p = _llra_run("bs32", T)
+    tau, dtm = p["tau"], p["dt_max"]
+    assert (1 - 3 * dtm / tau) ** (T / dtm) - 1e-12 <= frac <= np.exp(-3 * T / tau) + 1e-12
+    assert o.scalar("accepts") >= T / dtm
+    # a tsit54 step moves by dt k (b sums to 1): the same run relaxes far less
+    frac_t, _, _ = _llra_run("tsit54", T)
+    assert (1 - dtm / tau) ** (T / dtm) - 1e-12 <= frac_t <= np.exp(-T / tau) + 1e-12 and frac < frac_t
