@@ -544,3 +544,48 @@ def test_integrators_match_oracle(case, dt_method):
         pass
     assert g.scalar("attempts") == o.scalar("attempts")
     compare(o, g, wtol=1e-8 if case == "llra" else 1e-6)
+
+
+# ---------------------------------------------------------------- init-time rheology options
+RHEO_CASES = [
+    dict(lumping="bottom"), dict(lumping="min"), dict(lumping="max"), dict(lumping="mean"),
+    dict(lumping="logmean"), dict(lumping="freqdomain", layering="parallel", dl=60e3),
+    dict(viscosity_scaling_method="min"), dict(viscosity_scaling_method="max"),
+    dict(viscosity_scaling_method="scale", viscosity_scaling=2.5, correct_compression=1),
+    dict(mantle="gaussian_plus", lithosphere="gaussian_minus"),
+    dict(mantle="gaussian_minus", lithosphere="gaussian_plus"),
+]
+
+
+@pytest.mark.parametrize("opts", RHEO_CASES, ids=lambda d: "-".join(f"{k}={v}" for k, v in d.items()))
+def test_rheology_options_match_oracle(opts):
+    """Lumping (lv_elva.f90:207-306), layering, viscosity scaling and the
+    rheology tail (fastisostasy.f90:413-442), Gaussian rheologies
+    (isos_utils.f90:960-1035): the init fields of the library against the
+    oracle's on a two-layer 96^2 domain (N = 128), then 10 updates."""
+    nx, dx = 96, 25e3
+    n = W.padded_size(nx, dx, 400e3)
+    xc = W.padded_coords(n, dx)
+    X, Y = np.meshgrid(xc, xc)
+    g = np.exp(-(X * X + Y * Y) / (2 * 600e3 ** 2))
+    he = 120e3 - 50e3 * g
+    eta = np.stack([10.0 ** (21.0 - 1.5 * g), np.full((n, n), 3e21)])
+    p = W.params(min_pad=400e3, nl=2, zl=[120e3, 300e3], viscosities=[1e21, 3e21], layering="equalized",
+                 lumping="freqdomain", lithosphere="fields", mantle="fields", dt_max=0.05)
+    p.update(opts)
+    kw = {}
+    if p["lithosphere"] == "fields":
+        kw["he_field"] = he
+    if p["mantle"] == "fields":
+        kw["eta_field"] = eta
+    o = O.OracleIsos(O.make_params(**p), nx, nx, dx, **kw)
+    gg = G.IsosClass(G.make_params(**p), nx, nx, dx, **kw)
+    assert gg.n == o.n == n
+    for f in ("kappa", "R", "eta_eff", "D_lith", "He_lith"):
+        a, b = o.get_full(f), gg.full(f)
+        assert np.allclose(a, b, rtol=1e-13, atol=0), (f, np.abs(a - b).max())
+    wl = dict(params=p, nx=nx, dx=dx, H_ice=W.parabolic_dome(nx, dx, 800e3, 2500.0),
+              z_bed=np.full((nx, nx), 1e6), **kw)
+    for o2, g2 in run_pair(wl, [0.0] + [10.0 * k for k in range(1, 11)]):
+        pass
+    compare(o2, g2, wtol=1e-6)
