@@ -615,3 +615,32 @@ def test_distorted_grid_matches_oracle(prec):
         g.update(wl["H_ice"], t, gb)
     assert np.abs(o.get("we")).max() > 1e-3
     compare(o, g, wtol=1e-6 if prec == "fp64" else 1e-3, rel=None if prec == "fp64" else 1e-5)
+
+
+# ---------------------------------------------------------------- generic transform sizes
+@pytest.mark.parametrize("n_ice,pad_cells,expect_n", [(64, 16, 96), (80, 20, 120), (96, 32, 160), (160, 32, 224)])
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_generic_sizes_match_oracle(n_ice, pad_cells, expect_n, prec):
+    """Padded sizes with factors 3, 5 and 7 run the mixed-radix LDS kernels
+    (and the non-fused stencil); against the oracle over 300 yr."""
+    wl = small_disc(n_ice=n_ice, dx=50e3, pad=pad_cells * 50e3)
+    times = [0.0] + [10.0 * k for k in range(1, 31)]
+    for o, g in run_pair(wl, times, precision=prec):
+        pass
+    assert g.n == o.n == expect_n
+    compare(o, g, wtol=1e-6 if prec == "fp64" else 1e-3, rel=None if prec == "fp64" else 1e-5)
+
+
+def test_generic_kernels_on_power_of_two(monkeypatch):
+    """FI_GENERIC_FFT forces the mixed-radix kernels for N = 128: same results
+    as the register-FFT path to FP64 rounding, and as the oracle."""
+    wl = small_disc(n_ice=96, dx=50e3, pad=16 * 50e3)
+    times = [0.0] + [10.0 * k for k in range(1, 21)]
+    for o, fast in run_pair(wl, times):
+        pass
+    monkeypatch.setenv("FI_GENERIC_FFT", "1")
+    for o2, gen in run_pair(wl, times):
+        pass
+    assert gen.n == 128
+    compare(o2, gen, wtol=1e-6)
+    assert np.abs(fast.out("w") - gen.out("w")).max() < 1e-9
