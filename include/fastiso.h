@@ -173,7 +173,7 @@ int fi_bsl_restart_read(const char* path, double* bsl, double* A_ocean);
 int fi_isos_destroy(fi_isos* h);
 
 /* bsl_init / bsl_update (src/barysealevel.f90:40-106, 195-252).
- * method 0 = const, 1 = fastiso; nvec > 0 gives the A_ocean(bsl) table of
+ * method 0 = const, 1 = fastiso (2 and 3 need fi_bsl_create_series); nvec > 0 gives the A_ocean(bsl) table of
  * A_ocean_path (otherwise the ocean surface is constant). */
 int fi_bsl_create(int method, double bsl_init, double A_ocean_pd, int nvec, const double* bsl_vec,
                   const double* A_vec, double year_bp, fi_bsl** out);
