@@ -143,7 +143,12 @@ __device__ __forceinline__ StepState decide_step(StepState s, double amax, doubl
         const double err = sqrt(s2 / (tol * tol) / nn);
         s.tol = tol; s.err = err;
         accept = err < 1;
-        if (accept) s.dt = fmin(dt * pow(1 / err, 1 / ph->q), ph->dt_max);
+        // pow(x, 1) is x exactly (C99 F.9.4.4): the default q = 1 skips the
+        // long FP64 pow sequence that every wave of the fused pass would run
+        if (accept) {
+            const double inv = 1 / err;
+            s.dt = fmin(dt * (ph->q == 1.0 ? inv : pow(inv, 1 / ph->q)), ph->dt_max);
+        }
         else s.dt = dt * 0.5;
     } else if (m == 0) {
         s.dt = dt;
