@@ -399,15 +399,34 @@ __global__ void __launch_bounds__(C * (N / 8)) kf_col(const Member* __restrict__
                                                      const double2* __restrict__ tw64) {
     const Member& M = mem[blockIdx.y];
     const kptr<Ctl> c = kview(COMMIT ? M.ctl_next : M.ctl);
-    if (COMMIT && blockIdx.x == 0 && threadIdx.x == 0) copy_ctl(M.ctl, c);
-    if ((c->status != ST_RUN) | (c->k_valid != 0) | (c->skip != 0)) return;
+    // block 0 commits after its last load: a one-lane store ahead of the loads
+    // would make every later wait a full one
+    const bool commit = COMMIT && blockIdx.x == 0 && threadIdx.x == 0;
+    if ((c->status != ST_RUN) | (c->k_valid != 0) | (c->skip != 0)) {
+        if (commit) copy_ctl(M.ctl, c);
+        return;
+    }
     using PL = P2Plan<N>;
     constexpr int TPL = PL::TPL, E = PL::E;
     extern __shared__ __align__(16) unsigned char smem[];
     cplx<T>* tw_l = (cplx<T>*)smem;
     cplx<T>* buf = tw_l + N;
     const int tid = threadIdx.x, line = tid % C, t = tid / C;
-    load_tw<T, N>(tw_l, (const cplx<T>*)twv);
+    // Loads in the order of use, each group behind a scheduling barrier so
+    // that every wait covers only its own group: twiddles (stored to LDS
+    // once the field loads are in flight), the spectrum (first transform),
+    // then kappa*R (needed only after the first transform).
+    constexpr int THREADS = C * (N / 8), TWQ = (2 * N + THREADS - 1) / THREADS;
+    T twr[TWQ];
+    {
+        gptr<T> g = gin<T>(twv);
+#pragma unroll
+        for (int k = 0; k < TWQ; ++k) {
+            if constexpr (THREADS * TWQ == 2 * N) twr[k] = g[tid + THREADS * k];
+            else if (tid + THREADS * k < 2 * N) twr[k] = g[tid + THREADS * k];
+        }
+    }
+    __builtin_amdgcn_sched_barrier(0);
     const int c0 = blockIdx.x * 2 * C;
     const int ca = c0 + 2 * line, cb = ca + 1;   // adjacent columns share a complex line
     using V2 = typename vec2<T>::type;
@@ -415,19 +434,29 @@ __global__ void __launch_bounds__(C * (N / 8)) kf_col(const Member* __restrict__
     gptr<V2> kR = gin<V2>(M.kR);
     cplx<T> v[E], m[E];
     T ka[E], kb[E];
+    V2 xs[E], xk[E];
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-        const V2 x = src[((size_t)(t + TPL * e) * N + ca) >> 1];
-        v[e] = mk<T>(x.x, x.y);
-    }
-    // kappa*R is independent of the transform: load it now
+    for (int e = 0; e < E; ++e) xs[e] = src[((size_t)(t + TPL * e) * N + ca) >> 1];
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-        const V2 x = kR[((size_t)(t + TPL * e) * N + ca) >> 1];
-        ka[e] = x.x;
-        kb[e] = x.y;
+    for (int e = 0; e < E; ++e) xk[e] = kR[((size_t)(t + TPL * e) * N + ca) >> 1];
+    // cas weights of the corner terms (used by the threads holding rows 0 and N-1)
+    const double2 wa = tw64[ca], wb = tw64[cb];
+    __builtin_amdgcn_sched_barrier(0);
+    // (no branch around the store when the threads cover the table evenly: a
+    // join here would make the compiler wait for every load in flight)
+#pragma unroll
+    for (int k = 0; k < TWQ; ++k) {
+        if constexpr (THREADS * TWQ == 2 * N) ((T*)tw_l)[tid + THREADS * k] = twr[k];
+        else if (tid + THREADS * k < 2 * N) ((T*)tw_l)[tid + THREADS * k] = twr[k];
     }
     __syncthreads();
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        v[e] = mk<T>(xs[e].x, xs[e].y);
+        ka[e] = xk[e].x;
+        kb[e] = xk[e].y;
+    }
     ColXch<T, N> xc(buf, line, C);
     reg_fft<T, N>(v, t, tw_l, xc);
     reg_mirror<T, N>(v, m, t, xc);
@@ -458,11 +487,12 @@ __global__ void __launch_bounds__(C * (N / 8)) kf_col(const Member* __restrict__
             const int q = (k == 0) ? 0 : 2;
             const double va = (double)ha, vb = (double)hb;
             cor[(size_t)ca * 4 + q] = va;
-            cor[(size_t)ca * 4 + q + 1] = va * (tw64[ca].x + tw64[ca].y);
+            cor[(size_t)ca * 4 + q + 1] = va * (wa.x + wa.y);
             cor[(size_t)cb * 4 + q] = vb;
-            cor[(size_t)cb * 4 + q + 1] = vb * (tw64[cb].x + tw64[cb].y);
+            cor[(size_t)cb * 4 + q + 1] = vb * (wb.x + wb.y);
         }
     }
+    if (commit) copy_ctl(M.ctl, c);
 }
 
 // Reduce the per-thread row statistics of two rows over the TPL threads of a
@@ -529,7 +559,10 @@ __global__ void __launch_bounds__(THREADS) kf_row_inv(const Member* __restrict__
     const double dt = attempt_dt(meth, c->dt, c->t, c->tf, ph->dt_max);
     const int cur = c->cur;
     const bool k_valid = c->k_valid != 0;
-    if (blockIdx.x == 0 && tid == 0) M.ctl->pending = 1;   // the step control reads this attempt next
+    // block 0 marks the decision pending (the step control reads this attempt
+    // next) after its last load: a one-lane store ahead of the loads would
+    // make every later wait a full one
+    const bool mark = blockIdx.x == 0 && tid == 0;
     gptr<double> X = gin<double>(M.X[cur]);
     gmut<double> Xn = gout<double>(M.X[cur ^ 1]);
     gmut<double> dw = gout<double>(M.dwdt);
@@ -576,39 +609,51 @@ __global__ void __launch_bounds__(THREADS) kf_row_inv(const Member* __restrict__
             }
             row_reduce_store<TPL>(a1a, a2a, s2a, a1b, a2b, s2b, line, t, part + (size_t)(ra[k] / 2) * 4);
         }
+        if (mark) M.ctl->pending = 1;
     } else {
         __shared__ double qs[4];
         // N*N is a power of two here: the division by it is a multiplication
         const double S = (double)(float)((double)N * (double)N), rS = 1.0 / S;
         gptr<T> src = gin<T>(M.g1);
+        // Loads in the order of use, each group behind a scheduling barrier so
+        // that every wait covers only the groups issued before it: corner
+        // terms, twiddles, spectrum, X (consumed last).
+        // The four corner sums over the N per-column terms of the column pass:
+        // quantity q is summed lane-strided, then by a fixed butterfly, by wave
+        // q (wave 0 .. NW-1 take q, q + NW, ... when there are fewer than four
+        // waves; with more, waves 4.. repeat waves 0..3), so the order is set by
+        // N alone.
+        constexpr int NW = THREADS / 64, QPW = NW >= 4 ? 1 : 4 / NW, CPL = N / 64;
+        static_assert(NW >= 1 && (NW >= 4 || 4 % NW == 0) && N % 64 == 0, "corner-sum layout");
+        const int wave = tid >> 6, lane = tid & 63;
+        double cq[QPW][CPL];
+        {
+            gptr<double> cp = gin<double>(M.corners);
+#pragma unroll
+            for (int j = 0; j < QPW; ++j) {
+                const int qi = NW >= 4 ? (wave & 3) : wave + NW * j;
+#pragma unroll
+                for (int i = 0; i < CPL; ++i) cq[j][i] = cp[(size_t)(lane + 64 * i) * 4 + qi];
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
         constexpr int TWQ = (2 * N + THREADS - 1) / THREADS;
         T twr[TWQ];
         {
             gptr<T> g = gin<T>(twv);
 #pragma unroll
-            for (int k = 0; k < TWQ; ++k)
-                if (tid + THREADS * k < 2 * N) twr[k] = g[tid + THREADS * k];
+            for (int k = 0; k < TWQ; ++k) {
+                if constexpr (THREADS * TWQ == 2 * N) twr[k] = g[tid + THREADS * k];
+                else if (tid + THREADS * k < 2 * N) twr[k] = g[tid + THREADS * k];
+            }
         }
+        __builtin_amdgcn_sched_barrier(0);
         cplx<T> v[TILES][E], m[E];
 #pragma unroll
-        for (int e = 0; e < E; ++e) v[0][e] = mk<T>(src[idx(ra[0], e)], src[idx(ra[0] + 1, e)]);
-        if (tid < 64) {
-            // the four corner sums over the N per-column terms of the column
-            // pass: lane-strided, then a fixed butterfly (order set by N alone)
-            gptr<double> cp = gin<double>(M.corners);
-            double q0 = 0, q1 = 0, q2 = 0, q3 = 0;
-#pragma unroll
-            for (int col = tid; col < N; col += 64) {
-                q0 += cp[(size_t)col * 4 + 0]; q1 += cp[(size_t)col * 4 + 1];
-                q2 += cp[(size_t)col * 4 + 2]; q3 += cp[(size_t)col * 4 + 3];
-            }
-            q0 = wave_sum(q0); q1 = wave_sum(q1); q2 = wave_sum(q2); q3 = wave_sum(q3);
-            if (tid == 0) { qs[0] = q0; qs[1] = q1; qs[2] = q2; qs[3] = q3; }
-        }
-#pragma unroll
-        for (int k = 1; k < TILES; ++k)
+        for (int k = 0; k < TILES; ++k)
 #pragma unroll
             for (int e = 0; e < E; ++e) v[k][e] = mk<T>(src[idx(ra[k], e)], src[idx(ra[k] + 1, e)]);
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int k = 0; k < TILES; ++k)
 #pragma unroll
@@ -616,9 +661,22 @@ __global__ void __launch_bounds__(THREADS) kf_row_inv(const Member* __restrict__
                 xa[k][e] = X[idx(ra[k], e)];
                 xb[k][e] = X[idx(ra[k] + 1, e)];
             }
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int k = 0; k < TWQ; ++k)
-            if (tid + THREADS * k < 2 * N) ((T*)tw_l)[tid + THREADS * k] = twr[k];
+        for (int j = 0; j < QPW; ++j) {
+            double q = 0;
+#pragma unroll
+            for (int i = 0; i < CPL; ++i) q += cq[j][i];
+            q = wave_sum(q);
+            if (lane == 0) qs[NW >= 4 ? (wave & 3) : wave + NW * j] = q;
+        }
+        // (no branch around the store when the threads cover the table evenly:
+        // a join here would make the compiler wait for every load in flight)
+#pragma unroll
+        for (int k = 0; k < TWQ; ++k) {
+            if constexpr (THREADS * TWQ == 2 * N) ((T*)tw_l)[tid + THREADS * k] = twr[k];
+            else if (tid + THREADS * k < 2 * N) ((T*)tw_l)[tid + THREADS * k] = twr[k];
+        }
         __syncthreads();
         // x(1,1) + x(nx,ny) + x(1,ny) + x(nx,1)  (isos_utils.f90:116)
         const double cc = 0.25 * (qs[0] * rS + qs[3] * rS + qs[2] * rS + qs[1] * rS);
@@ -646,6 +704,7 @@ __global__ void __launch_bounds__(THREADS) kf_row_inv(const Member* __restrict__
             }
             row_reduce_store<TPL>(a1a, a2a, s2a, a1b, a2b, s2b, line, t, part + (size_t)(ra[k] / 2) * 4);
         }
+        if (mark) M.ctl->pending = 1;
     }
 }
 
