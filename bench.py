@@ -106,6 +106,7 @@ def run_batched(nmem, steps, warmup, precision, prof_kernel=None, timed=True):
     wall = time.perf_counter() - t0
     prof = None
     attempts_timed = sum(h.scalar("attempts") for h in hs)
+    accepts_timed = sum(h.scalar("accepts") for h in hs)
     if prof_kernel is not None:
         # Kernel timing: the same number of steps again, continuing the run,
         # with HIP events on every launch of one kernel (the events slow the
@@ -127,13 +128,13 @@ def run_batched(nmem, steps, warmup, precision, prof_kernel=None, timed=True):
         check(load().fi_prof_collect(ctypes.byref(ms), ctypes.byref(cnt)))
         check(load().fi_prof_select(-1))
         prof = (ms.value, cnt.value, bool(load().fi_prof_fused()))
-    attempts = attempts_timed
+    attempts, accepts = attempts_timed, accepts_timed
     w = [h.full("w") for h in hs]
     n = hs[0].n
     batch.close()
     for h in hs:
         h.close()
-    return dict(wall=wall, lat=lat, prof=prof, attempts=attempts, w=w, n=n, t_end=t)
+    return dict(wall=wall, lat=lat, prof=prof, attempts=attempts, accepts=accepts, w=w, n=n, t_end=t)
 
 
 def run_plain(members, steps, warmup):
@@ -221,6 +222,18 @@ def run_serving(requests, n_domains, capacity_seconds):
                           "p50_ms": round(float(np.percentile(1e3 * lat_hi, 50)), 3),
                           "p99_ms": round(float(np.percentile(1e3 * lat_hi, 99)), 3),
                           "mean_group": round((st2["served"] - st1["served"]) / max(1, st2["groups"] - st1["groups"]), 2)}}
+
+
+def cpu_model():
+    """The host CPU's model name (first 'model name' line of /proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def _cpu_worker(args):
@@ -355,11 +368,12 @@ def main():
     print(f"# batched mixed: {args.members} x {n}^2, {args.steps} steps in {res['wall']:.3f} s, "
           f"{res['attempts']:.0f} attempts total", file=sys.stderr, flush=True)
 
-    max_err = None
+    max_err = rms_err = None
     if not args.no_check:
         ref = run_batched(args.members, args.steps, args.warmup, "fp64",
                           prof_kernel=args.roofline_kernel if args.roofline_kernel >= 0 else None)
         max_err = max(float(np.abs(a - b).max()) for a, b in zip(res["w"], ref["w"]))
+        rms_err = float(np.sqrt(np.mean([np.mean((a - b) ** 2) for a, b in zip(res["w"], ref["w"])])))
         print(f"# fp64 batch for the error check: {ref['wall']:.3f} s, max |dw| = {max_err:.3e} m",
               file=sys.stderr, flush=True)
 
@@ -384,6 +398,7 @@ def main():
                "sample": f"oracle (plain-C FP64 restatement of the reference path), {workers} processes on "
                          f"{workers} ensemble members (N={cp['n']}) at once, {cp['steps']} coupling steps of "
                          f"10 yr in {cp['wall']:.1f} s",
+               "cpu_model": cpu_model(), "host_threads": os.cpu_count(),
                "single_core": {"value": round(c1["value"], 1), "cores": 1,
                                "sample": f"1 member, {c1['steps']} coupling steps in {c1['wall']:.1f} s"}}
 
@@ -407,12 +422,14 @@ def main():
                        "steps": args.plain_steps},
         "speedup_vs_plain_fp64": round(value / plain_value, 2),
         "max_err_vs_fp64_m": max_err,
+        "rms_err_vs_fp64_m": rms_err,
         "step_latency_ms": ({"p50": serve["p50_ms"], "p99": serve["p99_ms"], "of": "serving requests"} if serve
                             else None),
         "batch_step_ms": {"p50": round(1e3 * percentile(res["lat"], 50), 4),
                           "p99": round(1e3 * percentile(res["lat"], 99), 4)},
         "serving": serve,
         "attempts_per_member_step": round(res["attempts"] / args.members / (args.steps + args.warmup), 2),
+        "accepts_per_member_step": round(res["accepts"] / args.members / (args.steps + args.warmup), 2),
         "roofline": roof,
         "cpu_baseline": cpu,
     }
