@@ -300,10 +300,14 @@ __global__ void __launch_bounds__(THREADS) kf_srow_fwd(const Member* __restrict_
     constexpr int TWQ = (2 * N + THREADS - 1) / THREADS;
     T twr[TWQ];
     {
+        // (no branch around the load when the threads cover the table evenly:
+        // the join would wait for the load before any row load is issued)
         gptr<T> g = gin<T>(twv);
 #pragma unroll
-        for (int k = 0; k < TWQ; ++k)
-            if (tid + THREADS * k < 2 * N) twr[k] = g[tid + THREADS * k];
+        for (int k = 0; k < TWQ; ++k) {
+            if constexpr (THREADS * TWQ == 2 * N) twr[k] = g[tid + THREADS * k];
+            else if (tid + THREADS * k < 2 * N) twr[k] = g[tid + THREADS * k];
+        }
     }
     // (loaded whether or not a decision is pending: no branch for the
     // compiler to reorder the decision around)
@@ -385,8 +389,10 @@ __global__ void __launch_bounds__(THREADS) kf_srow_fwd(const Member* __restrict_
         return;
     }
 #pragma unroll
-    for (int k = 0; k < TWQ; ++k)
-        if (tid + THREADS * k < 2 * N) ((T*)tw_l)[tid + THREADS * k] = twr[k];
+    for (int k = 0; k < TWQ; ++k) {
+        if constexpr (THREADS * TWQ == 2 * N) ((T*)tw_l)[tid + THREADS * k] = twr[k];
+        else if (tid + THREADS * k < 2 * N) ((T*)tw_l)[tid + THREADS * k] = twr[k];
+    }
     srow_body<T, N, THREADS>(M, ph, R0, tid, top, bottom, smem, wv, dv, cv, ev, issue);
     write_next();
 }
