@@ -60,27 +60,44 @@ __global__ void __launch_bounds__(PRE_THREADS) k_pre_haf(const Member* __restric
     gptr<double> Hin = gin<double>(M.H_in), zss = gin<double>(M.z_ss), zbed = gin<double>(M.z_bed),
                  Haf_ref = gin<double>(M.Haf_ref), Hice_ref = gin<double>(M.Hice_ref);
     gmut<double> Hice = gout<double>(M.Hice), Haf = gout<double>(M.Haf);
+    // every load of the thread's cells first (indices clamped, so no branch),
+    // then the arithmetic and the guarded stores
+    constexpr int IT = PRE_CELLS / PRE_THREADS;
+    double Hv[IT], zs[IT], zb[IT], hr[IT], ir[IT], Av[IT];
+    bool inside[IT];
 #pragma unroll
-    for (int it = 0; it < PRE_CELLS / PRE_THREADS; ++it) {
+    for (int it = 0; it < IT; ++it) {
+        size_t c = (size_t)blockIdx.x * PRE_CELLS + it * PRE_THREADS + threadIdx.x;
+        c = c < nn ? c : nn - 1;
+        if (new_H) {
+            // out2in embedding of the ice grid (isos_utils.f90:330-345)
+            const int i = (int)(c % N) - M.icrop1, j = (int)(c / N) - M.jcrop1;
+            inside[it] = i >= 0 && i < M.nx_ice && j >= 0 && j < M.ny_ice;
+            const int ic = i < 0 ? 0 : (i >= M.nx_ice ? M.nx_ice - 1 : i);
+            const int jc = j < 0 ? 0 : (j >= M.ny_ice ? M.ny_ice - 1 : j);
+            Hv[it] = Hin[(size_t)jc * M.nx_ice + ic];
+        } else {
+            inside[it] = true;
+            Hv[it] = Hice[c];
+        }
+        zs[it] = zss[c]; zb[it] = zbed[c]; hr[it] = Haf_ref[c]; ir[it] = Hice_ref[c];
+        Av[it] = uniform ? A_uni : M.A[c];
+    }
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
         const size_t c = (size_t)blockIdx.x * PRE_CELLS + it * PRE_THREADS + threadIdx.x;
         if (c >= nn) break;
-        double H;
-        if (new_H) {
-            const int i = (int)(c % N) - M.icrop1, j = (int)(c / N) - M.jcrop1;
-            H = (i >= 0 && i < M.nx_ice && j >= 0 && j < M.ny_ice) ? Hin[(size_t)j * M.nx_ice + i] : 0.0;
-            Hice[c] = H;
-        } else {
-            H = Hice[c];
-        }
+        const double H = inside[it] ? Hv[it] : 0.0;
+        if (new_H) Hice[c] = H;
         // calc_Haf (sealevel.f90:132-145)
-        double h = (zss[c] - zbed[c]) * rsw_ri;
+        double h = (zs[it] - zb[it]) * rsw_ri;
         if (h < 0.0) h = 0.0;
         h = H - h;
         if (h < 0.0) h = 0.0;
         Haf[c] = h;
-        const double A = uniform ? A_uni : M.A[c];
-        s1 += (h - Haf_ref[c]) * A;
-        s2 += (H - Hice_ref[c]) * Vden * A;
+        const double A = Av[it];
+        s1 += (h - hr[it]) * A;
+        s2 += (H - ir[it]) * Vden * A;
     }
     __shared__ double red[16];
     s1 = block_sum(s1, red);
@@ -120,13 +137,22 @@ __global__ void __launch_bounds__(64) k_pre_bsl(const Member* __restrict__ mem, 
     }
 }
 
-// phase 0: first pass after the BSL update; phase 1: after the elastic
-// convolution; phase 2: after the sea-surface convolution.  Without the
-// sea-surface convolution phases 1 and 2 run as one pass (phase 4), and
-// without either convolution all three do (phase 3); the fused passes apply
-// the same operations to each cell in the same order.
-template <typename T>
-__global__ void __launch_bounds__(PRE_THREADS) k_pre_fields(const Member* __restrict__ mem, int phase, int diag) {
+// Variant PH of the per-update field pass:
+//   PF_FULL   phase 0 without diagnostics, and phase 2 (after the sea-surface
+//             convolution): z_ss, masks, load and full column anomalies
+//   PF_DIAG   phase 0 with diagnostics: z_ss, masks and canom_load, then the
+//             input of the elastic convolution (or we = 0)
+//   PF_LITHO  phase 1 (after the elastic convolution): canom_litho,
+//             canom_full, mass_anom (fastisostasy.f90:811-813)
+//   PF_ALL    phases 0, 1 and 2 without convolutions (we = 0, dz_ss = 0)
+//   PF_NOSSH  phases 1 and 2 without the sea-surface convolution (dz_ss = 0)
+// The fused passes apply the same operations to each cell in the same order.
+// Each thread loads every input of its cells first and then computes and
+// stores: a store between loads would make the next wait cover it too.
+enum { PF_FULL = 0, PF_LITHO = 1, PF_ALL = 3, PF_NOSSH = 4, PF_DIAG = 5 };
+
+template <typename T, int PH>
+__global__ void __launch_bounds__(PRE_THREADS) k_pre_fields(const Member* __restrict__ mem) {
     const Member& M = mem[blockIdx.y];
     const kptr<PhysDev> ph = kview(M.phys);
     const int N = M.n;
@@ -145,76 +171,89 @@ __global__ void __launch_bounds__(PRE_THREADS) k_pre_fields(const Member* __rest
                  Haf = gin<double>(M.Haf), Haf_ref = gin<double>(M.Haf_ref), we_ref = gin<double>(M.we_ref),
                  cmantle = gin<double>(M.canom_mantle);
     gmut<T> cf_rhs = gout<T>(M.cfull);
+    constexpr bool SEA = PH != PF_LITHO;                                     // z_ss, masks, load
+    constexpr bool DZ = PH == PF_FULL || PH == PF_DIAG || PH == PF_ALL;       // reads dz_ss
+    constexpr bool WE = PH == PF_FULL || PH == PF_LITHO || PH == PF_NOSSH;    // reads we
+    constexpr bool FULL = PH != PF_DIAG;                                      // reads we_ref, canom_mantle
+    struct In {
+        uint8_t m;
+        double zr, hf, zb, hi, hfr, dz, we, wr, cm, cl, aux;
+    };
+    constexpr int IT = PRE_CELLS / PRE_THREADS;
+    In v[IT];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        size_t c = (size_t)blockIdx.x * PRE_CELLS + it * PRE_THREADS + threadIdx.x;
+        c = c < nn ? c : nn - 1;
+        v[it].m = mask[c];
+        if constexpr (SEA) {
+            v[it].zr = zss_ref[c]; v[it].hf = Haf[c]; v[it].zb = zbed[c]; v[it].hi = Hice[c]; v[it].hfr = Haf_ref[c];
+        }
+        if constexpr (DZ) v[it].dz = dzss[c];
+        if constexpr (WE) v[it].we = we[c];
+        if constexpr (FULL) { v[it].wr = we_ref[c]; v[it].cm = cmantle[c]; }
+        if constexpr (PH == PF_LITHO) {
+            v[it].cl = cload[c];
+            v[it].aux = uniform ? A_uni : M.A[c];
+        }
+        if constexpr (PH == PF_DIAG) v[it].aux = uniform ? 1.0 : M.K2[c];
+    }
     // calc_z_ss, calc_masks, seawater and load anomalies for one cell
-    auto sea_load = [&](size_t c, uint8_t mold, bool active, double dz, uint8_t& mk, bool store = true) {
-        const double z_ss = bsl + zss_ref[c] + dz;
-        const double haf = Haf[c];
-        mk = calc_mask(zbed[c], z_ss, Hice[c], haf, mold);
+    auto sea_load = [&](size_t c, const In& x, uint8_t mold, bool active, double dz, uint8_t& mk, bool store) {
+        const double z_ss = bsl + x.zr + dz;
+        const double haf = x.hf;
+        mk = calc_mask(x.zb, z_ss, x.hi, haf, mold);
         if (store) {
             zss[c] = z_ss;
             mask[c] = mk;
         }
-        const double ice = 0.0 + rho_ice * (haf - Haf_ref[c]);
-        const double sw = 0.0 + ((mk & MB_OCEAN) ? rho_water * (z_ss - zss_ref[c]) : 0.0);
+        const double ice = 0.0 + rho_ice * (haf - x.hfr);
+        const double sw = 0.0 + ((mk & MB_OCEAN) ? rho_water * (z_ss - x.zr) : 0.0);
         double load = ice;
         if (interactive) load = load + sw;
         return active ? load : 0.0;
     };
-    auto full_of = [&](size_t c, bool active, double load, double wec) {
+    auto full_of = [&](size_t c, const In& x, bool active, double load, double wec) {
         if (keep_load) cload[c] = load;   // the last calc_columnanoms_load of the update (:833)
-        const double litho = 0.0 + rho_litho * (wec - we_ref[c]);
-        const double full = active ? (load + litho + cmantle[c]) : 0.0;
+        const double litho = 0.0 + rho_litho * (wec - x.wr);
+        const double full = active ? (load + litho + x.cm) : 0.0;
         cfull[c] = full;
         cf_rhs[c] = (T)full;
     };
-    // phase 1 alone: canom_litho, canom_full, mass_anom (fastisostasy.f90:811-813)
-    auto phase1 = [&](size_t c, bool active, double wec) {
-        const double litho = 0.0 + rho_litho * (wec - we_ref[c]);
-        const double full = active ? (cload[c] + litho + cmantle[c]) : 0.0;
-        const double A = uniform ? A_uni : M.A[c];
-        if (hssh) conv[c] = active ? A * full : 0.0;
-        else dzss[c] = 0.0;
-    };
 #pragma unroll
-    for (int it = 0; it < PRE_CELLS / PRE_THREADS; ++it) {
+    for (int it = 0; it < IT; ++it) {
         const size_t c = (size_t)blockIdx.x * PRE_CELLS + it * PRE_THREADS + threadIdx.x;
         if (c >= nn) break;
-        const uint8_t mold = mask[c];
+        const In& x = v[it];
+        const uint8_t mold = x.m;
         const bool active = mold & MB_ACTIVE;
         uint8_t mk;
-        if (phase == 1) {
-            phase1(c, active, we[c]);
-            continue;
-        }
-        if (phase == 4) {
-            // phases 1 and 2 without the sea-surface convolution: dz_ss = 0
+        if constexpr (PH == PF_LITHO) {
+            const double litho = 0.0 + rho_litho * (x.we - x.wr);
+            const double full = active ? (x.cl + litho + x.cm) : 0.0;
+            if (hssh) conv[c] = active ? x.aux * full : 0.0;
+            else dzss[c] = 0.0;
+        } else if constexpr (PH == PF_NOSSH) {
             dzss[c] = 0.0;
-            const double load = sea_load(c, mold, active, 0.0, mk);
-            full_of(c, active, load, we[c]);
-            continue;
-        }
-        // (phase 3 overwrites z_ss and the masks below: no stores here)
-        const double load = sea_load(c, mold, active, dzss[c], mk, phase != 3);
-        if (phase == 3) {
-            // phases 0, 1 and 2 without convolutions: we = 0, dz_ss = 0
+            const double load = sea_load(c, x, mold, active, 0.0, mk, true);
+            full_of(c, x, active, load, x.we);
+        } else if constexpr (PH == PF_ALL) {
+            // (z_ss and the masks are overwritten by the second pass: no stores in the first)
+            const double load = sea_load(c, x, mold, active, x.dz, mk, false);
             cload[c] = load;
             we[c] = 0.0;
             dzss[c] = 0.0;
-            const double load2 = sea_load(c, mk, active, 0.0, mk);
-            full_of(c, active, load2, 0.0);
-            continue;
-        }
-        if (phase == 0 && diag) {
+            const double load2 = sea_load(c, x, mk, active, 0.0, mk, true);
+            full_of(c, x, active, load2, 0.0);
+        } else if constexpr (PH == PF_DIAG) {
+            const double load = sea_load(c, x, mold, active, x.dz, mk, true);
             cload[c] = load;
-            if (elastic) {
-                const double K2 = uniform ? 1.0 : M.K2[c];
-                conv[c] = load * g * K2;
-            } else {
-                we[c] = 0.0;
-            }
-            continue;
+            if (elastic) conv[c] = load * g * x.aux;
+            else we[c] = 0.0;
+        } else {
+            const double load = sea_load(c, x, mold, active, x.dz, mk, true);
+            full_of(c, x, active, load, x.we);
         }
-        full_of(c, active, load, we[c]);
     }
 }
 
@@ -306,19 +345,30 @@ __global__ void __launch_bounds__(PRE_THREADS) k_finalize(const Member* __restri
                  dz = gin<double>(M.dz_ss);
     gmut<double> W = gout<double>(M.W), cm = gout<double>(M.canom_mantle), zbed = gout<double>(M.z_bed);
     double mw = 0.0, mwe = 0.0, mdz = 0.0, nanf = 0.0;
+    // every load of the thread's cells first (indices clamped into the field,
+    // so no branch), then the arithmetic and the guarded stores: a store
+    // between loads would make the next wait cover it too
+    constexpr int IT = PRE_CELLS / PRE_THREADS;
+    double ws[IT], wr[IT], wea[IT], zr[IT], dzv[IT];
 #pragma unroll
-    for (int it = 0; it < PRE_CELLS / PRE_THREADS; ++it) {
+    for (int it = 0; it < IT; ++it) {
+        size_t c = (size_t)blockIdx.x * PRE_CELLS + it * PRE_THREADS + threadIdx.x;
+        c = c < nn ? c : nn - 1;
+        ws[it] = wsrc[c]; wr[it] = w_ref[c]; wea[it] = wev[c]; zr[it] = zb_ref[c]; dzv[it] = dz[c];
+    }
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
         const size_t c = (size_t)blockIdx.x * PRE_CELLS + it * PRE_THREADS + threadIdx.x;
         if (c >= nn) break;
-        const double w = wsrc[c] - cw;
+        const double w = ws[it] - cw;
         W[c] = w;
-        cm[c] = 0.0 + rho_um * (w - w_ref[c]);
-        const double we = wev[c];
-        const double zb = zb_ref[c] + w + we;
+        cm[c] = 0.0 + rho_um * (w - wr[it]);
+        const double we = wea[it];
+        const double zb = zr[it] + w + we;
         zbed[c] = zb;
         mw = fmax(mw, fabs(w));
         mwe = fmax(mwe, fabs(we));
-        mdz = fmax(mdz, fabs(dz[c]));
+        mdz = fmax(mdz, fabs(dzv[it]));
         if (zb != zb || w != w) nanf = 1.0;
     }
     __shared__ double red[4][16];
@@ -436,9 +486,24 @@ void launch_pre_haf(const Member* d_mem, int nmem, int n, bool new_H, hipStream_
 void launch_pre_bsl(const Member* d_mem, int nmem, int n, hipStream_t st) {
     hipLaunchKernelGGL(k_pre_bsl, dim3(nmem), dim3(64), 0, st, d_mem, nmem, pre_blocks(n));
 }
+template <typename T>
+static void pre_fields_t(const Member* d_mem, int nmem, int n, int phase, bool diag, hipStream_t st) {
+    const dim3 g(pre_blocks(n), nmem), b(PRE_THREADS);
+    switch (phase) {
+    case 0:
+        if (diag) hipLaunchKernelGGL((k_pre_fields<T, PF_DIAG>), g, b, 0, st, d_mem);
+        else hipLaunchKernelGGL((k_pre_fields<T, PF_FULL>), g, b, 0, st, d_mem);
+        break;
+    case 1: hipLaunchKernelGGL((k_pre_fields<T, PF_LITHO>), g, b, 0, st, d_mem); break;
+    case 2: hipLaunchKernelGGL((k_pre_fields<T, PF_FULL>), g, b, 0, st, d_mem); break;
+    case 3: hipLaunchKernelGGL((k_pre_fields<T, PF_ALL>), g, b, 0, st, d_mem); break;
+    case 4: hipLaunchKernelGGL((k_pre_fields<T, PF_NOSSH>), g, b, 0, st, d_mem); break;
+    }
+}
+
 void launch_pre_fields(const Member* d_mem, int nmem, int n, int phase, bool diag, bool fp64, hipStream_t st) {
-    if (fp64) hipLaunchKernelGGL(k_pre_fields<double>, dim3(pre_blocks(n), nmem), dim3(PRE_THREADS), 0, st, d_mem, phase, (int)diag);
-    else hipLaunchKernelGGL(k_pre_fields<float>, dim3(pre_blocks(n), nmem), dim3(PRE_THREADS), 0, st, d_mem, phase, (int)diag);
+    if (fp64) pre_fields_t<double>(d_mem, nmem, n, phase, diag, st);
+    else pre_fields_t<float>(d_mem, nmem, n, phase, diag, st);
 }
 void launch_begin(const Member* d_mem, int nmem, double tf, int stepping, bool per_member, hipStream_t st) {
     hipLaunchKernelGGL(k_begin, dim3((nmem + 63) / 64), dim3(64), 0, st, d_mem, nmem, tf, stepping,
