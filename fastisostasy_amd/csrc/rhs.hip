@@ -6,15 +6,14 @@
 //   k_stencil      FD flexure + load -> f (lv_elva.f90:95-113)
 //   k_row_fwd      Hartley transform along x (first half of DHT2)
 //   k_col          along y: DHT, divide by kappa*R (lv_elva.f90:115), DHT;
-//                  also the partial sums that give the four corners of the
-//                  inverse transform (apply_zerobc_at_corners_dp)
+//                  also the per-column terms that give the four corners of
+//                  the inverse transform (apply_zerobc_at_corners_dp)
 //   k_row_inv      along x: DHT, /N^2, corner BC, * sec_per_year
 //                  (lv_elva.f90:118-123), then the stage sums x1/x2 and the
-//                  per-block max|x1|, max|x2|, sum (x2-x1)^2
+//                  per-row max|x1|, max|x2|, sum (x2-x1)^2
 //                  (after a rejected attempt the RHS is unchanged because it
 //                  reads now%w, fastisostasy.f90:720: only this epilogue runs)
-//                  and, in the last block to finish, the scalar tol, error
-//                  norm, accept / reject and dt control
+//   k_decide       the scalar tol, error norm, accept / reject and dt control
 //
 // Every kernel reads the domain's Ctl and returns at once if the domain is not
 // stepping, so the same launch sequence serves a batch whose members need
@@ -195,15 +194,16 @@ __global__ void __launch_bounds__(256) k_stencil_u(const Member* __restrict__ me
 }
 
 // ------------------------------------------------------------------ row pass
-// Block: 2L rows [r0, r0 + 2L); rows r0+l (real part) and r0+L+l (imaginary
-// part) share complex line l.  INV selects the inverse pass + epilogue.
+// Block: 2L rows [r0, r0 + 2L); neighbouring rows r0+2l (real part) and
+// r0+2l+1 (imaginary part) share complex line l, so a row's rounding does not
+// depend on the block size.
 template <typename T>
 __device__ __forceinline__ void load_rows(cplx<T>* buf, const T* __restrict__ src, int N, int L, int r0) {
     const int tid = threadIdx.x, nt = blockDim.x;
     T* bf = (T*)buf;
     for (int q = tid; q < 2 * L * N; q += nt) {
         const int mr = q / N, i = q % N;
-        const int l = mr % L, part = mr / L;
+        const int l = mr >> 1, part = mr & 1;
         bf[2 * (i * L + l) + part] = src[(size_t)(r0 + mr) * N + i];
     }
     __syncthreads();
@@ -224,7 +224,7 @@ __global__ void __launch_bounds__(512) k_row_fwd(const Member* __restrict__ mem,
     T* out = (T*)M.g2;
     for (int q = threadIdx.x; q < 2 * L * N; q += blockDim.x) {
         const int mr = q / N, k = q % N;
-        const int l = mr % L, part = mr / L;
+        const int l = mr >> 1, part = mr & 1;
         T ha, hb;
         dht_pair(buf[k * L + l], buf[((N - k) % N) * L + l], ha, hb);
         out[(size_t)(r0 + mr) * N + k] = part ? hb : ha;
@@ -232,7 +232,8 @@ __global__ void __launch_bounds__(512) k_row_fwd(const Member* __restrict__ mem,
 }
 
 // ------------------------------------------------------------------ column pass
-// Block: real columns [c0, c0 + 2C); columns c0+l and c0+C+l share line l.
+// Block: real columns [c0, c0 + 2C); neighbouring columns c0+2l and c0+2l+1
+// share line l (a column's rounding does not depend on the block size).
 template <typename T>
 __global__ void __launch_bounds__(512) k_col(const Member* __restrict__ mem, FftPlan plan,
                                              const void* __restrict__ twv, const double2* __restrict__ tw64,
@@ -250,7 +251,7 @@ __global__ void __launch_bounds__(512) k_col(const Member* __restrict__ mem, Fft
     T* bf = (T*)buf;
     for (int q = tid; q < 2 * C * N; q += nt) {
         const int k = q / (2 * C), mc = q % (2 * C);
-        bf[2 * (k * C + (mc % C)) + mc / C] = src[(size_t)k * N + c0 + mc];
+        bf[2 * (k * C + (mc >> 1)) + (mc & 1)] = src[(size_t)k * N + c0 + mc];
     }
     __syncthreads();
     lds_fft<T>(buf, plan, C, C, tw);
@@ -261,12 +262,12 @@ __global__ void __launch_bounds__(512) k_col(const Member* __restrict__ mem, Fft
         T ha, hb, ham, hbm;
         dht_pair(zk, zm, ha, hb);
         dht_pair(zm, zk, ham, hbm);
-        ha = kdiv(ha, kR[(size_t)k * N + c0 + l]);
-        hb = kdiv(hb, kR[(size_t)k * N + c0 + C + l]);
+        ha = kdiv(ha, kR[(size_t)k * N + c0 + 2 * l]);
+        hb = kdiv(hb, kR[(size_t)k * N + c0 + 2 * l + 1]);
         buf[k * C + l] = mk<T>(ha, hb);
         if (km != k) {
-            ham = kdiv(ham, kR[(size_t)km * N + c0 + l]);
-            hbm = kdiv(hbm, kR[(size_t)km * N + c0 + C + l]);
+            ham = kdiv(ham, kR[(size_t)km * N + c0 + 2 * l]);
+            hbm = kdiv(hbm, kR[(size_t)km * N + c0 + 2 * l + 1]);
             buf[km * C + l] = mk<T>(ham, hbm);
         }
     }
@@ -275,31 +276,30 @@ __global__ void __launch_bounds__(512) k_col(const Member* __restrict__ mem, Fft
     T* dst = (T*)M.g1;
     for (int q = tid; q < 2 * C * N; q += nt) {
         const int k = q / (2 * C), mc = q % (2 * C);
-        const int l = mc % C;
+        const int l = mc >> 1;
         T ha, hb;
         dht_pair(buf[k * C + l], buf[((N - k) % N) * C + l], ha, hb);
-        dst[(size_t)k * N + c0 + mc] = (mc / C) ? hb : ha;
-    }
-    // corner partials: rows 0 and N-1 summed with weights 1 and cas(2 pi i (N-1)/N)
-    if (tid < 4) {
-        const int k = (tid < 2) ? 0 : N - 1;
-        const bool cs = tid & 1;
-        double acc = 0.0;
-        for (int mc = 0; mc < 2 * C; ++mc) {
-            const int l = mc % C;
-            T ha, hb;
-            dht_pair(buf[k * C + l], buf[((N - k) % N) * C + l], ha, hb);
-            const double v = (double)((mc / C) ? hb : ha);
-            const int col = c0 + mc;
-            acc += cs ? v * (tw64[col].x + tw64[col].y) : v;
+        const T val = (mc & 1) ? hb : ha;
+        dst[(size_t)k * N + c0 + mc] = val;
+        // per-column terms of the four corners of the inverse transform (rows 0
+        // and N-1, weights 1 and cas(2 pi col (N-1) / N)); the row pass sums
+        // them in an order set by N alone, whatever the blocking
+        if (k == 0 || k == N - 1) {
+            const int col = c0 + mc, qc = (k == 0) ? 0 : 2;
+            const double v = (double)val;
+            M.corners[(size_t)col * 4 + qc] = v;
+            M.corners[(size_t)col * 4 + qc + 1] = v * (tw64[col].x + tw64[col].y);
         }
-        M.corners[(size_t)blockIdx.x * 4 + tid] = acc;
     }
 }
 
+// The epilogue runs one row per wave (rows wave, wave + waves, ...), each
+// lane taking columns lane, lane + 64, ...; every row leaves one error partial
+// (max(max|x1|, max|x2|), sum (x2-x1)^2) reduced by a fixed butterfly, so the
+// error norm of a domain depends on N alone, not on the blocking.
 template <typename T>
 __global__ void __launch_bounds__(512) k_row_inv(const Member* __restrict__ mem, FftPlan plan,
-                                                 const void* __restrict__ twv, int L, int ncolblk) {
+                                                 const void* __restrict__ twv, int L) {
     const Member& M = mem[blockIdx.y];
     const Ctl& c = *M.ctl;
     if (c.status != ST_RUN) return;
@@ -308,12 +308,12 @@ __global__ void __launch_bounds__(512) k_row_inv(const Member* __restrict__ mem,
     cplx<T>* buf = (cplx<T>*)smem;
     __shared__ double corner_c;
     const int N = M.n, r0 = blockIdx.x * 2 * L;
+    const int nw = blockDim.x >> 6, wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const cplx<T>* tw = (const cplx<T>*)twv;
     const double dt = attempt_dt(c, ph);
     const double* __restrict__ X = M.X[c.cur];
     double* __restrict__ Xn = M.X[c.cur ^ 1];
-    double a1 = 0.0, a2 = 0.0, s2 = 0.0;
-    auto update = [&](size_t cidx, double k) {
+    auto update = [&](size_t cidx, double k, double& a1, double& a2, double& s2) {
         double x1;
         const double x2 = stage_sums(ph.dt_method, X[cidx], k, dt, x1);
         Xn[cidx] = x2;
@@ -321,41 +321,55 @@ __global__ void __launch_bounds__(512) k_row_inv(const Member* __restrict__ mem,
         const double d = x2 - x1;
         s2 += d * d;
     };
+    auto row_done = [&](int mr, double a1, double a2, double s2) {
+        a1 = wave_max(a1); a2 = wave_max(a2); s2 = wave_sum(s2);
+        if (lane == 0) {
+            M.partials[(size_t)(r0 + mr) * 4 + 0] = fmax(a1, a2);
+            M.partials[(size_t)(r0 + mr) * 4 + 1] = s2;
+        }
+    };
     if (c.k_valid) {
         // rejected attempt before: the RHS is unchanged, only dt differs
-        for (int q = threadIdx.x; q < 2 * L * N; q += blockDim.x) {
-            const size_t cidx = (size_t)r0 * N + q;
-            update(cidx, M.dwdt[cidx]);
+        for (int mr = wv; mr < 2 * L; mr += nw) {
+            double a1 = 0.0, a2 = 0.0, s2 = 0.0;
+            for (int i = lane; i < N; i += 64) {
+                const size_t cidx = (size_t)(r0 + mr) * N + i;
+                update(cidx, M.dwdt[cidx], a1, a2, s2);
+            }
+            row_done(mr, a1, a2, s2);
         }
     } else {
-        if (threadIdx.x == 0) {
-            // four corners of the raw inverse transform, fixed summation order
+        const double S = (double)(float)((double)N * (double)N);
+        if (wv == 0) {
+            // the four corner sums over the N per-column terms of the column
+            // pass: lane-strided, then a fixed butterfly (order set by N alone)
             double v[4] = {0, 0, 0, 0};
-            for (int b = 0; b < ncolblk; ++b)
-                for (int q = 0; q < 4; ++q) v[q] += M.corners[(size_t)b * 4 + q];
-            const double S = (double)(float)((double)N * (double)N);
+            for (int col = lane; col < N; col += 64)
+                for (int q = 0; q < 4; ++q) v[q] += M.corners[(size_t)col * 4 + q];
+            for (int q = 0; q < 4; ++q) v[q] = wave_sum(v[q]);
             // x(1,1) + x(nx,ny) + x(1,ny) + x(nx,1)  (isos_utils.f90:116)
-            corner_c = 0.25 * (v[0] / S + v[3] / S + v[2] / S + v[1] / S);
+            if (lane == 0) corner_c = 0.25 * (v[0] / S + v[3] / S + v[2] / S + v[1] / S);
         }
         load_rows<T>(buf, (const T*)M.g1, N, L, r0);   // also synchronises corner_c
         lds_fft<T>(buf, plan, L, L, tw);
-        const double S = (double)(float)((double)N * (double)N);
         const double cc = corner_c, spy = ph.sec_per_year;
         const bool store_k = !(c.t + dt < c.tf);
-        for (int q = threadIdx.x; q < 2 * L * N; q += blockDim.x) {
-            const int mr = q / N, i = q % N;
-            const int l = mr % L, part = mr / L;
-            T ha, hb;
-            dht_pair(buf[i * L + l], buf[((N - i) % N) * L + l], ha, hb);
-            const double raw = (double)(part ? hb : ha);
-            const double k = (raw / S - cc) * spy;
-            const size_t cidx = (size_t)(r0 + mr) * N + i;
-            if (store_k) M.dwdt[cidx] = k;
-            update(cidx, k);
+        for (int mr = wv; mr < 2 * L; mr += nw) {
+            const int l = mr >> 1, part = mr & 1;
+            double a1 = 0.0, a2 = 0.0, s2 = 0.0;
+            for (int i = lane; i < N; i += 64) {
+                T ha, hb;
+                dht_pair(buf[i * L + l], buf[((N - i) % N) * L + l], ha, hb);
+                const double raw = (double)(part ? hb : ha);
+                const double k = (raw / S - cc) * spy;
+                const size_t cidx = (size_t)(r0 + mr) * N + i;
+                if (store_k) M.dwdt[cidx] = k;
+                update(cidx, k, a1, a2, s2);
+            }
+            row_done(mr, a1, a2, s2);
         }
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) M.ctl->pending = 1;
-    block_reduce3(a1, a2, s2, M.partials + (size_t)blockIdx.x * 4);
 }
 
 // Attempt of methods 0-2: RHS dwdt = (w_equilibrium - now%w) / tau
@@ -475,12 +489,12 @@ void launch_attempt_t(const Member* d_mem, int nmem, const RhsShape& s, hipStrea
     if (!s.fast || !launch_fast_pass(d_mem, nmem, s, st, fused ? 4 : 2))
         launch_k(2, k_col<T>, dim3(s.n / (2 * s.C), nmem), dim3(s.threads_col), lds_bytes<T>(s.n, s.C), st,
                  d_mem, s.plan, (const void*)s.tw, s.tw64, s.C);
-    // partials of the error norm: one per line pair of rows (fast pass), one per row block (generic)
+    // partials of the error norm: one per line pair of rows (fast pass), one per row (generic)
     int nblk = s.n / 2;
     if (!s.fast || !launch_fast_pass(d_mem, nmem, s, st, 3)) {
         launch_k(3, k_row_inv<T>, dim3(s.n / (2 * s.L), nmem), dim3(s.threads_row), lds_bytes<T>(s.n, s.L), st,
-                 d_mem, s.plan, (const void*)s.tw, s.L, s.n / (2 * s.C));
-        nblk = s.n / (2 * s.L);
+                 d_mem, s.plan, (const void*)s.tw, s.L);
+        nblk = s.n;
     }
     // the fused pass of the next attempt applies this attempt's decision; the
     // last one of a launch round is applied by launch_decide
@@ -488,8 +502,8 @@ void launch_attempt_t(const Member* d_mem, int nmem, const RhsShape& s, hipStrea
 }
 
 void launch_decide(const Member* d_mem, int nmem, const RhsShape& s, hipStream_t st) {
-    // partials: one per row on the power-of-two path, one per row block otherwise
-    const int nblk = s.fast ? s.n / 2 : s.n / (2 * s.L);
+    // partials: one per line pair of rows on the power-of-two path, one per row otherwise
+    const int nblk = s.fast ? s.n / 2 : s.n;
     hipLaunchKernelGGL(k_decide, dim3(nmem), dim3(64), 0, st, d_mem, nblk);
 }
 

@@ -266,16 +266,19 @@ def test_batch_shared_bsl_matches_consecutive_updates(method):
     assert sb[0].bsl_now == bb[0].bsl_now
 
 
-@pytest.mark.parametrize("prec", ["fp64", "mixed"])
-def test_results_identical_alone_and_in_batch(prec, monkeypatch):
+@pytest.mark.parametrize("prec,nx", [("fp64", 256), ("mixed", 256), ("fp64", 384), ("mixed", 384)])
+def test_results_identical_alone_and_in_batch(prec, nx, monkeypatch):
     """A domain stepped alone and the same domain stepped inside a batch of
     other domains give the same bits, whatever block shapes the two launches
-    use: the corner and error-norm reductions follow an order fixed by N."""
+    use: the corner and error-norm reductions follow an order fixed by N, on
+    the power-of-two kernels (N = 256) and on the mixed-radix ones (N = 384,
+    one line pair per block alone, 8 and 16 in the batch)."""
     target = 5
     others = [11, 12, 13]
+    print(f"alone vs batch: {prec} N={nx}", flush=True)
 
     def make(m):
-        wl = W.ensemble_member(m)
+        wl = W.ensemble_member(m, nx=nx)
         p = G.make_params(precision=prec, **wl["params"])
         h = G.IsosClass(p, wl["nx"], wl["nx"], wl["dx"], he_field=wl["he_field"], eta_field=wl["eta_field"])
         b = G.bsl_init("const", 0.0)
@@ -284,9 +287,11 @@ def test_results_identical_alone_and_in_batch(prec, monkeypatch):
 
     monkeypatch.setenv("FI_ROW_THREADS", "64")
     monkeypatch.setenv("FI_COL_LINES", "1")
+    monkeypatch.setenv("FI_GEN_LINES", "1,1")
     alone, ab, awl = make(target)
     monkeypatch.setenv("FI_ROW_THREADS", "256")
     monkeypatch.setenv("FI_COL_LINES", "16")
+    monkeypatch.setenv("FI_GEN_LINES", "8,16")
     mix = [make(m) for m in others[:2]] + [make(target)] + [make(others[2])]
     batch = G.Batch([h for h, _, _ in mix])
     for k in range(1, 6):
