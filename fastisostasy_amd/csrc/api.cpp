@@ -177,7 +177,8 @@ int make_shape(int n, int nmem, bool fp64, bool uniform, RhsShape& s) {
     // depend on the blocking or on the batch it is launched in.  LDS per block
     // stays within 64 KiB.
     const size_t cbytes = fp64 ? 16 : 8;
-    auto fits = [&](int lines) { return (size_t)n * lines * cbytes <= 65536; };
+    // data lines plus the twiddle table in LDS
+    auto fits = [&](int lines) { return (size_t)n * (lines + 1) * cbytes <= 65536; };
     if (!fits(1)) return fail(FI_E_UNSUPPORTED, "padded size N=%d too large for one LDS line", n);
     int nbmax = 0;   // butterflies of the largest stage of one line
     for (int q = 0; q < s.plan.nf; ++q) nbmax = std::max(nbmax, n / s.plan.radix[q]);

@@ -150,6 +150,18 @@ __device__ __forceinline__ void fft_stage(cplx<T>* buf, int N, int L, int P, int
     __syncthreads();
 }
 
+// Copy the n-entry twiddle table into LDS right after the `lines` data lines
+// of buf (the caller's barrier before the first stage publishes it), so the
+// stages read their twiddles from LDS instead of issuing a global load, and a
+// wait, per butterfly.
+template <typename T>
+__device__ __forceinline__ const cplx<T>* stage_twiddles(cplx<T>* buf, int n, int lines, const void* twv) {
+    cplx<T>* twl = buf + (size_t)n * lines;
+    const cplx<T>* __restrict__ g = (const cplx<T>*)twv;
+    for (int q = threadIdx.x; q < n; q += blockDim.x) twl[q] = g[q];
+    return twl;
+}
+
 // Full forward FFT of L lines (caller has synchronised after filling buf).
 template <typename T>
 __device__ __forceinline__ void lds_fft(cplx<T>* buf, const FftPlan& plan, int L, int P,

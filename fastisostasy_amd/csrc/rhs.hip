@@ -218,7 +218,7 @@ __global__ void __launch_bounds__(512) k_row_fwd(const Member* __restrict__ mem,
     extern __shared__ __align__(16) unsigned char smem[];
     cplx<T>* buf = (cplx<T>*)smem;
     const int N = M.n, r0 = blockIdx.x * 2 * L;
-    const cplx<T>* tw = (const cplx<T>*)twv;
+    const cplx<T>* tw = stage_twiddles<T>(buf, N, L, twv);
     load_rows<T>(buf, (const T*)M.g1, N, L, r0);
     lds_fft<T>(buf, plan, L, L, tw);
     T* out = (T*)M.g2;
@@ -245,7 +245,7 @@ __global__ void __launch_bounds__(512) k_col(const Member* __restrict__ mem, Fft
     cplx<T>* buf = (cplx<T>*)smem;
     const int N = M.n, c0 = blockIdx.x * 2 * C;
     const int tid = threadIdx.x, nt = blockDim.x;
-    const cplx<T>* tw = (const cplx<T>*)twv;
+    const cplx<T>* tw = stage_twiddles<T>(buf, N, C, twv);
     const T* __restrict__ src = (const T*)M.g2;
     const T* __restrict__ kR = (const T*)M.kR;
     T* bf = (T*)buf;
@@ -309,7 +309,6 @@ __global__ void __launch_bounds__(512) k_row_inv(const Member* __restrict__ mem,
     __shared__ double corner_c;
     const int N = M.n, r0 = blockIdx.x * 2 * L;
     const int nw = blockDim.x >> 6, wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const cplx<T>* tw = (const cplx<T>*)twv;
     const double dt = attempt_dt(c, ph);
     const double* __restrict__ X = M.X[c.cur];
     double* __restrict__ Xn = M.X[c.cur ^ 1];
@@ -350,7 +349,8 @@ __global__ void __launch_bounds__(512) k_row_inv(const Member* __restrict__ mem,
             // x(1,1) + x(nx,ny) + x(1,ny) + x(nx,1)  (isos_utils.f90:116)
             if (lane == 0) corner_c = 0.25 * (v[0] / S + v[3] / S + v[2] / S + v[1] / S);
         }
-        load_rows<T>(buf, (const T*)M.g1, N, L, r0);   // also synchronises corner_c
+        const cplx<T>* tw = stage_twiddles<T>(buf, N, L, twv);
+        load_rows<T>(buf, (const T*)M.g1, N, L, r0);   // also synchronises corner_c and tw
         lds_fft<T>(buf, plan, L, L, tw);
         const double cc = corner_c, spy = ph.sec_per_year;
         const bool store_k = !(c.t + dt < c.tf);
@@ -414,7 +414,7 @@ __global__ void __launch_bounds__(64) k_decide(const Member* __restrict__ mem, i
 
 // ------------------------------------------------------------------ launchers
 template <typename T>
-static size_t lds_bytes(int n, int lines) { return (size_t)n * lines * sizeof(cplx<T>); }
+static size_t lds_bytes(int n, int lines) { return (size_t)n * (lines + 1) * sizeof(cplx<T>); }   // + twiddles
 
 // Optional HIP-event timing of one kernel of the attempt sequence (for
 // benchmarks): events are recorded on the launch stream around every launch
