@@ -150,6 +150,28 @@ __device__ __forceinline__ void fft_stage(cplx<T>* buf, int N, int L, int P, int
     __syncthreads();
 }
 
+// Copy `total` elements, element q read by ld(q) and written by st(q, v), with
+// U loads in flight per thread ahead of their stores: a plain load-store loop
+// waits for every load in turn.  Loads are unconditional (indices past the end
+// re-read element `base`), so no branch splits the wait counts.
+template <int U, class Ld, class St>
+__device__ __forceinline__ void staged_copy(int total, Ld ld, St st) {
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int base = tid; base < total; base += U * nt) {
+        decltype(ld(0)) v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int q = base + u * nt;
+            v[u] = ld(q < total ? q : base);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int q = base + u * nt;
+            if (q < total) st(q, v[u]);
+        }
+    }
+}
+
 // Copy the n-entry twiddle table into LDS right after the `lines` data lines
 // of buf (the caller's barrier before the first stage publishes it), so the
 // stages read their twiddles from LDS instead of issuing a global load, and a

@@ -199,13 +199,12 @@ __global__ void __launch_bounds__(256) k_stencil_u(const Member* __restrict__ me
 // depend on the block size.
 template <typename T>
 __device__ __forceinline__ void load_rows(cplx<T>* buf, const T* __restrict__ src, int N, int L, int r0) {
-    const int tid = threadIdx.x, nt = blockDim.x;
     T* bf = (T*)buf;
-    for (int q = tid; q < 2 * L * N; q += nt) {
-        const int mr = q / N, i = q % N;
-        const int l = mr >> 1, part = mr & 1;
-        bf[2 * (i * L + l) + part] = src[(size_t)(r0 + mr) * N + i];
-    }
+    staged_copy<8>(2 * L * N, [&](int q) { return src[(size_t)r0 * N + q]; },
+                   [&](int q, T v) {
+                       const int mr = q / N, i = q % N;
+                       bf[2 * (i * L + (mr >> 1)) + (mr & 1)] = v;
+                   });
     __syncthreads();
 }
 
@@ -249,10 +248,11 @@ __global__ void __launch_bounds__(512) k_col(const Member* __restrict__ mem, Fft
     const T* __restrict__ src = (const T*)M.g2;
     const T* __restrict__ kR = (const T*)M.kR;
     T* bf = (T*)buf;
-    for (int q = tid; q < 2 * C * N; q += nt) {
-        const int k = q / (2 * C), mc = q % (2 * C);
-        bf[2 * (k * C + (mc >> 1)) + (mc & 1)] = src[(size_t)k * N + c0 + mc];
-    }
+    staged_copy<8>(2 * C * N, [&](int q) { return src[(size_t)(q / (2 * C)) * N + c0 + q % (2 * C)]; },
+                   [&](int q, T v) {
+                       const int k = q / (2 * C), mc = q % (2 * C);
+                       bf[2 * (k * C + (mc >> 1)) + (mc & 1)] = v;
+                   });
     __syncthreads();
     lds_fft<T>(buf, plan, C, C, tw);
     // Hartley values, spectral divide, re-pack (lv_elva.f90:115)
@@ -312,9 +312,9 @@ __global__ void __launch_bounds__(512) k_row_inv(const Member* __restrict__ mem,
     const double dt = attempt_dt(c, ph);
     const double* __restrict__ X = M.X[c.cur];
     double* __restrict__ Xn = M.X[c.cur ^ 1];
-    auto update = [&](size_t cidx, double k, double& a1, double& a2, double& s2) {
+    auto update = [&](size_t cidx, double x, double k, double& a1, double& a2, double& s2) {
         double x1;
-        const double x2 = stage_sums(ph.dt_method, X[cidx], k, dt, x1);
+        const double x2 = stage_sums(ph.dt_method, x, k, dt, x1);
         Xn[cidx] = x2;
         a1 = fmax(a1, fabs(x1)); a2 = fmax(a2, fabs(x2));
         const double d = x2 - x1;
@@ -327,13 +327,25 @@ __global__ void __launch_bounds__(512) k_row_inv(const Member* __restrict__ mem,
             M.partials[(size_t)(r0 + mr) * 4 + 1] = s2;
         }
     };
+    // columns of a row in chunks of EU per lane, the chunk's loads first (the
+    // stores of the previous chunk would otherwise enter every wait)
+    constexpr int EU = 4;
     if (c.k_valid) {
         // rejected attempt before: the RHS is unchanged, only dt differs
         for (int mr = wv; mr < 2 * L; mr += nw) {
+            const size_t rb = (size_t)(r0 + mr) * N;
             double a1 = 0.0, a2 = 0.0, s2 = 0.0;
-            for (int i = lane; i < N; i += 64) {
-                const size_t cidx = (size_t)(r0 + mr) * N + i;
-                update(cidx, M.dwdt[cidx], a1, a2, s2);
+            for (int i0 = lane; i0 < N; i0 += 64 * EU) {
+                double xv[EU], kv[EU];
+#pragma unroll
+                for (int u = 0; u < EU; ++u) {
+                    const int i = i0 + 64 * u < N ? i0 + 64 * u : i0;
+                    xv[u] = X[rb + i];
+                    kv[u] = M.dwdt[rb + i];
+                }
+#pragma unroll
+                for (int u = 0; u < EU; ++u)
+                    if (i0 + 64 * u < N) update(rb + i0 + 64 * u, xv[u], kv[u], a1, a2, s2);
             }
             row_done(mr, a1, a2, s2);
         }
@@ -356,15 +368,23 @@ __global__ void __launch_bounds__(512) k_row_inv(const Member* __restrict__ mem,
         const bool store_k = !(c.t + dt < c.tf);
         for (int mr = wv; mr < 2 * L; mr += nw) {
             const int l = mr >> 1, part = mr & 1;
+            const size_t rb = (size_t)(r0 + mr) * N;
             double a1 = 0.0, a2 = 0.0, s2 = 0.0;
-            for (int i = lane; i < N; i += 64) {
-                T ha, hb;
-                dht_pair(buf[i * L + l], buf[((N - i) % N) * L + l], ha, hb);
-                const double raw = (double)(part ? hb : ha);
-                const double k = (raw / S - cc) * spy;
-                const size_t cidx = (size_t)(r0 + mr) * N + i;
-                if (store_k) M.dwdt[cidx] = k;
-                update(cidx, k, a1, a2, s2);
+            for (int i0 = lane; i0 < N; i0 += 64 * EU) {
+                double xv[EU];
+#pragma unroll
+                for (int u = 0; u < EU; ++u) xv[u] = X[rb + (i0 + 64 * u < N ? i0 + 64 * u : i0)];
+#pragma unroll
+                for (int u = 0; u < EU; ++u) {
+                    const int i = i0 + 64 * u;
+                    if (i >= N) continue;
+                    T ha, hb;
+                    dht_pair(buf[i * L + l], buf[((N - i) % N) * L + l], ha, hb);
+                    const double raw = (double)(part ? hb : ha);
+                    const double k = (raw / S - cc) * spy;
+                    if (store_k) M.dwdt[rb + i] = k;
+                    update(rb + i, xv[u], k, a1, a2, s2);
+                }
             }
             row_done(mr, a1, a2, s2);
         }
