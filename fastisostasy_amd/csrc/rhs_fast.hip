@@ -548,7 +548,11 @@ __device__ __forceinline__ void row_reduce_store(double a1a, double a2a, double 
 template <typename T, int N, int THREADS, int TILES, int METH>
 __global__ void __launch_bounds__(THREADS) kf_row_inv(const Member* __restrict__ mem, const void* __restrict__ twv,
                                                       int ncolblk) {
-    const Member& M = mem[blockIdx.y];
+    // XCD-aware order: a domain's row blocks share an XCD, so its corner terms
+    // and twiddles come from one L2 instead of eight
+    int mi, rblk;
+    xcd_order(mi, rblk);
+    const Member& M = mem[mi];
     const kptr<Ctl> c = kview(M.ctl);
     if ((c->status != ST_RUN) | (c->skip != 0)) return;
     const kptr<PhysDev> ph = kview(M.phys);
@@ -560,7 +564,7 @@ __global__ void __launch_bounds__(THREADS) kf_row_inv(const Member* __restrict__
     const int tid = threadIdx.x, line = tid / TPL, t = tid % TPL;
     int ra[TILES];
 #pragma unroll
-    for (int k = 0; k < TILES; ++k) ra[k] = (blockIdx.x * TILES + k) * 2 * L + 2 * line;   // rb = ra + 1
+    for (int k = 0; k < TILES; ++k) ra[k] = (rblk * TILES + k) * 2 * L + 2 * line;   // rb = ra + 1
     const int meth = ph->dt_method;
     const double dt = attempt_dt(meth, c->dt, c->t, c->tf, ph->dt_max);
     const int cur = c->cur;
@@ -568,7 +572,7 @@ __global__ void __launch_bounds__(THREADS) kf_row_inv(const Member* __restrict__
     // block 0 marks the decision pending (the step control reads this attempt
     // next) after its last load: a one-lane store ahead of the loads would
     // make every later wait a full one
-    const bool mark = blockIdx.x == 0 && tid == 0;
+    const bool mark = rblk == 0 && tid == 0;
     gptr<double> X = gin<double>(M.X[cur]);
     gmut<double> Xn = gout<double>(M.X[cur ^ 1]);
     gmut<double> dw = gout<double>(M.dwdt);
