@@ -501,127 +501,161 @@ int launch_attempts(const Member* d, int nmem, const RhsShape& s, hipStream_t st
 // function of the call times only.
 bool diag_due(const fi_isos* h, double time) { return time - h->time_diag >= (double)1e-3f; }
 
-// One isos_update for a group of members that share N, precision and options.
-// All members advance to `time`, or member m to times[m] when `times` is given
-// (the Member copies on the device then carry the targets in tf_req).
-int run_update(std::vector<fi_isos*>& hs, Member* d_mem, const RhsShape& shape, SumBuf& sb, double time,
-               bool stepping, const double* times = nullptr, bool graphs = true) {
-    const int nmem = (int)hs.size();
-    const fi_isos& h0 = *hs[0];
+// One isos_update for a group of members that share N, precision and options,
+// as a job bound to one stream: job_begin (the pre-update kernels), rounds of
+// attempts each ending in the finalize and an asynchronous readback of the
+// per-domain summaries (job_round), the host's look at the summaries
+// (job_more), job_end (the instability checks).  All members advance to
+// `time`, or member m to times[m] when `times` is given (the Member copies on
+// the device then carry the targets in tf_req).  run_update drives one job on
+// the library stream; the request server drives several on their own streams.
+struct UpdateJob {
+    std::vector<fi_isos*> hs;
+    Member* d_mem = nullptr;
+    const RhsShape* shape = nullptr;
+    SumBuf sb;
+    const double* times = nullptr;
+    double time = 0.0;
+    bool stepping = true, graphs = true;
+    hipStream_t st = nullptr;
+    int K = 1, ahead = 1;
+    long long total = 0;
+    int sb_lo[MAX_SUB + 1] = {};
+    const RhsShape* sshape[MAX_SUB] = {};
+    double t_of(int m) const { return times ? times[m] : time; }
+};
+
+int job_begin(UpdateJob& j) {
+    const int nmem = (int)j.hs.size();
+    const fi_isos& h0 = *j.hs[0];
     const int n = h0.n;
-    auto t_of = [&](int m) { return times ? times[m] : time; };
-    const bool diag = diag_due(hs[0], t_of(0));
+    const RhsShape& shape = *j.shape;
+    Member* d_mem = j.d_mem;
+    hipStream_t st = j.st;
+    const bool diag = diag_due(j.hs[0], j.t_of(0));
     for (int m = 1; m < nmem; ++m)
-        if (diag_due(hs[m], t_of(m)) != diag) return fail(FI_E_STATE, "batch members disagree on the diagnostics cadence");
+        if (diag_due(j.hs[m], j.t_of(m)) != diag) return fail(FI_E_STATE, "batch members disagree on the diagnostics cadence");
     if (diag)
-        for (fi_isos* h : hs) h->time_diag = h->time_diag + h->par.dt_diagnostics;
-    launch_pre_haf(d_mem, nmem, n, true, g_stream);
-    launch_pre_bsl(d_mem, nmem, n, g_stream);
+        for (fi_isos* h : j.hs) h->time_diag = h->time_diag + h->par.dt_diagnostics;
+    launch_pre_haf(d_mem, nmem, n, true, st);
+    launch_pre_bsl(d_mem, nmem, n, st);
     if (!diag) {
-        launch_pre_fields(d_mem, nmem, n, 0, false, h0.fp64, g_stream);
+        launch_pre_fields(d_mem, nmem, n, 0, false, h0.fp64, st);
     } else if (!h0.par.include_elastic && !h0.par.heterogeneous_ssh) {
-        launch_pre_fields(d_mem, nmem, n, 3, true, h0.fp64, g_stream);   // phases 0-2 in one pass
+        launch_pre_fields(d_mem, nmem, n, 3, true, h0.fp64, st);   // phases 0-2 in one pass
     } else {
-        launch_pre_fields(d_mem, nmem, n, 0, true, h0.fp64, g_stream);
-        if (h0.par.include_elastic) launch_convolution(d_mem, nmem, conv_shape_of(&h0), 0, g_stream);
+        launch_pre_fields(d_mem, nmem, n, 0, true, h0.fp64, st);
+        if (h0.par.include_elastic) launch_convolution(d_mem, nmem, conv_shape_of(&h0), 0, st);
         if (h0.par.heterogeneous_ssh) {
-            launch_pre_fields(d_mem, nmem, n, 1, true, h0.fp64, g_stream);
-            launch_convolution(d_mem, nmem, conv_shape_of(&h0), 1, g_stream);
-            launch_pre_fields(d_mem, nmem, n, 2, true, h0.fp64, g_stream);
+            launch_pre_fields(d_mem, nmem, n, 1, true, h0.fp64, st);
+            launch_convolution(d_mem, nmem, conv_shape_of(&h0), 1, st);
+            launch_pre_fields(d_mem, nmem, n, 2, true, h0.fp64, st);
         } else {
-            launch_pre_fields(d_mem, nmem, n, 4, true, h0.fp64, g_stream);   // phases 1-2 in one pass
+            launch_pre_fields(d_mem, nmem, n, 4, true, h0.fp64, st);   // phases 1-2 in one pass
         }
     }
-    launch_begin(d_mem, nmem, time, stepping ? 1 : 0, times != nullptr, g_stream);
-    if (stepping && shape.method != 3) {
+    launch_begin(d_mem, nmem, j.time, j.stepping ? 1 : 0, j.times != nullptr, st);
+    if (j.stepping && shape.method != 3) {
         if (shape.method == 2) {
-            launch_weq_input(d_mem, nmem, n, g_stream);
-            launch_convolution(d_mem, nmem, conv_shape_of(&h0), 2, g_stream);
+            launch_weq_input(d_mem, nmem, n, st);
+            launch_convolution(d_mem, nmem, conv_shape_of(&h0), 2, st);
         }
-        launch_weq(d_mem, nmem, n, shape.method, g_stream);
+        launch_weq(d_mem, nmem, n, shape.method, st);
     }
     HIPCHK(hipGetLastError());
-    if (stepping) {
-        // Attempts until every member reached `time`.  Members that are done
-        // skip the launches on the device, so the host may launch ahead: after
-        // each readback it launches as many attempts as the slowest member
-        // still needs at its current dt (at least one).
-        // first round: as many attempts as the slowest member needs at the dt it
-        // ended its previous update with, plus two for rejections (an extra
-        // attempt costs a few microseconds of gated no-op kernels, a second
-        // round a host round trip); the loop below corrects the estimate
-        int ahead = 1;
-        {
-            double need = 0.0;
-            for (int m = 0; m < nmem; ++m)
-                if (hs[m]->last_dt > 0.0) need = std::max(need, std::ceil((t_of(m) - hs[m]->last_t) / hs[m]->last_dt));
-            if (need > 0.0) need += 2.0;
-            ahead = (int)std::min(256.0, std::max(1.0, need));
-        }
-        long long total = 0;
-        const int K = shape.method == 3 ? sub_count(nmem) : 1;
-        int sb_lo[MAX_SUB + 1];
-        const RhsShape* sshape[MAX_SUB] = {};
-        if (K > 1) {
-            int rc = ensure_sub(K);
+    if (!j.stepping) return 0;
+    // Attempts until every member reached `time`.  Members that are done skip
+    // the launches on the device, so the host may launch ahead: the first
+    // round launches as many attempts as the slowest member needs at the dt it
+    // ended its previous update with, plus two for rejections (an extra attempt
+    // costs a few microseconds of gated no-op kernels, a second round a host
+    // round trip); after each readback job_more corrects the estimate.
+    double need = 0.0;
+    for (int m = 0; m < nmem; ++m)
+        if (j.hs[m]->last_dt > 0.0) need = std::max(need, std::ceil((j.t_of(m) - j.hs[m]->last_t) / j.hs[m]->last_dt));
+    if (need > 0.0) need += 2.0;
+    j.ahead = (int)std::min(256.0, std::max(1.0, need));
+    // concurrent sub-batches only on the library stream (their streams are shared)
+    j.K = shape.method == 3 && j.st == g_stream ? sub_count(nmem) : 1;
+    if (j.K > 1) {
+        int rc = ensure_sub(j.K);
+        if (rc) return rc;
+        for (int k = 0; k <= j.K; ++k) j.sb_lo[k] = (int)((long long)nmem * k / j.K);
+        for (int k = 0; k < j.K; ++k) {
+            rc = sub_shape(n, j.sb_lo[k + 1] - j.sb_lo[k], h0.fp64, h0.uniform, shape.dt_method, j.sshape[k]);
             if (rc) return rc;
-            for (int k = 0; k <= K; ++k) sb_lo[k] = (int)((long long)nmem * k / K);
-            for (int k = 0; k < K; ++k) {
-                rc = sub_shape(n, sb_lo[k + 1] - sb_lo[k], h0.fp64, h0.uniform, shape.dt_method, sshape[k]);
-                if (rc) return rc;
-            }
         }
-        for (;;) {
-            if (K == 1) {
-                int rc = launch_attempts(d_mem, nmem, shape, g_stream, ahead, graphs);
-                if (rc) return rc;
-                launch_decide(d_mem, nmem, shape, g_stream);
-            } else {
-                HIPCHK(hipEventRecord(g_sub.start, g_stream));
-                for (int k = 0; k < K; ++k) HIPCHK(hipStreamWaitEvent(g_sub.s[k], g_sub.start, 0));
-                // interleave the sub-batches' launches so both queues fill together
-                for (int a = 0; a < ahead; a += GRAPH_REPS)
-                    for (int k = 0; k < K; ++k) {
-                        int rc = launch_attempts(d_mem + sb_lo[k], sb_lo[k + 1] - sb_lo[k], *sshape[k], g_sub.s[k],
-                                                 std::min(GRAPH_REPS, ahead - a), graphs);
-                        if (rc) return rc;
-                    }
-                for (int k = 0; k < K; ++k) {
-                    launch_decide(d_mem + sb_lo[k], sb_lo[k + 1] - sb_lo[k], *sshape[k], g_sub.s[k]);
-                    HIPCHK(hipEventRecord(g_sub.done[k], g_sub.s[k]));
-                    HIPCHK(hipStreamWaitEvent(g_stream, g_sub.done[k], 0));
-                }
-            }
-            // finalize the members that are done (gated per member on the
-            // device) before the readback: usually no second round follows
-            launch_finalize(d_mem, nmem, n, g_stream);
-            HIPCHK(hipGetLastError());
-            total += ahead;
-            int rc = gather(d_mem, nmem, sb);
-            if (rc) return rc;
-            double need = 0.0;
-            bool running = false;
-            for (int m = 0; m < nmem; ++m) {
-                const CtlSummary& c = sb.h[m];
-                if (c.status != ST_RUN) continue;
-                running = true;
-                need = std::max(need, std::ceil((c.tf - c.t) / std::max(c.dt, 1e-300)));
-            }
-            if (!running) break;
-            ahead = (int)std::min(256.0, std::max(1.0, need + 1.0));
-            if (total > 100000000LL) return fail(FI_E_STATE, "integrator did not reach the target time");
-        }
-    } else {
-        launch_finalize(d_mem, nmem, n, g_stream);
-        HIPCHK(hipGetLastError());
-        int rcg = gather(d_mem, nmem, sb);
-        if (rcg) return rcg;
     }
-    for (int m = 0; m < nmem; ++m) {
-        const CtlSummary& c = sb.h[m];
-        hs[m]->time_prog = t_of(m);
-        hs[m]->last_t = c.t;
-        hs[m]->last_dt = stepping ? c.dt : 0.0;
+    return 0;
+}
+
+// One round: the attempts (when stepping) and the decision of the last one,
+// then the finalize of the members that are done (gated per member on the
+// device, so that usually no second round follows) and the asynchronous
+// readback of the summaries, all on the job's stream.
+int job_round(UpdateJob& j) {
+    const int nmem = (int)j.hs.size();
+    const int n = j.hs[0]->n;
+    const RhsShape& shape = *j.shape;
+    Member* d_mem = j.d_mem;
+    if (j.stepping) {
+        if (j.K == 1) {
+            int rc = launch_attempts(d_mem, nmem, shape, j.st, j.ahead, j.graphs);
+            if (rc) return rc;
+            launch_decide(d_mem, nmem, shape, j.st);
+        } else {
+            HIPCHK(hipEventRecord(g_sub.start, j.st));
+            for (int k = 0; k < j.K; ++k) HIPCHK(hipStreamWaitEvent(g_sub.s[k], g_sub.start, 0));
+            // interleave the sub-batches' launches so both queues fill together
+            for (int a = 0; a < j.ahead; a += GRAPH_REPS)
+                for (int k = 0; k < j.K; ++k) {
+                    int rc = launch_attempts(d_mem + j.sb_lo[k], j.sb_lo[k + 1] - j.sb_lo[k], *j.sshape[k], g_sub.s[k],
+                                             std::min(GRAPH_REPS, j.ahead - a), j.graphs);
+                    if (rc) return rc;
+                }
+            for (int k = 0; k < j.K; ++k) {
+                launch_decide(d_mem + j.sb_lo[k], j.sb_lo[k + 1] - j.sb_lo[k], *j.sshape[k], g_sub.s[k]);
+                HIPCHK(hipEventRecord(g_sub.done[k], g_sub.s[k]));
+                HIPCHK(hipStreamWaitEvent(j.st, g_sub.done[k], 0));
+            }
+        }
+        j.total += j.ahead;
+    }
+    launch_finalize(d_mem, nmem, n, j.st);
+    HIPCHK(hipGetLastError());
+    launch_gather(d_mem, nmem, j.sb.d, j.st);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(j.sb.h, j.sb.d, sizeof(CtlSummary) * nmem, hipMemcpyDeviceToHost, j.st));
+    return 0;
+}
+
+// After a round's readback has completed: whether another round is needed
+// (as many attempts as the slowest running member needs at its current dt).
+int job_more(UpdateJob& j, bool& more) {
+    more = false;
+    if (!j.stepping) return 0;
+    double need = 0.0;
+    bool running = false;
+    for (int m = 0; m < (int)j.hs.size(); ++m) {
+        const CtlSummary& c = j.sb.h[m];
+        if (c.status != ST_RUN) continue;
+        running = true;
+        need = std::max(need, std::ceil((c.tf - c.t) / std::max(c.dt, 1e-300)));
+    }
+    if (!running) return 0;
+    j.ahead = (int)std::min(256.0, std::max(1.0, need + 1.0));
+    if (j.total > 100000000LL) return fail(FI_E_STATE, "integrator did not reach the target time");
+    more = true;
+    return 0;
+}
+
+int job_end(UpdateJob& j) {
+    for (int m = 0; m < (int)j.hs.size(); ++m) {
+        const CtlSummary& c = j.sb.h[m];
+        fi_isos* h = j.hs[m];
+        h->time_prog = j.t_of(m);
+        h->last_t = c.t;
+        h->last_dt = j.stepping ? c.dt : 0.0;
         // check_isos_instabilities + NaN check (fastisostasy.f90:860-869)
         if (c.maxw > 1e5) return fail(FI_E_UNSTABLE, "large_check:: Error: Large values detected in array w");
         if (c.maxwe > 1e5) return fail(FI_E_UNSTABLE, "large_check:: Error: Large values detected in array we");
@@ -629,6 +663,32 @@ int run_update(std::vector<fi_isos*>& hs, Member* d_mem, const RhsShape& shape, 
         if (c.nonfinite) return fail(FI_E_NAN, "isos_update:: Error: NaNs detected.");
     }
     return 0;
+}
+
+int run_update(std::vector<fi_isos*>& hs, Member* d_mem, const RhsShape& shape, SumBuf& sb, double time,
+               bool stepping, const double* times = nullptr, bool graphs = true) {
+    UpdateJob j;
+    j.hs = hs;
+    j.d_mem = d_mem;
+    j.shape = &shape;
+    j.sb = sb;
+    j.times = times;
+    j.time = time;
+    j.stepping = stepping;
+    j.graphs = graphs;
+    j.st = g_stream;
+    int rc = job_begin(j);
+    if (rc) return rc;
+    for (;;) {
+        rc = job_round(j);
+        if (rc) return rc;
+        HIPCHK(hipStreamSynchronize(j.st));
+        bool more = false;
+        rc = job_more(j, more);
+        if (rc) return rc;
+        if (!more) break;
+    }
+    return job_end(j);
 }
 
 int bind_bsl(fi_isos* h, fi_bsl* b) {
@@ -1553,23 +1613,36 @@ struct fi_server {
     Member* h_stage = nullptr;   // pinned staging of the same
     int cap = 0;                 // capacity of d_mem / h_stage / sb
     SumBuf sb;
-    std::map<std::tuple<int, int, int, int, int>, RhsShape> shapes;  // (n, fp64, uniform, nmem, method)
+    // (n, fp64, uniform, nmem, method, dt_method)
+    std::map<std::tuple<int, int, int, int, int, int>, RhsShape> shapes;
     long long rounds = 0, groups = 0, served = 0;
+    // groups of one round run concurrently, one stream each (default priority)
+    static constexpr int NSTREAM = 8;
+    hipStream_t st[NSTREAM] = {};
+    std::vector<hipEvent_t> ev;   // one per concurrent group: its last readback
+    hipEvent_t start = nullptr;   // library-stream work queued before the round
 };
 
 namespace {
 
+// every stream the server launches on, drained
+void server_drain(fi_server* s) {
+    (void)hipStreamSynchronize(g_stream);
+    for (hipStream_t t : s->st)
+        if (t) (void)hipStreamSynchronize(t);
+}
+
 int server_reserve(fi_server* s, int n) {
     if (n <= s->cap) return 0;
-    (void)hipStreamSynchronize(g_stream);
+    server_drain(s);
     purge_graphs(s->d_mem, s->d_mem + s->cap);
     if (s->d_mem) (void)hipFree(s->d_mem);
     if (s->h_stage) (void)hipHostFree(s->h_stage);
     sum_free(s->sb);
+    const int c = std::max(n, 2 * s->cap);
     s->d_mem = nullptr;
     s->h_stage = nullptr;
     s->cap = 0;
-    const int c = std::max(n, 2 * s->cap);
     HIPCHK(hipMalloc(&s->d_mem, sizeof(Member) * c));
     HIPCHK(hipHostMalloc(&s->h_stage, sizeof(Member) * c, hipHostMallocDefault));
     int rc = sum_alloc(s->sb, c);
@@ -1666,8 +1739,37 @@ int fi_server_run(fi_server* s, int* completed) {
     }
     int rc = server_reserve(s, (int)cand.size());
     if (rc) return rc;
+    // Groups run concurrently, each on its own stream, when the server batches
+    // (max_batch > 1) and no bsl object is shared by two groups (domains
+    // sharing one are updated in batch order, which needs one stream).
+    bool concurrent = s->max_batch > 1 && grp.size() > 1;
+    if (concurrent) {
+        std::map<const fi_bsl*, size_t> owner;
+        for (size_t g = 0; g < grp.size() && concurrent; ++g)
+            for (int i : grp[g]) {
+                auto it = owner.emplace(s->slots[i].b, g).first;
+                if (it->second != g) { concurrent = false; break; }
+            }
+    }
+    if (concurrent) {
+        for (int k = 0; k < fi_server::NSTREAM; ++k)
+            if (!s->st[k]) HIPCHK(hipStreamCreateWithFlags(&s->st[k], hipStreamNonBlocking));
+        while (s->ev.size() < grp.size()) {
+            hipEvent_t e = nullptr;
+            HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            s->ev.push_back(e);
+        }
+        // the round starts after everything already queued on the library stream
+        if (!s->start) HIPCHK(hipEventCreateWithFlags(&s->start, hipEventDisableTiming));
+        HIPCHK(hipEventRecord(s->start, g_stream));
+        for (hipStream_t t : s->st) HIPCHK(hipStreamWaitEvent(t, s->start, 0));
+    }
+    std::vector<UpdateJob> jobs(grp.size());
+    std::vector<std::vector<double>> tvec(grp.size());
     int base = 0;
-    for (const auto& g : grp) {
+    for (size_t gi = 0; gi < grp.size(); ++gi) {
+        const auto& g = grp[gi];
+        hipStream_t gst = concurrent ? s->st[gi % fi_server::NSTREAM] : g_stream;
         const int nm = (int)g.size();
         std::vector<fi_isos*> hs;
         std::vector<double> times;
@@ -1677,7 +1779,7 @@ int fi_server_run(fi_server* s, int* completed) {
             const fi_server::Req& r = sl.q.front();
             if (!r.H.empty())
                 HIPCHK(hipMemcpyAsync(h->hm.H_in, r.H.data(), sizeof(double) * r.H.size(), hipMemcpyHostToDevice,
-                                      g_stream));
+                                      gst));
             Member M = h->hm;
             M.bsl = sl.b->d;
             M.tf_req = r.time;
@@ -1695,9 +1797,9 @@ int fi_server_run(fi_server* s, int* completed) {
         // the staged copies of earlier groups are still being read by queued copies;
         // each group uses its own stage range, so no wait is needed
         HIPCHK(hipMemcpyAsync(s->d_mem + base, s->h_stage + base, sizeof(Member) * nm, hipMemcpyHostToDevice,
-                              g_stream));
+                              gst));
         const fi_isos* h0 = hs[0];
-        auto sk = std::make_tuple(h0->n, (int)h0->fp64, (int)h0->uniform, nm, h0->par.method);
+        auto sk = std::make_tuple(h0->n, (int)h0->fp64, (int)h0->uniform, nm, h0->par.method, h0->par.dt_method);
         auto it = s->shapes.find(sk);
         if (it == s->shapes.end()) {
             RhsShape shp{};
@@ -1707,16 +1809,60 @@ int fi_server_run(fi_server* s, int* completed) {
             shp.dt_method = h0->par.dt_method;
             it = s->shapes.emplace(sk, shp).first;
         }
-        // groupings change from round to round: plain launches, no graph captures
-        rc = run_update(hs, s->d_mem + base, it->second, s->sb, 0.0, true, times.data(), false);
-        if (rc) return rc;
-        for (int q = 0; q < nm; ++q) {
+        if (!concurrent) {
+            // groupings change from round to round: plain launches, no graph captures
+            SumBuf sb{s->sb.d + base, s->sb.h + base, nm};
+            rc = run_update(hs, s->d_mem + base, it->second, sb, 0.0, true, times.data(), false);
+            if (rc) return rc;
+        } else {
+            tvec[gi] = times;
+            UpdateJob& j = jobs[gi];
+            j.hs = hs;
+            j.d_mem = s->d_mem + base;
+            j.shape = &it->second;
+            j.sb = SumBuf{s->sb.d + base, s->sb.h + base, nm};
+            j.times = tvec[gi].data();
+            j.stepping = true;
+            j.graphs = false;
+            j.st = gst;
+            rc = job_begin(j);
+            if (!rc) rc = job_round(j);
+            if (rc) { server_drain(s); return rc; }
+            HIPCHK(hipEventRecord(s->ev[gi], gst));
+        }
+        base += nm;
+    }
+    if (concurrent) {
+        // rounds of every group until all are done; a group that needs another
+        // round launches it while the others' work is still on the GPU
+        std::vector<char> fin(grp.size(), 0);
+        size_t left = grp.size();
+        while (left > 0) {
+            for (size_t gi = 0; gi < grp.size(); ++gi) {
+                if (fin[gi]) continue;
+                HIPCHK(hipEventSynchronize(s->ev[gi]));
+                bool more = false;
+                rc = job_more(jobs[gi], more);
+                if (!rc && more) {
+                    rc = job_round(jobs[gi]);
+                    if (!rc) HIPCHK(hipEventRecord(s->ev[gi], jobs[gi].st));
+                }
+                if (rc) { server_drain(s); return rc; }
+                if (!more) { fin[gi] = 1; --left; }
+            }
+        }
+        for (auto& j : jobs) {
+            rc = job_end(j);
+            if (rc) return rc;
+        }
+    }
+    for (const auto& g : grp) {
+        for (int q = 0; q < (int)g.size(); ++q) {
             fi_server::Slot& sl = s->slots[g[q]];
             s->done.push_back(sl.q.front().tag);
             sl.q.pop_front();
         }
-        *completed += nm;
-        base += nm;
+        *completed += (int)g.size();
         ++s->groups;
     }
     ++s->rounds;
@@ -1744,11 +1890,15 @@ int fi_server_stats(fi_server* s, long long* rounds, long long* groups, long lon
 
 int fi_server_destroy(fi_server* s) {
     if (!s) return 0;
-    (void)hipStreamSynchronize(g_stream);
+    server_drain(s);
     purge_graphs(s->d_mem, s->d_mem + s->cap);
     if (s->d_mem) (void)hipFree(s->d_mem);
     if (s->h_stage) (void)hipHostFree(s->h_stage);
     sum_free(s->sb);
+    for (hipStream_t t : s->st)
+        if (t) (void)hipStreamDestroy(t);
+    for (hipEvent_t e : s->ev) (void)hipEventDestroy(e);
+    if (s->start) (void)hipEventDestroy(s->start);
     delete s;
     return 0;
 }

@@ -14,9 +14,9 @@ pytestmark = pytest.mark.gpu
 FI_E_CAPACITY = -7
 
 
-def _domain(k, nx, prec):
+def _domain(k, nx, prec, **over):
     wl = W.ensemble_member(k, nx=nx, dx=50e3, seed=7000 + k)
-    p = G.make_params(precision=prec, **wl["params"])
+    p = G.make_params(precision=prec, **dict(wl["params"], **over))
     h = G.IsosClass(p, wl["nx"], wl["nx"], wl["dx"], he_field=wl["he_field"], eta_field=wl["eta_field"])
     b = G.bsl_init("const", 0.0)
     h.init_state(wl["z_bed"], np.zeros_like(wl["H_ice"]), 0.0, b)
@@ -56,6 +56,35 @@ def test_server_mix_matches_single_updates():
             h.update(wl["H_ice"], 10.0 * k, b)
     for (a, _, _), (s, _, _) in zip(alone, served):
         assert a.scalar("t") == s.scalar("t") == 10.0 * steps
+        assert a.scalar("attempts") == s.scalar("attempts")
+        for f in ("w", "z_bed", "dwdt"):
+            assert np.array_equal(a.out(f), s.out(f)), f
+    srv.close()
+
+
+def test_server_integrators_in_one_round():
+    """Two domains of one size and precision that differ only in the
+    integrator (tsit54, bs32) are served as separate groups of the same round,
+    concurrently, and each reproduces its own single-domain updates bit for
+    bit (the launch shape, which compiles the integrator into the inverse row
+    pass, is chosen per integrator)."""
+    spec = [dict(dt_method="tsit54"), dict(dt_method="bs32")]
+    alone = [_domain(k, 64, "mixed", **o) for k, o in enumerate(spec)]
+    served = [_domain(k, 64, "mixed", **o) for k, o in enumerate(spec)]
+    srv = S.Server(hbm_budget_bytes=8e9, max_batch=64)
+    slots = [srv.admit(h, b) for h, b, _ in served]
+    tag = 0
+    for step in range(1, 5):
+        for k in range(len(spec)):
+            srv.submit(slots[k], 10.0 * step, tag, H_ice=served[k][2]["H_ice"] if step == 1 else None)
+            tag += 1
+        while srv.pending():
+            srv.run()
+    assert sorted(srv.completed()) == list(range(tag))
+    for h, b, wl in alone:
+        for step in range(1, 5):
+            h.update(wl["H_ice"], 10.0 * step, b)
+    for (a, _, _), (s, _, _) in zip(alone, served):
         assert a.scalar("attempts") == s.scalar("attempts")
         for f in ("w", "z_bed", "dwdt"):
             assert np.array_equal(a.out(f), s.out(f)), f
