@@ -214,7 +214,9 @@ int fi_batch_destroy(fi_batch* b);
  *                      NULL to keep the current ice); tags order requests
  *   fi_server_run      one round: the oldest request of every domain with one,
  *                      grouped by N, precision, options and diagnostics phase,
- *                      one batched update per group; returns the count served
+ *                      one batched update per group (with max_batch > 1 the
+ *                      groups run concurrently on separate streams unless two
+ *                      share a bsl object); returns the count served
  *   fi_server_completed  tags of served requests, in completion order
  * A request's results are bitwise those of the same update run alone.
  * Handles must not be updated through fi_isos_update while admitted. */
