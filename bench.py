@@ -199,13 +199,14 @@ def run_serving(requests, n_domains, capacity_seconds):
     st0 = srv.stats()
     rate = 1.2 * cap1
     arrivals, dom = W.serving_requests(len(doms), requests, rate)
-    lat, wall = S.open_loop(srv, slots, t_now, arrivals, dom)
+    prog = 10000 if requests > 20000 else 0
+    lat, wall = S.open_loop(srv, slots, t_now, arrivals, dom, progress=prog)
     st1 = srv.stats()
     cells = sum(int(doms[int(k)]["nx"]) ** 2 for k in dom)
     # the same stream at 0.8 x the batched capacity: latency under load
     rate_hi = 0.8 * capb
     arr_hi, dom_hi = W.serving_requests(len(doms), requests, rate_hi, seed=5556)
-    lat_hi, wall_hi = S.open_loop(srv, slots, t_now, arr_hi, dom_hi)
+    lat_hi, wall_hi = S.open_loop(srv, slots, t_now, arr_hi, dom_hi, progress=prog)
     st2 = srv.stats()
     srv.close()
     for h in hs:

@@ -7,6 +7,7 @@ launches run in the native library; this module only feeds requests and reads
 completions.
 """
 import ctypes
+import sys
 import time
 
 import numpy as np
@@ -88,14 +89,16 @@ def closed_loop_rate(server, slots, t_now, seconds, step=10.0):
     return served / (time.perf_counter() - t0)
 
 
-def open_loop(server, slots, t_now, arrivals, domains, step=10.0):
+def open_loop(server, slots, t_now, arrivals, domains, step=10.0, progress=0):
     """Serve a request stream with arrival times `arrivals` (seconds from the
     start) on domain indices `domains`; returns per-request latencies
-    (seconds, arrival to completion) and the wall time."""
+    (seconds, arrival to completion) and the wall time.  With `progress` > 0
+    a line goes to stderr every `progress` completed requests."""
     n = len(arrivals)
     lat = np.full(n, np.nan)
     i = 0
     done = 0
+    next_report = progress if progress > 0 else n + 1
     t0 = time.perf_counter()
     while done < n:
         now = time.perf_counter() - t0
@@ -116,4 +119,7 @@ def open_loop(server, slots, t_now, arrivals, domains, step=10.0):
         for tag in server.completed():
             lat[tag] = t_end - arrivals[tag]
             done += 1
+        if done >= next_report:
+            print(f"# open loop: {done}/{n} requests served in {t_end:.1f} s", file=sys.stderr, flush=True)
+            next_report += progress
     return lat, time.perf_counter() - t0
