@@ -1766,6 +1766,18 @@ int fi_server_run(fi_server* s, int* completed) {
     }
     std::vector<UpdateJob> jobs(grp.size());
     std::vector<std::vector<double>> tvec(grp.size());
+    // a group whose update has completed leaves the queues at once, so that a
+    // later group's failure does not make a retry update it twice
+    auto retire = [&](size_t gi) {
+        for (int i : grp[gi]) {
+            fi_server::Slot& sl = s->slots[i];
+            s->done.push_back(sl.q.front().tag);
+            sl.q.pop_front();
+        }
+        *completed += (int)grp[gi].size();
+        s->served += (long long)grp[gi].size();
+        ++s->groups;
+    };
     int base = 0;
     for (size_t gi = 0; gi < grp.size(); ++gi) {
         const auto& g = grp[gi];
@@ -1814,6 +1826,7 @@ int fi_server_run(fi_server* s, int* completed) {
             SumBuf sb{s->sb.d + base, s->sb.h + base, nm};
             rc = run_update(hs, s->d_mem + base, it->second, sb, 0.0, true, times.data(), false);
             if (rc) return rc;
+            retire(gi);
         } else {
             tvec[gi] = times;
             UpdateJob& j = jobs[gi];
@@ -1851,22 +1864,15 @@ int fi_server_run(fi_server* s, int* completed) {
                 if (!more) { fin[gi] = 1; --left; }
             }
         }
-        for (auto& j : jobs) {
-            rc = job_end(j);
-            if (rc) return rc;
+        int first_rc = 0;
+        for (size_t gi = 0; gi < grp.size(); ++gi) {
+            rc = job_end(jobs[gi]);
+            if (rc == 0) retire(gi);
+            else if (first_rc == 0) first_rc = rc;
         }
-    }
-    for (const auto& g : grp) {
-        for (int q = 0; q < (int)g.size(); ++q) {
-            fi_server::Slot& sl = s->slots[g[q]];
-            s->done.push_back(sl.q.front().tag);
-            sl.q.pop_front();
-        }
-        *completed += (int)g.size();
-        ++s->groups;
+        if (first_rc) return first_rc;
     }
     ++s->rounds;
-    s->served += *completed;
     return 0;
 }
 

@@ -222,7 +222,17 @@ class Batch:
 
     def update(self, time, bsls, H_ice=None):
         n = len(self.members)
-        keep = [None] * n if H_ice is None else [_c(a) for a in H_ice]
+        if H_ice is None:
+            # (the pointer arrays of an update without new ice are reused while
+            # the bsl objects stay the same)
+            key = tuple(b.h.value for b in bsls)
+            if getattr(self, "_key", None) != key:
+                self._key = key
+                self._harr = (ctypes.POINTER(ctypes.c_double) * n)(*([None] * n))
+                self._barr = (ctypes.c_void_p * n)(*key)
+            check(load().fi_batch_update(self.h, self._harr, float(time), self._barr))
+            return
+        keep = [_c(a) for a in H_ice]
         Harr = (ctypes.POINTER(ctypes.c_double) * n)(*[_dp(a) for a in keep])
         barr = (ctypes.c_void_p * n)(*[b.h.value for b in bsls])
         check(load().fi_batch_update(self.h, Harr, float(time), barr))
