@@ -305,12 +305,14 @@ def test_results_identical_alone_and_in_batch(prec, nx, monkeypatch):
 
 
 def test_sub_batches_match_one_launch(monkeypatch):
-    """A batch split into concurrent sub-batches on several streams gives the
-    same bits as one launch over the whole batch."""
+    """A batch split into concurrent sub-batches on several streams, and one
+    launch with two row tiles per block in the inverse row pass, give the same
+    bits as one plain launch over the whole batch."""
     members = [W.ensemble_member(m, nx=64, dx=50e3) for m in range(32)]
 
-    def run(k):
+    def run(k, tiles=1):
         monkeypatch.setenv("FI_SUBBATCHES", str(k))
+        monkeypatch.setenv("FI_ROW_TILES", str(tiles))
         hs, bs = [], []
         for wl in members:
             p = G.make_params(precision="mixed", **wl["params"])
@@ -328,7 +330,9 @@ def test_sub_batches_match_one_launch(monkeypatch):
             h.close()
         return out
 
-    one, four = run(1), run(4)
+    one, four, tiled = run(1), run(4), run(1, tiles=2)
+    for a, b in zip(one, tiled):
+        assert np.array_equal(a, b)
     for a, b in zip(one, four):
         assert np.array_equal(a, b)
 
