@@ -235,10 +235,15 @@ int make_conv_shape(int n, bool fp64, ConvShape& s) {
     auto ok = [&](int lines) { return fits(lines) && nbmax * lines <= FFT_MAXIT * 512; };
     int L = 1;
     while (L < 4 && n % (4 * L) == 0 && ok(2 * L)) L *= 2;
+    // the column pass's lines are independent (its blocking does not change
+    // results): room for the twiddle table in LDS besides the lines; the row
+    // passes pair rows r0 + l and r0 + L + l, so L stays as is and they hold
+    // the table in LDS only where it fits
     int C = 1;
-    while (C < 8 && ok(2 * C)) C *= 2;
+    while (C < 8 && ok(2 * C) && fits(2 * C + 1)) C *= 2;
     s.Lr = L;
     s.Cc = C;
+    s.rows_twl = fits(L + 1);
     int rc = fft_threads(s.plan, L, s.threads_row);
     if (rc) return rc;
     rc = fft_threads(s.plan, C, s.threads_col);

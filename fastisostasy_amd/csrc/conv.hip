@@ -22,24 +22,27 @@
 namespace fi {
 
 // pass 1: input rows -> half spectra.  src real N x N; dst complex N x (N+1)
-template <typename T>
+// TWL: the twiddle table is held in LDS after the L lines.
+template <typename T, bool TWL>
 __global__ void __launch_bounds__(512) c_rows_r2c(const Member* __restrict__ mem, FftPlan plan,
                                                   const void* __restrict__ twv, int L, int which_src) {
     const Member& M = mem[blockIdx.y];
     extern __shared__ __align__(16) unsigned char smem[];
     cplx<T>* buf = (cplx<T>*)smem;
     const int N = M.n, Mm = 2 * N, r0 = blockIdx.x * 2 * L;
-    const cplx<T>* tw = (const cplx<T>*)twv;
+    const cplx<T>* tw = TWL ? stage_twiddles<T>(buf, Mm, L, twv) : (const cplx<T>*)twv;
     const double* __restrict__ src = M.conv_in;
     (void)which_src;
     const int tid = threadIdx.x, nt = blockDim.x;
     T* bf = (T*)buf;
-    for (int q = tid; q < 2 * L * Mm; q += nt) {
-        const int mr = q / Mm, i = q % Mm;
-        const int l = mr % L, part = mr / L;
-        const int row = r0 + mr;
-        bf[2 * (i * L + l) + part] = (i < N && row < N) ? (T)src[(size_t)row * N + i] : (T)0;
-    }
+    // zero padding: rows >= N and columns >= N of the (2N)^2 input
+    auto inside = [&](int q) { return q % Mm < N && r0 + q / Mm < N; };
+    staged_copy<8>(2 * L * Mm,
+                   [&](int q) { return inside(q) ? src[(size_t)(r0 + q / Mm) * N + q % Mm] : src[0]; },
+                   [&](int q, double v) {
+                       const int mr = q / Mm, i = q % Mm;
+                       bf[2 * (i * L + mr % L) + mr / L] = inside(q) ? (T)v : (T)0;
+                   });
     __syncthreads();
     lds_fft<T>(buf, plan, L, L, tw);
     cplx<T>* dst = (cplx<T>*)M.conv_work;
@@ -67,14 +70,12 @@ __global__ void __launch_bounds__(512) c_cols(const Member* __restrict__ mem, Ff
     cplx<T>* buf = (cplx<T>*)smem;
     const int N = M.n, Mm = 2 * N, H = N + 1;
     const int k0 = blockIdx.x * C;
-    const cplx<T>* tw = (const cplx<T>*)twv;
+    const cplx<T>* tw = stage_twiddles<T>(buf, Mm, C, twv);   // the host sizes C to leave room
     const cplx<T>* __restrict__ w1 = (const cplx<T>*)M.conv_work;
     const int tid = threadIdx.x, nt = blockDim.x;
-    for (int q = tid; q < Mm * C; q += nt) {
-        const int r = q / C, l = q % C;
-        const int kx = k0 + l;
-        buf[r * C + l] = (r < N && kx < H) ? w1[(size_t)r * H + kx] : mk<T>((T)0, (T)0);
-    }
+    auto inside = [&](int q) { return q / C < N && k0 + q % C < H; };
+    staged_copy<8>(Mm * C, [&](int q) { return inside(q) ? w1[(size_t)(q / C) * H + k0 + q % C] : w1[0]; },
+                   [&](int q, cplx<T> v) { buf[q] = inside(q) ? v : mk<T>((T)0, (T)0); });
     __syncthreads();
     lds_fft<T>(buf, plan, C, C, tw);
     cplx<T>* spec = (cplx<T>*)(which == 0 ? M.FGE : which == 1 ? M.FGN : M.FGV);
@@ -87,14 +88,14 @@ __global__ void __launch_bounds__(512) c_cols(const Member* __restrict__ mem, Ff
         return;
     }
     // product with the kernel spectrum, then conjugate for the inverse FFT
-    for (int q = tid; q < Mm * C; q += nt) {
-        const int r = q / C, l = q % C;
-        const int kx = k0 + l;
-        cplx<T> v = mk<T>((T)0, (T)0);
-        if (kx < H) v = cmul(spec[(size_t)r * H + kx], buf[r * C + l]);
-        v.y = -v.y;
-        buf[r * C + l] = v;
-    }
+    staged_copy<8>(Mm * C,
+                   [&](int q) { return k0 + q % C < H ? spec[(size_t)(q / C) * H + k0 + q % C] : spec[0]; },
+                   [&](int q, cplx<T> sv) {
+                       cplx<T> v = mk<T>((T)0, (T)0);
+                       if (k0 + q % C < H) v = cmul(sv, buf[q]);
+                       v.y = -v.y;
+                       buf[q] = v;
+                   });
     __syncthreads();
     lds_fft<T>(buf, plan, C, C, tw);
     cplx<T>* w2 = (cplx<T>*)M.conv_work + (size_t)N * H;
@@ -110,7 +111,7 @@ __global__ void __launch_bounds__(512) c_cols(const Member* __restrict__ mem, Ff
 }
 
 // pass 3: cropped rows back to real space, scale, corner correction, crop.
-template <typename T>
+template <typename T, bool TWL>
 __global__ void __launch_bounds__(512) c_rows_c2r(const Member* __restrict__ mem, FftPlan plan,
                                                   const void* __restrict__ twv, int L, int which) {
     const Member& M = mem[blockIdx.y];
@@ -120,7 +121,7 @@ __global__ void __launch_bounds__(512) c_rows_c2r(const Member* __restrict__ mem
     cplx<T>* buf = (cplx<T>*)smem;
     __shared__ double corr;
     const int N = M.n, Mm = 2 * N, H = N + 1, r0 = blockIdx.x * 2 * L;
-    const cplx<T>* tw = (const cplx<T>*)twv;
+    const cplx<T>* tw = TWL ? stage_twiddles<T>(buf, Mm, L, twv) : (const cplx<T>*)twv;
     const cplx<T>* __restrict__ w2 = (const cplx<T>*)M.conv_work + (size_t)N * H;
     const int tid = threadIdx.x, nt = blockDim.x;
     if (tid == 0) {
@@ -133,18 +134,19 @@ __global__ void __launch_bounds__(512) c_rows_c2r(const Member* __restrict__ mem
         corr = 0.25 * (f00 + fee + fe0 + f0e);
     }
     // Hermitian extension of two half-spectrum rows, conjugated for the inverse
-    for (int q = tid; q < L * Mm; q += nt) {
-        const int l = q / Mm, k = q % Mm;
-        const int ra = r0 + l, rb = r0 + L + l;
-        cplx<T> xa, xb;
-        if (k <= N) { xa = w2[(size_t)ra * H + k]; xb = w2[(size_t)rb * H + k]; }
-        else {
-            xa = w2[(size_t)ra * H + (Mm - k)]; xa.y = -xa.y;
-            xb = w2[(size_t)rb * H + (Mm - k)]; xb.y = -xb.y;
-        }
-        // z = xa + i xb, conjugated
-        buf[k * L + l] = mk<T>(xa.x - xb.y, -(xa.y + xb.x));
-    }
+    struct Pair { cplx<T> a, b; };
+    staged_copy<4>(L * Mm,
+                   [&](int q) {
+                       const int l = q / Mm, k = q % Mm, kk = k <= N ? k : Mm - k;
+                       return Pair{w2[(size_t)(r0 + l) * H + kk], w2[(size_t)(r0 + L + l) * H + kk]};
+                   },
+                   [&](int q, Pair p) {
+                       const int l = q / Mm, k = q % Mm;
+                       cplx<T> xa = p.a, xb = p.b;
+                       if (k > N) { xa.y = -xa.y; xb.y = -xb.y; }
+                       // z = xa + i xb, conjugated
+                       buf[k * L + l] = mk<T>(xa.x - xb.y, -(xa.y + xb.x));
+                   });
     __syncthreads();
     lds_fft<T>(buf, plan, L, L, tw);
     const double scale = (double)Mm * (double)Mm;
@@ -165,14 +167,25 @@ template <typename T>
 static size_t lbytes(int m, int lines) { return (size_t)m * lines * sizeof(cplx<T>); }
 
 template <typename T>
+static void rows_r2c(const Member* d_mem, int nmem, const ConvShape& s, int which, hipStream_t st) {
+    const dim3 g((s.n + 2 * s.Lr - 1) / (2 * s.Lr), nmem), b(s.threads_row);
+    if (s.rows_twl)
+        hipLaunchKernelGGL((c_rows_r2c<T, true>), g, b, lbytes<T>(s.m, s.Lr + 1), st, d_mem, s.plan, s.tw, s.Lr, which);
+    else
+        hipLaunchKernelGGL((c_rows_r2c<T, false>), g, b, lbytes<T>(s.m, s.Lr), st, d_mem, s.plan, s.tw, s.Lr, which);
+}
+
+template <typename T>
 static void conv_t(const Member* d_mem, int nmem, const ConvShape& s, int which, hipStream_t st) {
     const int n = s.n, H = n + 1;
-    hipLaunchKernelGGL(c_rows_r2c<T>, dim3((n + 2 * s.Lr - 1) / (2 * s.Lr), nmem), dim3(s.threads_row),
-                       lbytes<T>(s.m, s.Lr), st, d_mem, s.plan, s.tw, s.Lr, which);
-    hipLaunchKernelGGL(c_cols<T>, dim3((H + s.Cc - 1) / s.Cc, nmem), dim3(s.threads_col), lbytes<T>(s.m, s.Cc), st,
-                       d_mem, s.plan, s.tw, s.Cc, 1, which);
-    hipLaunchKernelGGL(c_rows_c2r<T>, dim3(n / (2 * s.Lr), nmem), dim3(s.threads_row), lbytes<T>(s.m, s.Lr), st,
-                       d_mem, s.plan, s.tw, s.Lr, which);
+    rows_r2c<T>(d_mem, nmem, s, which, st);
+    hipLaunchKernelGGL(c_cols<T>, dim3((H + s.Cc - 1) / s.Cc, nmem), dim3(s.threads_col),
+                       lbytes<T>(s.m, s.Cc + 1), st, d_mem, s.plan, s.tw, s.Cc, 1, which);
+    const dim3 g(n / (2 * s.Lr), nmem), b(s.threads_row);
+    if (s.rows_twl)
+        hipLaunchKernelGGL((c_rows_c2r<T, true>), g, b, lbytes<T>(s.m, s.Lr + 1), st, d_mem, s.plan, s.tw, s.Lr, which);
+    else
+        hipLaunchKernelGGL((c_rows_c2r<T, false>), g, b, lbytes<T>(s.m, s.Lr), st, d_mem, s.plan, s.tw, s.Lr, which);
 }
 
 void launch_convolution(const Member* d_mem, int nmem, const ConvShape& s, int which, hipStream_t st) {
@@ -184,10 +197,9 @@ void launch_convolution(const Member* d_mem, int nmem, const ConvShape& s, int w
 template <typename T>
 static void spec_t(const Member* d_mem, int nmem, const ConvShape& s, int which, hipStream_t st) {
     const int n = s.n, H = n + 1;
-    hipLaunchKernelGGL(c_rows_r2c<T>, dim3((n + 2 * s.Lr - 1) / (2 * s.Lr), nmem), dim3(s.threads_row),
-                       lbytes<T>(s.m, s.Lr), st, d_mem, s.plan, s.tw, s.Lr, which);
-    hipLaunchKernelGGL(c_cols<T>, dim3((H + s.Cc - 1) / s.Cc, nmem), dim3(s.threads_col), lbytes<T>(s.m, s.Cc), st,
-                       d_mem, s.plan, s.tw, s.Cc, 0, which);
+    rows_r2c<T>(d_mem, nmem, s, which, st);
+    hipLaunchKernelGGL(c_cols<T>, dim3((H + s.Cc - 1) / s.Cc, nmem), dim3(s.threads_col),
+                       lbytes<T>(s.m, s.Cc + 1), st, d_mem, s.plan, s.tw, s.Cc, 0, which);
 }
 
 void launch_kernel_spectrum(const Member* d_mem, int nmem, const ConvShape& s, int which,

@@ -36,6 +36,7 @@ struct ConvShape {
     FftPlan plan;     // length m
     const void* tw;   // length-m twiddles in conv precision
     bool fp64;
+    bool rows_twl;    // the row passes also hold the twiddle table in LDS (the column pass always does)
 };
 
 void launch_attempt(const Member* d_mem, int nmem, const RhsShape& s, hipStream_t st);
