@@ -110,9 +110,11 @@ __device__ __forceinline__ void xcd_order(int& member, int& blk) {
 
 // The march and the row transform of one block.  issue(tau) issues the
 // global loads due at march step tau (inputs arrive SR_LA steps ahead of use).
-template <typename T, int N, int THREADS, class Issue>
-__device__ __forceinline__ void srow_body(const Member& M, const kptr<PhysDev> ph, int R0, int tid, bool top,
-                                          bool bottom, unsigned char* smem,
+// Rows are counted in march order: first_edge / last_edge say that the march
+// starts / ends at a domain edge; UP marches run against the row order.
+template <typename T, int N, int THREADS, bool UP, class Issue>
+__device__ __forceinline__ void srow_body(const Member& M, const kptr<PhysDev> ph, int R0, int tid, bool first_edge,
+                                          bool last_edge, unsigned char* smem,
                                           double (&wv)[SRow<T, N, THREADS>::ROWS + 4][SRow<T, N, THREADS>::CPT],
                                           T (&dv)[SRow<T, N, THREADS>::ROWS + 2][SRow<T, N, THREADS>::CPT],
                                           T (&cv)[SRow<T, N, THREADS>::ROWS][SRow<T, N, THREADS>::CPT],
@@ -179,7 +181,7 @@ __device__ __forceinline__ void srow_body(const Member& M, const kptr<PhysDev> p
                 double wa = wv[m + 3][q], wb = wv[m + 2][q], wc = wv[m + 1][q];
                 double whi = wx[m + 3][q], wlo = wx[m + 1][q], sy = inv2;
                 if constexpr (m <= 0) {
-                    if (top) {
+                    if (first_edge) {
                         wa = wv[2 + 2][q]; wb = wv[1 + 2][q]; wc = wv[0 + 2][q];
                         if constexpr (m == -1) { whi = wx[2][q]; wlo = wx[2][q]; }
                         else { wlo = wx[m + 2][q]; }
@@ -187,12 +189,18 @@ __device__ __forceinline__ void srow_body(const Member& M, const kptr<PhysDev> p
                     }
                 }
                 if constexpr (m >= ROWS - 1) {
-                    if (bottom) {
+                    if (last_edge) {
                         wa = wv[ROWS - 1 + 2][q]; wb = wv[ROWS - 2 + 2][q]; wc = wv[ROWS - 3 + 2][q];
                         if constexpr (m == ROWS - 1) { whi = wx[m + 2][q]; }
                         else { whi = wx[m + 1][q]; wlo = wx[m + 1][q]; }
                         sy = inv1;
                     }
+                }
+                // (an upward march meets row g + 1 before row g - 1: the
+                // operands keep the reference's order, w(g+1) first)
+                if constexpr (UP) {
+                    const double t1 = wa; wa = wc; wc = t1;
+                    const double t2 = whi; whi = wlo; wlo = t2;
                 }
                 const double wyy = (wa - 2 * wb + wc) * invq;
                 const double wxy = (whi - wlo) * sy;
@@ -218,13 +226,17 @@ __device__ __forceinline__ void srow_body(const Member& M, const kptr<PhysDev> p
                 double ya = my[j + 2][q], yb = my[j + 1][q], yc = my[j][q];
                 double hi = mx[j + 2][q], lo = mx[j][q], sy = inv2;
                 if constexpr (j == 0) {
-                    if (top) { ya = my[2 + 1][q]; yb = my[1 + 1][q]; yc = my[0 + 1][q]; lo = mx[j + 1][q]; sy = inv1; }
+                    if (first_edge) { ya = my[2 + 1][q]; yb = my[1 + 1][q]; yc = my[0 + 1][q]; lo = mx[j + 1][q]; sy = inv1; }
                 }
                 if constexpr (j == ROWS - 1) {
-                    if (bottom) {
+                    if (last_edge) {
                         ya = my[ROWS - 1 + 1][q]; yb = my[ROWS - 2 + 1][q]; yc = my[ROWS - 3 + 1][q];
                         hi = mx[j + 1][q]; sy = inv1;
                     }
+                }
+                if constexpr (UP) {
+                    const double t1 = ya; ya = yc; yc = t1;
+                    const double t2 = hi; hi = lo; lo = t2;
                 }
                 const double myy = (ya - 2 * yb + yc) * invq;
                 const double mxyxy = (hi - lo) * sy;
@@ -233,7 +245,7 @@ __device__ __forceinline__ void srow_body(const Member& M, const kptr<PhysDev> p
                 T f;
                 if constexpr (sizeof(T) == 8) f = (T)(num / (2.0 * (double)ev[j][q]));
                 else f = (T)(num * (double)ev[j][q]);   // mixed: eta holds 1/(2 eta)
-                sf[j * N + i] = f;
+                sf[(UP ? ROWS - 1 - j : j) * N + i] = f;
             }
         }
         if constexpr (m + 2 <= ROWS + 1) {
@@ -261,13 +273,17 @@ __device__ __forceinline__ void srow_body(const Member& M, const kptr<PhysDev> p
     }
 }
 
-template <typename T, int N, int THREADS>
-__global__ void __launch_bounds__(THREADS) kf_srow_fwd(const Member* __restrict__ mem, const void* __restrict__ twv) {
+// One row block of the fused pass.  UP: the block marches from its last row
+// to its first.  Neighbouring blocks march in opposite directions, so the two
+// blocks that share halo rows load them at the same point of their marches
+// (both at the start or both at the end) and the second load finds them in
+// L2; marching the same way, one block would load them first and the other
+// a dozen steps later, after L2 had evicted them.
+template <typename T, int N, int THREADS, bool UP>
+__device__ __forceinline__ void srow_block(const Member* __restrict__ mem, const void* __restrict__ twv, int mi,
+                                           int rblk) {
     using SR = SRow<T, N, THREADS>;
     constexpr int ROWS = SR::ROWS, CPT = SR::CPT;
-    static_assert(CPT >= 1 && CPT * THREADS == N && ROWS >= 4, "fused stencil shape");
-    int mi, rblk;
-    xcd_order(mi, rblk);
     const Member& M = mem[mi];
     const kptr<Ctl> c = kview(M.ctl);
     const kptr<PhysDev> ph = kview(M.phys);
@@ -332,17 +348,19 @@ __global__ void __launch_bounds__(THREADS) kf_srow_fwd(const Member* __restrict_
     // tau - 2 + SR_LA, each SR_LA steps before its first use.  Row bases are
     // uniform (scalar); the lane offset is one 32-bit value for every row.
     gptr<double> w = gin<double>(wpred);
+    // global row of march row r (-2 .. ROWS+1)
+    auto grow = [&](int r) { return UP ? R0 + ROWS - 1 - r : R0 + r; };
     auto issue = [&](auto tc) {
         constexpr int tau = decltype(tc)::value;
         constexpr int rw_ = tau + 2 + SR_LA, rd = tau + SR_LA, rc = tau - 2 + SR_LA;
 #pragma unroll
         for (int q = 0; q < CPT; ++q) {
             const unsigned col = tid + THREADS * q;
-            if constexpr (rw_ >= -2 && rw_ <= ROWS + 1) wv[rw_ + 2][q] = (w + (size_t)crow(R0 + rw_) * N)[col];
-            if constexpr (rd >= -1 && rd <= ROWS) dv[rd + 1][q] = (Dg + (size_t)crow(R0 + rd) * N)[col];
+            if constexpr (rw_ >= -2 && rw_ <= ROWS + 1) wv[rw_ + 2][q] = (w + (size_t)crow(grow(rw_)) * N)[col];
+            if constexpr (rd >= -1 && rd <= ROWS) dv[rd + 1][q] = (Dg + (size_t)crow(grow(rd)) * N)[col];
             if constexpr (rc >= 0 && rc < ROWS) {
-                cv[rc][q] = (Cg + (size_t)(R0 + rc) * N)[col];
-                ev[rc][q] = (Eg + (size_t)(R0 + rc) * N)[col];
+                cv[rc][q] = (Cg + (size_t)grow(rc) * N)[col];
+                ev[rc][q] = (Eg + (size_t)grow(rc) * N)[col];
             }
         }
         // keep the issue order: each step waits only on loads SR_LA steps old
@@ -393,8 +411,20 @@ __global__ void __launch_bounds__(THREADS) kf_srow_fwd(const Member* __restrict_
         if constexpr (THREADS * TWQ == 2 * N) ((T*)tw_l)[tid + THREADS * k] = twr[k];
         else if (tid + THREADS * k < 2 * N) ((T*)tw_l)[tid + THREADS * k] = twr[k];
     }
-    srow_body<T, N, THREADS>(M, ph, R0, tid, top, bottom, smem, wv, dv, cv, ev, issue);
+    // the march meets the domain edge it starts from first: top for a downward
+    // block, bottom for an upward one
+    srow_body<T, N, THREADS, UP>(M, ph, R0, tid, UP ? bottom : top, UP ? top : bottom, smem, wv, dv, cv, ev, issue);
     write_next();
+}
+
+template <typename T, int N, int THREADS>
+__global__ void __launch_bounds__(THREADS) kf_srow_fwd(const Member* __restrict__ mem, const void* __restrict__ twv) {
+    using SR = SRow<T, N, THREADS>;
+    static_assert(SR::CPT >= 1 && SR::CPT * THREADS == N && SR::ROWS >= 4, "fused stencil shape");
+    int mi, rblk;
+    xcd_order(mi, rblk);
+    if (rblk & 1) srow_block<T, N, THREADS, true>(mem, twv, mi, rblk);
+    else srow_block<T, N, THREADS, false>(mem, twv, mi, rblk);
 }
 
 // COMMIT: the attempt began with the fused stencil pass, which left the
