@@ -97,8 +97,10 @@ struct Member {
     void *D, *eta, *kR, *cfull;  // D_lith, eta_eff (plain: eta, mixed: 1/(2 eta)),
                                  // kappa*R (plain; mixed: 1/(kappa*R)), canom_full
     void *g1, *g2;               // FFT work arrays (N x N)
-    double* corners;             // per column-block Hartley corner partials [nblk][4]
-    double* partials;            // per block reduction partials [nblk][4]
+    double* corners;             // per-column terms of the four Hartley corners [N][4]
+    double* partials;            // error-norm partials [nblk][4]: one per row pair (power-of-two
+                                 // kernels), per row (mixed-radix), per row pair (relaxation),
+                                 // or per cell block (bookkeeping sums)
     double* conv_work;           // convolution work (complex, 2N x (N+1)) or null
     void* FGE;                   // elastic kernel half-spectrum, (2N) x (N+1) complex
     void* FGN;                   // sea-surface kernel half-spectrum

@@ -2,9 +2,10 @@
 // register Stockham FFT of fft_reg.hpp; same contract as the generic kernels of
 // rhs.hip (identical inputs, outputs, partial layouts and Ctl gating).
 //
+//   kf_srow_fwd  stencil fused with the row Hartley transform (x -> g2)
 //   kf_row_fwd   rows of f -> row Hartley transform        (g1 -> g2)
 //   kf_col       column Hartley transform, divide by kappa*R, column Hartley
-//                transform, corner partial sums             (g2 -> g1)
+//                transform, per-column corner terms         (g2 -> g1)
 //   kf_row_inv   row Hartley transform, /N^2, corner BC, * sec_per_year,
 //                stage sums, reduction partials             (g1 -> dwdt, x')
 //
