@@ -530,6 +530,18 @@ struct UpdateJob {
     double t_of(int m) const { return times ? times[m] : time; }
 };
 
+// Attempts launched beyond the first round's estimate: each costs a few
+// microseconds of gated no-op kernels when unused, a missing one a host round
+// trip.  One beats two (0.4%) and none (a rejection then forces a second
+// round) on the 64-member ensemble.  FI_MARGIN overrides (tuning).
+int first_round_margin() {
+    static const int m = [] {
+        const char* e = std::getenv("FI_MARGIN");
+        return e ? std::max(0, std::atoi(e)) : 1;
+    }();
+    return m;
+}
+
 int job_begin(UpdateJob& j) {
     const int nmem = (int)j.hs.size();
     const fi_isos& h0 = *j.hs[0];
@@ -572,13 +584,12 @@ int job_begin(UpdateJob& j) {
     // Attempts until every member reached `time`.  Members that are done skip
     // the launches on the device, so the host may launch ahead: the first
     // round launches as many attempts as the slowest member needs at the dt it
-    // ended its previous update with, plus two for rejections (an extra attempt
-    // costs a few microseconds of gated no-op kernels, a second round a host
-    // round trip); after each readback job_more corrects the estimate.
+    // ended its previous update with, plus a margin for rejections; after each
+    // readback job_more corrects the estimate.
     double need = 0.0;
     for (int m = 0; m < nmem; ++m)
         if (j.hs[m]->last_dt > 0.0) need = std::max(need, std::ceil((j.t_of(m) - j.hs[m]->last_t) / j.hs[m]->last_dt));
-    if (need > 0.0) need += 2.0;
+    if (need > 0.0) need += (double)first_round_margin();
     j.ahead = (int)std::min(256.0, std::max(1.0, need));
     // concurrent sub-batches only on the library stream (their streams are shared)
     j.K = shape.method == 3 && j.st == g_stream ? sub_count(nmem) : 1;
