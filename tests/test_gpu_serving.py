@@ -91,6 +91,40 @@ def test_server_integrators_in_one_round():
     srv.close()
 
 
+def test_server_shared_bsl_across_groups():
+    """Two domains of different sizes (separate launch groups) that share one
+    fastiso bsl object are served in one round in request order, on one
+    stream, and reproduce consecutive isos_update calls bit for bit, the
+    barystatic sea level included."""
+    def make(k, nx, b):
+        wl = W.ensemble_member(k, nx=nx, dx=50e3, seed=7100 + k)
+        p = G.make_params(precision="mixed", **wl["params"])
+        h = G.IsosClass(p, wl["nx"], wl["nx"], wl["dx"], he_field=wl["he_field"], eta_field=wl["eta_field"])
+        h.init_state(wl["z_bed"], np.zeros_like(wl["H_ice"]), 0.0, b)
+        return h, wl
+
+    ba, bs = G.bsl_init("fastiso", 0.0), G.bsl_init("fastiso", 0.0)
+    alone = [make(0, 64, ba), make(1, 96, ba)]
+    served = [make(0, 64, bs), make(1, 96, bs)]
+    srv = S.Server(hbm_budget_bytes=8e9, max_batch=64)
+    slots = [srv.admit(h, bs) for h, _ in served]
+    tag = 0
+    for step in range(1, 5):
+        for k in range(2):
+            srv.submit(slots[k], 10.0 * step, tag, H_ice=served[k][1]["H_ice"])
+            tag += 1
+        srv.run()
+        assert srv.pending() == 0
+        for h, wl in alone:
+            h.update(wl["H_ice"], 10.0 * step, ba)
+    assert sorted(srv.completed()) == list(range(tag))
+    assert ba.bsl_now == bs.bsl_now != 0.0
+    for (a, _), (s, _) in zip(alone, served):
+        for f in ("w", "z_bed"):
+            assert np.array_equal(a.out(f), s.out(f)), f
+    srv.close()
+
+
 def test_server_admission_budget():
     doms = [_domain(k, 64, "mixed") for k in range(3)]
     fp = doms[0][0].scalar("hbm_bytes")
