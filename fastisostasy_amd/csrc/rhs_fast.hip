@@ -91,7 +91,9 @@ template <typename T, int N, int THREADS> struct SRow {
 };
 
 // rows of canom_full / eta the march loads ahead of their use
-constexpr int SR_LA = 6;
+// (4: measured 23.7 us per launch on the 64-member ensemble against 24.9 at 6
+// and 25.7 at 8; nearer loads hold fewer registers)
+constexpr int SR_LA = 4;
 
 template <int B, int E, class F> __device__ __forceinline__ void static_for(F&& f) {
     if constexpr (B < E) {
