@@ -259,7 +259,9 @@ size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 }  // namespace
 
-// Summary buffers of a launch group (device + pinned host).
+// Summary buffer of a launch group: pinned, coherent host memory that the
+// gather kernel stores to directly through its device address `d`, so a
+// round's readback is that kernel alone (no blit copy queued behind it).
 struct SumBuf {
     CtlSummary* d = nullptr;
     CtlSummary* h = nullptr;
@@ -351,13 +353,12 @@ int write_ctl(const Member& m, const Ctl& c) {
 }
 
 int sum_alloc(SumBuf& b, int n) {
-    HIPCHK(hipMalloc(&b.d, sizeof(CtlSummary) * n));
-    HIPCHK(hipHostMalloc(&b.h, sizeof(CtlSummary) * n, hipHostMallocDefault));
+    HIPCHK(hipHostMalloc(&b.h, sizeof(CtlSummary) * n, hipHostMallocMapped | hipHostMallocCoherent));
+    HIPCHK(hipHostGetDevicePointer((void**)&b.d, b.h, 0));
     b.n = n;
     return 0;
 }
 void sum_free(SumBuf& b) {
-    if (b.d) (void)hipFree(b.d);
     if (b.h) (void)hipHostFree(b.h);
     b.d = nullptr;
     b.h = nullptr;
@@ -366,7 +367,6 @@ void sum_free(SumBuf& b) {
 int gather(Member* d_mem, int nmem, SumBuf& sb) {
     launch_gather(d_mem, nmem, sb.d, g_stream);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(sb.h, sb.d, sizeof(CtlSummary) * nmem, hipMemcpyDeviceToHost, g_stream));
     HIPCHK(hipStreamSynchronize(g_stream));
     return 0;
 }
@@ -639,9 +639,8 @@ int job_round(UpdateJob& j) {
     }
     launch_finalize(d_mem, nmem, n, j.st);
     HIPCHK(hipGetLastError());
-    launch_gather(d_mem, nmem, j.sb.d, j.st);
+    launch_gather(d_mem, nmem, j.sb.d, j.st);   // stores into pinned host memory
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(j.sb.h, j.sb.d, sizeof(CtlSummary) * nmem, hipMemcpyDeviceToHost, j.st));
     return 0;
 }
 
