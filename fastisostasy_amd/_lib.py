@@ -20,7 +20,7 @@ FIELDS = ["z_bed", "z_ss", "dz_ss", "w", "we", "dwdt", "canom_full", "Hice",
 FIELD_ID = {n: i for i, n in enumerate(FIELDS)}
 OUT_FIELDS = FIELDS[:11]
 SCALARS = ["n", "t", "dt", "bsl", "deltaV", "V_af", "V_den", "attempts", "accepts", "D_lith11", "err", "tol",
-           "hbm_bytes"]
+           "hbm_bytes", "status"]
 SCALAR_ID = {n: i for i, n in enumerate(SCALARS)}
 
 # every entry point of include/fastiso.h
@@ -33,7 +33,7 @@ EXPORTS = ["fi_last_error", "fi_version", "fi_padded_size", "fi_isos_create", "f
            "fi_bsl_destroy", "fi_batch_create", "fi_batch_update", "fi_batch_destroy", "fi_sync",
            "fi_prof_select", "fi_prof_collect", "fi_prof_fused", "fi_bench_copy",
            "fi_server_create", "fi_server_admit", "fi_server_release", "fi_server_submit", "fi_server_pending",
-           "fi_server_run", "fi_server_completed", "fi_server_stats", "fi_server_destroy"]
+           "fi_server_run", "fi_server_completed", "fi_server_failed", "fi_server_stats", "fi_server_destroy"]
 
 
 class FiParams(ctypes.Structure):
@@ -128,6 +128,7 @@ def load():
     L.fi_server_pending.argtypes = [vp, ctypes.POINTER(i)]
     L.fi_server_run.argtypes = [vp, ctypes.POINTER(i)]
     L.fi_server_completed.argtypes = [vp, ctypes.POINTER(ll), i, ctypes.POINTER(i)]
+    L.fi_server_failed.argtypes = [vp, ctypes.POINTER(ll), i, ctypes.POINTER(i)]
     L.fi_server_stats.argtypes = [vp, ctypes.POINTER(ll), ctypes.POINTER(ll), ctypes.POINTER(ll), dp]
     L.fi_server_destroy.argtypes = [vp]
     for name in EXPORTS:

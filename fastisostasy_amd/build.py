@@ -20,20 +20,30 @@ FLAGS = [
 ]
 
 
-def build(verbose=False):
-    objs = []
+def build(verbose=False, jobs=None):
+    """Compile every source (in parallel, `jobs` at a time) and link libfastiso.so."""
+    from concurrent.futures import ThreadPoolExecutor
+
     builddir = os.path.join(HERE, "_build")
     os.makedirs(builddir, exist_ok=True)
+    cmds, objs = [], []
     for src in SOURCES:
         path = os.path.join(CSRC, src)
         if not os.path.exists(path):
             continue
         obj = os.path.join(builddir, src + ".o")
-        cmd = [HIPCC] + FLAGS + ["-x", "hip", "-c", path, "-o", obj]
+        cmds.append([HIPCC] + FLAGS + ["-x", "hip", "-c", path, "-o", obj])
+        objs.append(obj)
+
+    def run(cmd):
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)
-        objs.append(obj)
+
+    jobs = jobs or min(len(cmds), int(os.environ.get("MAX_JOBS", "0") or 0) or os.cpu_count() or 1)
+    with ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
+        for f in [ex.submit(run, c) for c in cmds]:
+            f.result()
     tmp = LIB + ".tmp"
     subprocess.run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs, check=True)
     os.replace(tmp, LIB)

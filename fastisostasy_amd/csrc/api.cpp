@@ -284,6 +284,7 @@ struct fi_isos {
     bool use_restart = false;    // a restart was read: the next init_state takes the restart branch
     double time_diag = 0, time_prog = 0;
     double last_t = 0, last_dt = 0;  // ode%t and ode%dt after the last update (host copy; 0 = unknown)
+    int last_status = 0;         // status of the last update's instability and NaN checks
     char* arena = nullptr;
     size_t arena_bytes = 0;      // HBM footprint of the domain (one allocation)
     Member hm{};
@@ -371,12 +372,18 @@ int gather(Member* d_mem, int nmem, SumBuf& sb) {
     return 0;
 }
 
-// Concurrent sub-batches of the attempt loop.  One launch of an attempt
-// kernel over a large batch is a single occupancy round in which all blocks
-// load, then compute, in step; sub-batches on separate streams (default
-// priority) drift apart, so one sub-batch's transforms overlap another's
-// loads.  Results do not depend on the split (reductions are fixed by N).
-constexpr int MAX_SUB = 4;
+// Worker streams: the process uses at most 1 + MAX_SUB streams (the library
+// stream and these three, default priority), matching the 4 hardware queues
+// HIP gives a process by default; more streams would share queues and
+// serialise behind each other by accident.  They carry the concurrent
+// sub-batches of the attempt loop and the concurrent groups of a serving
+// round (never both at once: sub-batches run only on the library stream).
+// One launch of an attempt kernel over a large batch is a single occupancy
+// round in which all blocks load, then compute, in step; sub-batches on
+// separate streams drift apart, so one sub-batch's transforms overlap
+// another's loads.  Results do not depend on the split (reductions are fixed
+// by N).
+constexpr int MAX_SUB = 3;
 struct SubStreams {
     hipStream_t s[MAX_SUB] = {};
     hipEvent_t done[MAX_SUB] = {};
@@ -527,6 +534,7 @@ struct UpdateJob {
     long long total = 0;
     int sb_lo[MAX_SUB + 1] = {};
     const RhsShape* sshape[MAX_SUB] = {};
+    int failed_member = -1;      // set by job_end: the first member that failed its checks
     double t_of(int m) const { return times ? times[m] : time; }
 };
 
@@ -654,6 +662,8 @@ int job_more(UpdateJob& j, bool& more) {
     for (int m = 0; m < (int)j.hs.size(); ++m) {
         const CtlSummary& c = j.sb.h[m];
         if (c.status != ST_RUN) continue;
+        // rejections halve dt; it reaches zero only if every attempt is rejected
+        if (!(c.dt > 0.0)) return fail(FI_E_STATE, "member %d: the step size collapsed to zero", m);
         running = true;
         need = std::max(need, std::ceil((c.tf - c.t) / std::max(c.dt, 1e-300)));
     }
@@ -664,20 +674,42 @@ int job_more(UpdateJob& j, bool& more) {
     return 0;
 }
 
+// check_isos_instabilities (fastisostasy.f90:876-884: w, we, w_equilibrium,
+// dz_ss, in that order) and the NaN check of z_bed (:864-869) of one domain:
+// 0, FI_E_UNSTABLE or FI_E_NAN, with the array named in `what`.
+int instability_status(const CtlSummary& c, const char*& what) {
+    if (c.maxw > 1e5) { what = "large_check:: Error: Large values detected in array w"; return FI_E_UNSTABLE; }
+    if (c.maxwe > 1e5) { what = "large_check:: Error: Large values detected in array we"; return FI_E_UNSTABLE; }
+    if (c.maxweq > 1e5) {
+        what = "large_check:: Error: Large values detected in array w_equilibrium";
+        return FI_E_UNSTABLE;
+    }
+    if (c.maxdzss > 1e3) { what = "large_check:: Error: Large values detected in array dz_ss"; return FI_E_UNSTABLE; }
+    if (c.nonfinite) { what = "isos_update:: Error: NaNs detected."; return FI_E_NAN; }
+    what = "";
+    return 0;
+}
+
+// The end of an update: every member's times advance (its state has been
+// stepped on the device, failed or not) and keeps its own status
+// (FI_S_STATUS); the first failing member in batch order names the error.
 int job_end(UpdateJob& j) {
-    for (int m = 0; m < (int)j.hs.size(); ++m) {
+    int first = 0;
+    const int nmem = (int)j.hs.size();
+    for (int m = 0; m < nmem; ++m) {
         const CtlSummary& c = j.sb.h[m];
         fi_isos* h = j.hs[m];
         h->time_prog = j.t_of(m);
         h->last_t = c.t;
         h->last_dt = j.stepping ? c.dt : 0.0;
-        // check_isos_instabilities + NaN check (fastisostasy.f90:860-869)
-        if (c.maxw > 1e5) return fail(FI_E_UNSTABLE, "large_check:: Error: Large values detected in array w");
-        if (c.maxwe > 1e5) return fail(FI_E_UNSTABLE, "large_check:: Error: Large values detected in array we");
-        if (c.maxdzss > 1e3) return fail(FI_E_UNSTABLE, "large_check:: Error: Large values detected in array dz_ss");
-        if (c.nonfinite) return fail(FI_E_NAN, "isos_update:: Error: NaNs detected.");
+        const char* what;
+        h->last_status = instability_status(c, what);
+        if (h->last_status && !first) {
+            first = nmem > 1 ? fail(h->last_status, "member %d: %s", m, what) : fail(h->last_status, "%s", what);
+            j.failed_member = m;
+        }
     }
-    return 0;
+    return first;
 }
 
 int run_update(std::vector<fi_isos*>& hs, Member* d_mem, const RhsShape& shape, SumBuf& sb, double time,
@@ -960,7 +992,8 @@ static int create_impl(const fi_params* p, int nx_ice, int ny_ice, double dx, do
     size_t oHin = take((size_t)nx_ice * ny_ice * 8);
     size_t oD = take(nn * rp), oeta = take(nn * rp), okR = take(nn * rp), ocfr = take(nn * rp), og1 = take(nn * rp),
            og2 = take(nn * rp);
-    size_t ocor = take((size_t)n * 4 * 8), opart = take((size_t)h->part_blocks * 4 * 8);
+    // partials: 4 values per block for the attempt kernels, 5 for the finalize reduction
+    size_t ocor = take((size_t)n * 4 * 8), opart = take((size_t)h->part_blocks * 5 * 8);
     size_t oee = take(f64), okap = take(f64), oR = take(f64), oHe = take(f64), oDd = take(f64);
     size_t oscr = take(f64 + 64);
     size_t oweq = (p->method != 3) ? take(f64) : 0;
@@ -1526,6 +1559,7 @@ int fi_isos_get_scalar(fi_isos* h, int which, double* out) {
     case FI_S_ERR: *out = c.err; break;
     case FI_S_TOL: *out = c.tol; break;
     case FI_S_HBM_BYTES: *out = (double)h->arena_bytes; break;
+    case FI_S_STATUS: *out = (double)h->last_status; break;
     default: return fail(FI_E_ARG, "unknown scalar %d", which);
     }
     return 0;
@@ -1631,20 +1665,20 @@ struct fi_server {
     // (n, fp64, uniform, nmem, method, dt_method)
     std::map<std::tuple<int, int, int, int, int, int>, RhsShape> shapes;
     long long rounds = 0, groups = 0, served = 0;
-    // groups of one round run concurrently, one stream each (default priority)
-    static constexpr int NSTREAM = 8;
-    hipStream_t st[NSTREAM] = {};
+    // groups of one round run concurrently on the worker streams, round-robin
+    static constexpr int NSTREAM = MAX_SUB;
     std::vector<hipEvent_t> ev;   // one per concurrent group: its last readback
     hipEvent_t start = nullptr;   // library-stream work queued before the round
+    std::vector<long long> failed;   // tags of requests whose update failed its checks
 };
 
 namespace {
 
 // every stream the server launches on, drained
 void server_drain(fi_server* s) {
+    (void)s;
     (void)hipStreamSynchronize(g_stream);
-    for (hipStream_t t : s->st)
-        if (t) (void)hipStreamSynchronize(t);
+    for (int k = 0; k < g_sub.n; ++k) (void)hipStreamSynchronize(g_sub.s[k]);
 }
 
 int server_reserve(fi_server* s, int n) {
@@ -1767,8 +1801,8 @@ int fi_server_run(fi_server* s, int* completed) {
             }
     }
     if (concurrent) {
-        for (int k = 0; k < fi_server::NSTREAM; ++k)
-            if (!s->st[k]) HIPCHK(hipStreamCreateWithFlags(&s->st[k], hipStreamNonBlocking));
+        rc = ensure_sub(fi_server::NSTREAM);
+        if (rc) return rc;
         while (s->ev.size() < grp.size()) {
             hipEvent_t e = nullptr;
             HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -1777,26 +1811,36 @@ int fi_server_run(fi_server* s, int* completed) {
         // the round starts after everything already queued on the library stream
         if (!s->start) HIPCHK(hipEventCreateWithFlags(&s->start, hipEventDisableTiming));
         HIPCHK(hipEventRecord(s->start, g_stream));
-        for (hipStream_t t : s->st) HIPCHK(hipStreamWaitEvent(t, s->start, 0));
+        for (int k = 0; k < fi_server::NSTREAM; ++k) HIPCHK(hipStreamWaitEvent(g_sub.s[k], s->start, 0));
     }
     std::vector<UpdateJob> jobs(grp.size());
     std::vector<std::vector<double>> tvec(grp.size());
-    // a group whose update has completed leaves the queues at once, so that a
-    // later group's failure does not make a retry update it twice
-    auto retire = [&](size_t gi) {
+    // A group whose update has run on the device leaves the queues at once,
+    // whether its checks passed or not, so that no retry updates it twice:
+    // its requests complete (fi_server_completed) or fail (fi_server_failed).
+    // The round's return value is the first error; the other groups still run.
+    int first_rc = 0;
+    auto retire = [&](size_t gi, int status) {
         for (int i : grp[gi]) {
             fi_server::Slot& sl = s->slots[i];
-            s->done.push_back(sl.q.front().tag);
+            // after failed checks only the members that failed them; after any other error the whole group
+            const bool checks = status == FI_E_UNSTABLE || status == FI_E_NAN;
+            const bool bad = status != 0 && (!checks || sl.h->last_status != 0);
+            (bad ? s->failed : s->done).push_back(sl.q.front().tag);
             sl.q.pop_front();
+            if (!bad) ++*completed;
+            ++s->served;
         }
-        *completed += (int)grp[gi].size();
-        s->served += (long long)grp[gi].size();
         ++s->groups;
+        if (status && !first_rc) first_rc = status;
     };
+    // an update whose error came from its checks has been applied
+    auto applied = [](int status) { return status == 0 || status == FI_E_UNSTABLE || status == FI_E_NAN; };
+    std::vector<char> launched(grp.size(), 0);
     int base = 0;
     for (size_t gi = 0; gi < grp.size(); ++gi) {
         const auto& g = grp[gi];
-        hipStream_t gst = concurrent ? s->st[gi % fi_server::NSTREAM] : g_stream;
+        hipStream_t gst = concurrent ? g_sub.s[gi % fi_server::NSTREAM] : g_stream;
         const int nm = (int)g.size();
         std::vector<fi_isos*> hs;
         std::vector<double> times;
@@ -1831,7 +1875,7 @@ int fi_server_run(fi_server* s, int* completed) {
         if (it == s->shapes.end()) {
             RhsShape shp{};
             rc = make_shape(h0->n, nm, h0->fp64, h0->uniform, shp);
-            if (rc) return rc;
+            if (rc) { first_rc = rc; break; }
             shp.method = h0->par.method;
             shp.dt_method = h0->par.dt_method;
             it = s->shapes.emplace(sk, shp).first;
@@ -1840,8 +1884,8 @@ int fi_server_run(fi_server* s, int* completed) {
             // groupings change from round to round: plain launches, no graph captures
             SumBuf sb{s->sb.d + base, s->sb.h + base, nm};
             rc = run_update(hs, s->d_mem + base, it->second, sb, 0.0, true, times.data(), false);
-            if (rc) return rc;
-            retire(gi);
+            if (!applied(rc)) { first_rc = rc; break; }
+            retire(gi, rc);
         } else {
             tvec[gi] = times;
             UpdateJob& j = jobs[gi];
@@ -1855,39 +1899,62 @@ int fi_server_run(fi_server* s, int* completed) {
             j.st = gst;
             rc = job_begin(j);
             if (!rc) rc = job_round(j);
-            if (rc) { server_drain(s); return rc; }
-            HIPCHK(hipEventRecord(s->ev[gi], gst));
+            if (!rc) rc = hipEventRecord(s->ev[gi], gst) == hipSuccess ? 0 : fail(FI_E_HIP, "hipEventRecord");
+            if (rc) {
+                // this group's launches may be partly queued: let them finish, launch no more groups
+                first_rc = rc;
+                server_drain(s);
+                break;
+            }
+            launched[gi] = 1;
         }
         base += nm;
     }
     if (concurrent) {
-        // rounds of every group until all are done; a group that needs another
-        // round launches it while the others' work is still on the GPU
+        // rounds of every launched group until all are done; a group that needs
+        // another round launches it while the others' work is still on the GPU;
+        // a group is retired as soon as it has finished
+        size_t left = 0;
+        for (char l : launched) left += l;
         std::vector<char> fin(grp.size(), 0);
-        size_t left = grp.size();
         while (left > 0) {
             for (size_t gi = 0; gi < grp.size(); ++gi) {
-                if (fin[gi]) continue;
-                HIPCHK(hipEventSynchronize(s->ev[gi]));
+                if (!launched[gi] || fin[gi]) continue;
+                if (hipEventSynchronize(s->ev[gi]) != hipSuccess) {
+                    server_drain(s);
+                    return fail(FI_E_HIP, "hipEventSynchronize: serving round lost");
+                }
                 bool more = false;
                 rc = job_more(jobs[gi], more);
                 if (!rc && more) {
                     rc = job_round(jobs[gi]);
-                    if (!rc) HIPCHK(hipEventRecord(s->ev[gi], jobs[gi].st));
+                    if (!rc && hipEventRecord(s->ev[gi], jobs[gi].st) != hipSuccess) rc = fail(FI_E_HIP, "hipEventRecord");
+                    if (rc) {
+                        server_drain(s);
+                        if (!first_rc) first_rc = rc;
+                        more = false;
+                    }
                 }
-                if (rc) { server_drain(s); return rc; }
-                if (!more) { fin[gi] = 1; --left; }
+                if (!more) {
+                    fin[gi] = 1;
+                    --left;
+                    // job_end's status (checks) or the integrator's failure to finish
+                    retire(gi, rc ? rc : job_end(jobs[gi]));
+                }
             }
         }
-        int first_rc = 0;
-        for (size_t gi = 0; gi < grp.size(); ++gi) {
-            rc = job_end(jobs[gi]);
-            if (rc == 0) retire(gi);
-            else if (first_rc == 0) first_rc = rc;
-        }
-        if (first_rc) return first_rc;
     }
+    if (first_rc) return first_rc;
     ++s->rounds;
+    return 0;
+}
+
+int fi_server_failed(fi_server* s, long long* tags, int max, int* n) {
+    if (!s || !n || (max > 0 && !tags)) return fail(FI_E_ARG, "NULL argument");
+    const int k = std::min<int>(max, (int)s->failed.size());
+    for (int i = 0; i < k; ++i) tags[i] = s->failed[i];
+    s->failed.erase(s->failed.begin(), s->failed.begin() + k);
+    *n = k;
     return 0;
 }
 
@@ -1916,8 +1983,6 @@ int fi_server_destroy(fi_server* s) {
     if (s->d_mem) (void)hipFree(s->d_mem);
     if (s->h_stage) (void)hipHostFree(s->h_stage);
     sum_free(s->sb);
-    for (hipStream_t t : s->st)
-        if (t) (void)hipStreamDestroy(t);
     for (hipEvent_t e : s->ev) (void)hipEventDestroy(e);
     if (s->start) (void)hipEventDestroy(s->start);
     delete s;

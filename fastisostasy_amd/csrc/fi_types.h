@@ -25,12 +25,13 @@ struct Ctl {
     double bsl;                  // now%bsl
     double time_diag, time_prog; // par%time_diagnostics / time_prognostics
     double maxw, maxwe, maxdzss; // instability check values of the last update
+    double maxweq;               // max |w_equilibrium| of the last update (0 for LV-ELVA)
     double wcorner;              // corner correction applied to now%w at the end of an update
     double ref_bsl;              // ref%bsl
     int cur;                     // index of the live ode%x buffer (0/1)
     int w_is_x;                  // now%w currently equals ode%x[cur]
     int k_valid;                 // dwdt holds the RHS of the current now%w
-    int status;                  // 1 stepping, 0 idle; negative = error
+    int status;                  // ST_RUN stepping, ST_IDLE idle, ST_NAN stopped by a NaN error norm
     int diag;                    // diagnostics updated in the running update
     int nonfinite;               // a NaN was seen in z_bed
     int pending;                 // the last attempt's accept/reject is not applied yet
@@ -40,11 +41,12 @@ struct Ctl {
     long long attempts, accepts;
 };
 
-enum { ST_IDLE = 0, ST_RUN = 1 };
+// ST_NAN: stepping stopped by a NaN error norm (reported as FI_E_NAN)
+enum { ST_IDLE = 0, ST_RUN = 1, ST_NAN = -1 };
 
 // Host-visible summary of one domain, gathered after a group of launches.
 struct CtlSummary {
-    double t, dt, tf, maxw, maxwe, maxdzss;
+    double t, dt, tf, maxw, maxwe, maxdzss, maxweq;
     int status, nonfinite;
 };
 

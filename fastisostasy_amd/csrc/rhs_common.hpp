@@ -136,13 +136,14 @@ __device__ __forceinline__ StepState decide_step(StepState s, double amax, doubl
     const double dt = attempt_dt(m, s.dt, s.t, s.tf, ph->dt_max);
     const bool stored = s.k_valid || !(s.t + dt < s.tf);
     s.attempts += 1;
-    bool accept = true;
+    bool accept = true, nan_err = false;
     if (m == 2 || m == 3) {
         const double tol = ph->atol + ph->rtol * amax;
         const double nn = (double)n * (double)n;
         const double err = sqrt(s2 / (tol * tol) / nn);
         s.tol = tol; s.err = err;
         accept = err < 1;
+        nan_err = err != err;
         // pow(x, 1) is x exactly (C99 F.9.4.4): the default q = 1 skips the
         // long FP64 pow sequence that every wave of the fused pass would run
         if (accept) {
@@ -163,6 +164,10 @@ __device__ __forceinline__ StepState decide_step(StepState s, double amax, doubl
         s.k_valid = stored ? 1 : 0;
     }
     if (!(s.t < s.tf)) s.status = ST_IDLE;
+    // A NaN error norm rejects every attempt (NaN < 1 is false), and the
+    // reference then halves dt without end; the update stops instead and
+    // reports the NaN (the finalize turns ST_NAN into the NaN flag).
+    if (nan_err) s.status = ST_NAN;
     return s;
 }
 
@@ -171,7 +176,8 @@ __device__ __forceinline__ void copy_ctl(Ctl* d, const kptr<Ctl> s) {
     d->t = s->t; d->dt = s->dt; d->tf = s->tf; d->err = s->err; d->tol = s->tol;
     d->V_af = s->V_af; d->V_den = s->V_den; d->V_pov = s->V_pov; d->deltaV = s->deltaV;
     d->dbsl_total = s->dbsl_total; d->bsl = s->bsl; d->time_diag = s->time_diag; d->time_prog = s->time_prog;
-    d->maxw = s->maxw; d->maxwe = s->maxwe; d->maxdzss = s->maxdzss; d->wcorner = s->wcorner;
+    d->maxw = s->maxw; d->maxwe = s->maxwe; d->maxdzss = s->maxdzss; d->maxweq = s->maxweq;
+    d->wcorner = s->wcorner;
     d->ref_bsl = s->ref_bsl; d->cur = s->cur; d->w_is_x = s->w_is_x; d->k_valid = s->k_valid;
     d->status = s->status; d->diag = s->diag; d->nonfinite = s->nonfinite; d->pending = s->pending;
     d->skip = s->skip; d->fin = s->fin; d->pad_ = 0; d->attempts = s->attempts; d->accepts = s->accepts;

@@ -20,6 +20,8 @@
 namespace fi {
 
 constexpr int PRE_THREADS = 256, PRE_CELLS = 1024;   // cells per block
+// values per block of the finalize reduction: max |w|, |we|, |dz_ss|, NaN flag, max |w_equilibrium|
+constexpr int FIN_P = 5;
 
 enum { MB_OCEAN = 1, MB_GROUNDED = 2, MB_CONTINENT = 4, MB_ACTIVE = 8 };
 
@@ -344,17 +346,21 @@ __global__ void __launch_bounds__(PRE_THREADS) k_finalize(const Member* __restri
     gptr<double> w_ref = gin<double>(M.w_ref), wev = gin<double>(M.we), zb_ref = gin<double>(M.z_bed_ref),
                  dz = gin<double>(M.dz_ss);
     gmut<double> W = gout<double>(M.W), cm = gout<double>(M.canom_mantle), zbed = gout<double>(M.z_bed);
-    double mw = 0.0, mwe = 0.0, mdz = 0.0, nanf = 0.0;
+    // w_equilibrium exists for methods 0-2; LV-ELVA leaves it zero (the reference checks it too)
+    const bool has_weq = M.weq != nullptr;
+    gptr<double> weqv = gin<double>(has_weq ? M.weq : M.dz_ss);
+    double mw = 0.0, mwe = 0.0, mdz = 0.0, nanf = 0.0, meq = 0.0;
     // every load of the thread's cells first (indices clamped into the field,
     // so no branch), then the arithmetic and the guarded stores: a store
     // between loads would make the next wait cover it too
     constexpr int IT = PRE_CELLS / PRE_THREADS;
-    double ws[IT], wr[IT], wea[IT], zr[IT], dzv[IT];
+    double ws[IT], wr[IT], wea[IT], zr[IT], dzv[IT], eqv[IT];
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
         size_t c = (size_t)blockIdx.x * PRE_CELLS + it * PRE_THREADS + threadIdx.x;
         c = c < nn ? c : nn - 1;
         ws[it] = wsrc[c]; wr[it] = w_ref[c]; wea[it] = wev[c]; zr[it] = zb_ref[c]; dzv[it] = dz[c];
+        eqv[it] = has_weq ? weqv[c] : 0.0;
     }
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
@@ -369,17 +375,18 @@ __global__ void __launch_bounds__(PRE_THREADS) k_finalize(const Member* __restri
         mw = fmax(mw, fabs(w));
         mwe = fmax(mwe, fabs(we));
         mdz = fmax(mdz, fabs(dzv[it]));
+        meq = fmax(meq, fabs(eqv[it]));
         if (zb != zb || w != w) nanf = 1.0;
     }
-    __shared__ double red[4][16];
-    mw = wave_max(mw); mwe = wave_max(mwe); mdz = wave_max(mdz); nanf = wave_max(nanf);
+    __shared__ double red[FIN_P][16];
+    mw = wave_max(mw); mwe = wave_max(mwe); mdz = wave_max(mdz); nanf = wave_max(nanf); meq = wave_max(meq);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    if (lane == 0) { red[0][wv] = mw; red[1][wv] = mwe; red[2][wv] = mdz; red[3][wv] = nanf; }
+    if (lane == 0) { red[0][wv] = mw; red[1][wv] = mwe; red[2][wv] = mdz; red[3][wv] = nanf; red[4][wv] = meq; }
     __syncthreads();
     if (threadIdx.x == 0) {
         for (int q = 1; q < PRE_THREADS / 64; ++q)
-            for (int r = 0; r < 4; ++r) red[r][0] = fmax(red[r][0], red[r][q]);
-        for (int r = 0; r < 4; ++r) M.partials[(size_t)blockIdx.x * 4 + r] = red[r][0];
+            for (int r = 0; r < FIN_P; ++r) red[r][0] = fmax(red[r][0], red[r][q]);
+        for (int r = 0; r < FIN_P; ++r) M.partials[(size_t)blockIdx.x * FIN_P + r] = red[r][0];
     }
 }
 
@@ -387,12 +394,13 @@ __global__ void __launch_bounds__(64) k_fin_check(const Member* __restrict__ mem
     const Member& M = mem[blockIdx.x];
     Ctl& c = *M.ctl;
     if (c.status == ST_RUN || c.fin) return;
-    double v[4] = {0, 0, 0, 0};
+    double v[FIN_P] = {0, 0, 0, 0, 0};
     for (int b = threadIdx.x; b < nblk; b += 64)
-        for (int r = 0; r < 4; ++r) v[r] = fmax(v[r], M.partials[(size_t)b * 4 + r]);
-    for (int r = 0; r < 4; ++r) v[r] = wave_max(v[r]);
+        for (int r = 0; r < FIN_P; ++r) v[r] = fmax(v[r], M.partials[(size_t)b * FIN_P + r]);
+    for (int r = 0; r < FIN_P; ++r) v[r] = wave_max(v[r]);
     if (threadIdx.x == 0) {
-        c.maxw = v[0]; c.maxwe = v[1]; c.maxdzss = v[2]; c.nonfinite = v[3] > 0.0;
+        c.maxw = v[0]; c.maxwe = v[1]; c.maxdzss = v[2]; c.maxweq = v[4];
+        c.nonfinite = v[3] > 0.0 || c.status == ST_NAN;
         c.w_is_x = 0;
         c.status = ST_IDLE;
         c.fin = 1;
@@ -417,7 +425,7 @@ __global__ void k_gather(const Member* __restrict__ mem, int nmem, CtlSummary* _
     if (m >= nmem) return;
     const Ctl& c = *mem[m].ctl;
     CtlSummary s;
-    s.t = c.t; s.dt = c.dt; s.tf = c.tf; s.maxw = c.maxw; s.maxwe = c.maxwe; s.maxdzss = c.maxdzss;
+    s.t = c.t; s.dt = c.dt; s.tf = c.tf; s.maxw = c.maxw; s.maxwe = c.maxwe; s.maxdzss = c.maxdzss; s.maxweq = c.maxweq;
     s.status = c.status; s.nonfinite = c.nonfinite;
     out[m] = s;
 }

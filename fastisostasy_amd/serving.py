@@ -60,6 +60,14 @@ class Server:
         check(load().fi_server_completed(self.h, buf, max_tags, ctypes.byref(n)))
         return list(buf[:n.value])
 
+    def failed(self, max_tags=1 << 16):
+        """Tags of requests whose update ran but failed its instability / NaN
+        checks (they are not re-run)."""
+        buf = (ctypes.c_longlong * max_tags)()
+        n = ctypes.c_int()
+        check(load().fi_server_failed(self.h, buf, max_tags, ctypes.byref(n)))
+        return list(buf[:n.value])
+
     def stats(self):
         r, g, s = ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_longlong()
         used = ctypes.c_double()

@@ -88,7 +88,9 @@ enum fi_field {
 enum fi_scalar {
     FI_S_N = 0, FI_S_T, FI_S_DT, FI_S_BSL, FI_S_DELTAV, FI_S_V_AF, FI_S_V_DEN,
     FI_S_ATTEMPTS, FI_S_ACCEPTS, FI_S_D_LITH11, FI_S_ERR, FI_S_TOL,
-    FI_S_HBM_BYTES          /* device memory held by the handle (bytes) */
+    FI_S_HBM_BYTES,         /* device memory held by the handle (bytes) */
+    FI_S_STATUS             /* status of the handle's last update checks: 0, FI_E_UNSTABLE or FI_E_NAN
+                               (tells which members of a failed fi_batch_update failed) */
 };
 
 typedef struct fi_isos fi_isos;
@@ -218,6 +220,11 @@ int fi_batch_destroy(fi_batch* b);
  *                      groups run concurrently on separate streams unless two
  *                      share a bsl object); returns the count served
  *   fi_server_completed  tags of served requests, in completion order
+ *   fi_server_failed     tags of requests whose update ran but failed the
+ *                      instability / NaN checks (the domain's FI_S_STATUS
+ *                      says which); fi_server_run returned that error.  A
+ *                      request leaves the queue once its update has run, so
+ *                      the next fi_server_run never applies it twice.
  * A request's results are bitwise those of the same update run alone.
  * Handles must not be updated through fi_isos_update while admitted. */
 typedef struct fi_server fi_server;
@@ -228,6 +235,7 @@ int fi_server_submit(fi_server* s, int slot, const double* H_ice, double time, l
 int fi_server_pending(fi_server* s, int* n);
 int fi_server_run(fi_server* s, int* completed);
 int fi_server_completed(fi_server* s, long long* tags, int max, int* n);
+int fi_server_failed(fi_server* s, long long* tags, int max, int* n);
 int fi_server_stats(fi_server* s, long long* rounds, long long* groups, long long* served, double* hbm_used);
 int fi_server_destroy(fi_server* s);
 

@@ -593,6 +593,7 @@ struct or_isos {
     double dx, dy, sec_per_year, compressibility_correction, Vden_factor;
     double time_diagnostics, time_prognostics;
     int ref_was_set, use_restart;
+    int nan_stop;                /* the last integration stopped on a NaN error norm */
     double *K, *dxm, *A, *kappa, *R, *maskactive, *He_lith, *D_lith, *eta_eff;
     cplx *FGE, *FGN, *FGV;
     double tau, L_w;            /* domain%tau (uniform), par%L_w */
@@ -1115,6 +1116,10 @@ static void integrate(or_isos *s, double tf) {
         double acc = 0.0;
         for (size_t c = 0; c < nn; c++) { const double e = (x2[c] - x1[c]) / tol; acc += e * e; }
         const double error = sqrt(acc / (double)nn);
+        /* A NaN error norm rejects every attempt and the reference halves dt
+         * without end (integrators.f90:188-199); the update stops instead and
+         * is reported by the NaN check. */
+        if (isnan(error)) { s->nan_stop = 1; break; }
         if (error < 1) {
             memcpy(x, x2, sizeof(double) * nn);
             memcpy(s->now.w, x, sizeof(double) * nn);
@@ -1177,11 +1182,15 @@ static int isos_update_impl(or_isos *s, const double *H_ice, double time, or_bsl
     canom_mantle(s);
     for (size_t c = 0; c < nn; c++) s->now.z_bed[c] = s->ref.z_bed[c] + s->now.w[c] + s->now.we[c];
     s->time_prognostics = time;
+    /* check_isos_instabilities, fastisostasy.f90:876-884: w, we, w_equilibrium, dz_ss */
     if (large_check(s->now.w, nn, 1e5, "w") || large_check(s->now.we, nn, 1e5, "we") ||
-        large_check(s->now.dz_ss, nn, 1e3, "dz_ss"))
+        large_check(s->now.w_eq, nn, 1e5, "w_equilibrium") || large_check(s->now.dz_ss, nn, 1e3, "dz_ss"))
         return -2;
+    int nan = s->nan_stop;
+    s->nan_stop = 0;
     for (size_t c = 0; c < nn; c++)
-        if (isnan(s->now.z_bed[c])) { set_err("isos_update:: Error: NaNs detected."); return -3; }
+        if (isnan(s->now.z_bed[c])) nan = 1;
+    if (nan) { set_err("isos_update:: Error: NaNs detected."); return -3; }
     return 0;
 }
 
