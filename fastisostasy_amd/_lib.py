@@ -26,7 +26,7 @@ SCALAR_ID = {n: i for i, n in enumerate(SCALARS)}
 # every entry point of include/fastiso.h
 EXPORTS = ["fi_last_error", "fi_version", "fi_padded_size", "fi_isos_create", "fi_isos_create_gridded",
            "fi_isos_init_ref",
-           "fi_isos_init_state", "fi_isos_update", "fi_isos_get", "fi_isos_get_mask", "fi_isos_get_full",
+           "fi_isos_init_state", "fi_isos_update", "fi_isos_update_noalloc", "fi_isos_get", "fi_isos_get_mask", "fi_isos_get_full",
            "fi_isos_get_scalar", "fi_isos_restart_write", "fi_isos_restart_read", "fi_bsl_restart_write",
            "fi_bsl_restart_read", "fi_isos_destroy", "fi_bsl_create", "fi_bsl_create_series",
            "fi_bsl_read_series", "fi_bsl_update", "fi_bsl_get",
@@ -98,6 +98,7 @@ def load():
     L.fi_isos_init_ref.argtypes = [vp, dp, dp, dp, dp]
     L.fi_isos_init_state.argtypes = [vp, dp, dp, d, vp]
     L.fi_isos_update.argtypes = [vp, dp, d, vp]
+    L.fi_isos_update_noalloc.argtypes = [vp, dp, d, vp]
     L.fi_isos_get.argtypes = [vp, i, dp]
     L.fi_isos_get_mask.argtypes = [vp, i, ctypes.POINTER(ctypes.c_int8)]
     L.fi_isos_get_full.argtypes = [vp, i, dp]

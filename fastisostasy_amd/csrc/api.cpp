@@ -48,6 +48,37 @@ int fail(int code, const char* fmt, ...) {
 std::mutex g_mu;
 hipStream_t g_stream = nullptr;
 
+}  // namespace
+
+// Scratch of the allocation-free update (fi_isos_update_noalloc): one array
+// per GPU, held until the process ends.  It carries the per-update working
+// memory of one domain -- the launch descriptor, the per-column corner terms,
+// the reduction partials, the two transform work arrays and the convolution
+// work array -- which fi_isos_update takes from the domain's own arena.  Host
+// code never uses the name `scratch` as an address: the device address comes
+// from hipGetSymbolAddress, per device.
+constexpr size_t FI_NOALLOC_SCRATCH_BYTES = (size_t)13 << 20;   // 13 MiB: N <= 512 in either precision
+static __device__ __align__(256) unsigned char scratch[FI_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+std::map<int, char*> g_scratch_ptr;   // device ordinal -> address of `scratch` on that device
+
+// Address of `scratch` on the current device (resolved once per device; g_mu held by the caller).
+int scratch_address_locked(char** out) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    auto it = g_scratch_ptr.find(dev);
+    if (it == g_scratch_ptr.end()) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return fail(FI_E_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+        it = g_scratch_ptr.emplace(dev, (char*)p).first;
+    }
+    *out = it->second;
+    return 0;
+}
+
 int ensure_device() {
     std::lock_guard<std::mutex> lk(g_mu);
     if (g_stream) return 0;
@@ -55,7 +86,9 @@ int ensure_device() {
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
         return fail(FI_E_HIP, "no HIP device available: the LV-ELVA library has no CPU path");
     HIPCHK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
-    return 0;
+    // set-up of the allocation-free update on this device
+    char* p = nullptr;
+    return scratch_address_locked(&p);
 }
 
 bool make_plan(int n, FftPlan& p) {
@@ -1287,6 +1320,51 @@ int fi_isos_update(fi_isos* h, const double* H_ice, double time, fi_bsl* b) {
     HIPCHK(hipMemcpyAsync(h->hm.H_in, H_ice, sizeof(double) * h->nx_ice * h->ny_ice, hipMemcpyHostToDevice, g_stream));
     std::vector<fi_isos*> hs{h};
     return run_update(hs, h->d_mem, h->shape, h->sb, time, true);
+}
+
+// isos_update without any device allocation: the domain's per-update working
+// memory (see `scratch`) lives in the library's static scratch array instead
+// of the handle's arena; the kernels receive its address as their launch
+// descriptor.  Same kernels, same results as fi_isos_update.  The array is
+// shared by every call on a GPU and keeps the last call's contents; nothing
+// in it needs to start at zero (every word is written before it is read
+// within one update), and two calls that overlap on one GPU race.
+int fi_isos_update_noalloc(fi_isos* h, const double* H_ice, double time, fi_bsl* b) {
+    if (!h || !H_ice || !b) return fail(FI_E_ARG, "NULL argument");
+    if (!h->state_init) return fail(FI_E_STATE, "isos_update called before isos_init_state");
+    char* base = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        int rc = scratch_address_locked(&base);
+        if (rc) return rc;
+    }
+    const int n = h->n;
+    const size_t nn = (size_t)n * n, rp = h->fp64 ? 8 : 4;
+    size_t off = 0;
+    auto take = [&](size_t bytes) { size_t o = off; off += align256(bytes); return o; };
+    const size_t o_mem = take(sizeof(Member)), o_cor = take((size_t)n * 4 * 8),
+                 o_part = take((size_t)h->part_blocks * 5 * 8), o_g1 = take(nn * rp), o_g2 = take(nn * rp);
+    const size_t o_cw = h->has_conv ? take(2 * (size_t)n * (size_t)(n + 1) * (h->fp64 ? 16 : 8)) : 0;
+    if (off > FI_NOALLOC_SCRATCH_BYTES)
+        return fail(FI_E_CAPACITY,
+                    "fi_isos_update_noalloc: N=%d (%s%s) needs %zu bytes of scratch, the library holds %zu "
+                    "(N <= 512); use fi_isos_update",
+                    n, h->fp64 ? "fp64" : "mixed", h->has_conv ? ", convolutions" : "", off,
+                    FI_NOALLOC_SCRATCH_BYTES);
+    int rc = bind_bsl(h, b);
+    if (rc) return rc;
+    Member M = h->hm;
+    M.corners = (double*)(base + o_cor);
+    M.partials = (double*)(base + o_part);
+    M.g1 = base + o_g1;
+    M.g2 = base + o_g2;
+    if (h->has_conv) M.conv_work = (double*)(base + o_cw);
+    Member* d_mem = (Member*)(base + o_mem);
+    HIPCHK(hipMemcpyAsync(d_mem, &M, sizeof(Member), hipMemcpyHostToDevice, g_stream));
+    HIPCHK(hipMemcpyAsync(h->hm.H_in, H_ice, sizeof(double) * h->nx_ice * h->ny_ice, hipMemcpyHostToDevice, g_stream));
+    std::vector<fi_isos*> hs{h};
+    // plain launches: capturing an attempt graph would allocate
+    return run_update(hs, d_mem, h->shape, h->sb, time, true, nullptr, false);
 }
 
 static const double* field_ptr(fi_isos* h, int f, bool& is_mask, int& bit) {

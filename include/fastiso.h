@@ -149,6 +149,15 @@ int fi_isos_init_state(fi_isos* h, const double* z_bed, const double* H_ice, dou
 /* isos_update (src/fastisostasy.f90:731-872); dwdt_corr is accepted but, as in
  * the reference (:755-759), unused */
 int fi_isos_update(fi_isos* h, const double* H_ice, double time, fi_bsl* b);
+/* No counterpart in the reference.  fi_isos_update for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): it calls no hipMalloc / hipFree; the
+ * per-update working memory comes from a static 13 MiB array per GPU instead
+ * of the handle's arena, so the handle (created beforehand) is the only
+ * allocation.  Same results as fi_isos_update.  Limit: N <= 512 in either
+ * precision, with or without convolutions; larger N -> FI_E_CAPACITY.  Calls
+ * on one GPU share the array: calls that overlap in time race. */
+int fi_isos_update_noalloc(fi_isos* h, const double* H_ice, double time, fi_bsl* b);
 /* out%<field> after the last update (cropstate2out, src/isos_utils.f90:386-405):
  * double values, masks as 0.0 / 1.0 */
 int fi_isos_get(fi_isos* h, int field, double* dst);
