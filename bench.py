@@ -31,6 +31,10 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 KERNEL_NAMES = {0: "k_stencil", 1: "k_row_fwd", 2: "k_col", 3: "k_row_inv"}
 FUSED_NAME = "kf_srow_fwd (stencil + forward row pass)"
+# the power-of-two attempt kernels of the mixed 256^2 batch, as rocprofv3 names them
+FAST_NAMES = {0: "kf_srow_fwd<float, 256, 256>", 2: "kf_col<float, 256, 16, true>",
+              3: "kf_row_inv<float, 256, 256, 1, 3>"}
+PROF_ALL = 99
 
 
 def measured_traffic(kernel_name, cells):
@@ -81,7 +85,21 @@ def percentile(xs, q):
     return float(np.percentile(np.asarray(xs), q)) if xs else None
 
 
-def run_batched(nmem, steps, warmup, precision, prof_kernel=None, timed=True):
+def autotune(n, precision, nmem, reps=3):
+    """fi_autotune: launch shapes for `nmem` domains of N x N picked from timed
+    trial updates; returns the choice (0 = the built-in one)."""
+    import ctypes
+    from fastisostasy_amd._lib import check, load
+    choice = (ctypes.c_int * 6)()
+    t0 = time.perf_counter()
+    check(load().fi_autotune(n, 0 if precision == "fp64" else 1, nmem, reps, choice))
+    keys = ("threads_row", "col_lines", "row_tiles", "subbatches", "generic_row_lines", "generic_col_lines")
+    out = {k: int(v) for k, v in zip(keys, choice)}
+    out["seconds"] = round(time.perf_counter() - t0, 2)
+    return out
+
+
+def run_batched(nmem, steps, warmup, precision, prof_kernel=None, timed=True, extra_steps=0):
     from fastisostasy_amd import isostasy as G
     from fastisostasy_amd._lib import check, load
     import ctypes
@@ -124,10 +142,17 @@ def run_batched(nmem, steps, warmup, precision, prof_kernel=None, timed=True):
             del os.environ["FI_SUBBATCHES"]
         else:
             os.environ["FI_SUBBATCHES"] = saved
-        ms, cnt = ctypes.c_double(), ctypes.c_longlong()
-        check(load().fi_prof_collect(ctypes.byref(ms), ctypes.byref(cnt)))
+        prof = {}
+        for k in ((0, 2, 3) if prof_kernel == PROF_ALL else (prof_kernel,)):
+            ms, cnt = ctypes.c_double(), ctypes.c_longlong()
+            check(load().fi_prof_collect_kernel(k, ctypes.byref(ms), ctypes.byref(cnt)))
+            prof[k] = (ms.value, cnt.value)
+        prof["fused"] = bool(load().fi_prof_fused())
         check(load().fi_prof_select(-1))
-        prof = (ms.value, cnt.value, bool(load().fi_prof_fused()))
+    for _ in range(extra_steps):
+        # untimed steps that bring a run to the end time of another
+        t += 10.0
+        batch.update(t, bsls)
     attempts, accepts = attempts_timed, accepts_timed
     w = [h.full("w") for h in hs]
     n = hs[0].n
@@ -163,6 +188,76 @@ def run_plain(members, steps, warmup):
         n = h.n
         h.close()
     return dict(wall=total, n=n)
+
+
+def run_single(precision, updates=40, warmup=5, member=0, tune=True):
+    """The single-domain drop-in path: one 256^2 ensemble member (N = 256),
+    isos_update by isos_update, after the autotuner picked its launch shape
+    for one domain; per-update wall times and attempts."""
+    from fastisostasy_amd import isostasy as G
+    from fastisostasy_amd import workloads as W
+    shape = autotune(256, precision, 1) if tune else None
+    wl = W.ensemble_member(member)
+    h = G.IsosClass(G.make_params(precision=precision, **wl["params"]), wl["nx"], wl["nx"], wl["dx"],
+                    he_field=wl["he_field"], eta_field=wl["eta_field"])
+    b = G.bsl_init("const", 0.0)
+    h.init_state(wl["z_bed"], np.zeros_like(wl["H_ice"]), 0.0, b)
+    t = 0.0
+    for _ in range(warmup):
+        t += 10.0
+        h.update(wl["H_ice"], t, b)
+    a0 = h.scalar("attempts")
+    lat = []
+    for _ in range(updates):
+        t += 10.0
+        s0 = time.perf_counter()
+        h.update(wl["H_ice"], t, b)
+        lat.append(time.perf_counter() - s0)
+    att = (h.scalar("attempts") - a0) / updates
+    h.close()
+    return {"p50_ms": round(1e3 * percentile(lat, 50), 3), "p99_ms": round(1e3 * percentile(lat, 99), 3),
+            "attempts_per_update": round(att, 2), "us_per_attempt": round(1e6 * percentile(lat, 50) / att, 2),
+            "launch_shape": shape}
+
+
+def kernels_only(name, updates):
+    """Untimed updates of one workload, for a profiler's kernel statistics:
+    lv_rheology (N = 512, mixed, elastic convolution), sealevel (384^2 ->
+    N = 512, mixed, elastic and sea-surface convolutions every update), n384
+    (one 384^2 serving domain, mixed-radix kernels), single256 (one 256^2
+    ensemble member, the single-domain path)."""
+    from fastisostasy_amd import isostasy as G
+    from fastisostasy_amd import workloads as W
+    if name == "lv_rheology":
+        wl, dt, H_of_t = W.lv_rheology(), 0.5, None
+    elif name == "sealevel":
+        wl, dt = W.sealevel_cycle(), 10.0
+        H_of_t = lambda t: wl["H_max"] * float(W.sawtooth(t))
+    elif name == "n384":
+        wl, dt, H_of_t = W.ensemble_member(0, nx=384, seed=5555), 10.0, None
+    elif name == "single256":
+        wl, dt, H_of_t = W.ensemble_member(0), 10.0, None
+    else:
+        raise SystemExit(f"unknown workload {name}")
+    H = (lambda t: wl["H_ice"]) if H_of_t is None else H_of_t
+    h = G.IsosClass(G.make_params(precision="mixed", **wl["params"]), wl["nx"], wl["nx"], wl["dx"],
+                    he_field=wl.get("he_field"), eta_field=wl.get("eta_field"))
+    b = G.bsl_init(wl.get("bsl_method", "const"), 0.0)
+    t0 = 89.9e3 if name == "sealevel" else 0.0
+    if name == "sealevel":
+        h.init_ref(wl["z_bed"], np.zeros_like(wl["z_bed"]))
+    h.init_state(wl["z_bed"], H(t0) if name == "sealevel" else np.zeros_like(wl["z_bed"]), t0, b)
+    t = t0
+    s0 = time.perf_counter()
+    for _ in range(updates):
+        t += dt
+        G.bsl_update(b, t)
+        h.update(H(t), t, b)
+    wall = time.perf_counter() - s0
+    out = {"workload": name, "n": h.n, "updates": updates, "wall_s": round(wall, 4),
+           "attempts": h.scalar("attempts")}
+    h.close()
+    return out
 
 
 def run_serving(requests, n_domains, capacity_seconds):
@@ -298,24 +393,47 @@ def calibrate_copy(mib):
     print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": 5}))
 
 
-def roofline(res, args, cells, n):
-    """Roofline object of the profiled kernel: algorithmic bytes per launch over
-    the event-measured average launch time, and the calibrated HBM traffic."""
-    if not (res["prof"] and res["prof"][1] > 0):
+def kernel_roofline(prof, k, cells, n, fused):
+    """Algorithmic bytes per launch of attempt kernel k over its event-measured
+    average launch time, and the calibrated HBM traffic where measured."""
+    ms, cnt = prof[k]
+    if cnt <= 0:
         return None
-    ms, cnt, fused = res["prof"]
     avg_s = ms / cnt * 1e-3
-    bytes_per_launch = kernel_bytes_per_cell(args.roofline_kernel, 4, fused) * cells
-    achieved = bytes_per_launch / avg_s / 1e9
-    name = FUSED_NAME if (fused and args.roofline_kernel == 0) else KERNEL_NAMES[args.roofline_kernel]
-    traffic = None
-    if fused and args.roofline_kernel == 0 and n == 256:
-        traffic = measured_traffic("kf_srow_fwd<float, 256, 256>", cells)
-    return {"kernel": name, "bound": "hbm", "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic, "avg_launch_us": round(avg_s * 1e6, 2), "launches": cnt,
-            "bytes_per_launch": bytes_per_launch,
-            "copy_GBs": round(copy_bandwidth(), 1)}
+    nbytes = kernel_bytes_per_cell(k, 4, fused) * cells
+    achieved = nbytes / avg_s / 1e9
+    name = (FUSED_NAME if k == 0 else FAST_NAMES[k].split("<")[0]) if fused else KERNEL_NAMES[k]
+    traffic = measured_traffic(FAST_NAMES[k], cells) if (fused and n == 256 and k in FAST_NAMES) else None
+    return {"kernel": name, "bytes_per_launch": nbytes, "avg_launch_us": round(avg_s * 1e6, 2), "launches": cnt,
+            "achieved": round(achieved, 1), "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic}
+
+
+def roofline(res, args, cells, n):
+    """Roofline object: the dominant kernel (the fused stencil + forward row
+    pass) at the top level, every attempt kernel in `kernels`, and the three
+    transform passes together in `fft_passes` (their bytes over the sum of
+    their average launch times)."""
+    prof = res["prof"]
+    if not prof:
+        return None
+    fused = prof["fused"]
+    ks = {k: kernel_roofline(prof, k, cells, n, fused) for k in prof if k != "fused"}
+    ks = {k: v for k, v in ks.items() if v}
+    if not ks:
+        return None
+    top = ks.get(0) or next(iter(ks.values()))
+    out = {"kernel": top["kernel"], "bound": "hbm", "achieved": top["achieved"], "peak": HBM_PEAK_GBS,
+           "unit": "GB/s", "frac": top["frac"], "traffic": top["traffic"], "avg_launch_us": top["avg_launch_us"],
+           "launches": top["launches"], "bytes_per_launch": top["bytes_per_launch"]}
+    if len(ks) > 1:
+        out["kernels"] = [ks[k] for k in sorted(ks)]
+        tot_b = sum(v["bytes_per_launch"] for v in ks.values())
+        tot_us = sum(v["avg_launch_us"] for v in ks.values())
+        agg = tot_b / (tot_us * 1e-6) / 1e9
+        out["fft_passes"] = {"bytes_per_attempt": tot_b, "us_per_attempt": round(tot_us, 2),
+                             "achieved": round(agg, 1), "frac": round(agg / HBM_PEAK_GBS, 4)}
+    out["copy_GBs"] = round(copy_bandwidth(), 1)
+    return out
 
 
 def main():
@@ -331,11 +449,18 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=30)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-workers", type=int, default=16)
-    ap.add_argument("--roofline-kernel", type=int, default=0)
+    ap.add_argument("--roofline-kernel", type=int, default=PROF_ALL,
+                    help="attempt kernel timed with events: 0 fused stencil + row pass, 2 column pass, "
+                         "3 inverse row pass, 99 all three (default), -1 none")
     ap.add_argument("--serve-requests", type=int, default=3000)
     ap.add_argument("--serve-domains", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the built-in launch shapes instead of timing candidates first")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--kernels-only", default=None,
+                    help="untimed updates of one workload (lv_rheology, sealevel, n384, single256) for profiling runs")
+    ap.add_argument("--updates", type=int, default=10)
     ap.add_argument("--roofline-only", action="store_true",
                     help="only the event-timed kernel pass (whole batch per launch), for profiling runs")
     args = ap.parse_args()
@@ -345,14 +470,21 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     from fastisostasy_amd import _lib
+    if args.kernels_only:
+        _lib.load()
+        print(json.dumps(kernels_only(args.kernels_only, args.updates)), flush=True)
+        return
     if args.roofline_only:
         # every launch of the profiled kernel covers the whole batch, so a
         # profiler's per-kernel statistics describe the launch measured here
         os.environ["FI_SUBBATCHES"] = "1"
         _lib.load()
+        if not args.no_autotune:
+            tuned = autotune(256, "mixed", args.members)
+            print(f"# autotune (N=256, mixed, {args.members} members): {tuned}", file=sys.stderr, flush=True)
         res = run_batched(args.members, args.steps, args.warmup, "mixed",
-                          prof_kernel=max(args.roofline_kernel, 0), timed=False)
-        args.roofline_kernel = max(args.roofline_kernel, 0)
+                          prof_kernel=args.roofline_kernel if args.roofline_kernel >= 0 else PROF_ALL,
+                          timed=False)
         n = res["n"]
         print(json.dumps({"roofline": roofline(res, args, args.members * n * n, n),
                           "config": {"workload": f"ensemble {args.members} x 256^2 ice grid (N={n}), mixed",
@@ -360,6 +492,10 @@ def main():
         return
     _lib.load()
 
+    tuned = None
+    if not args.no_autotune:
+        tuned = autotune(256, "mixed", args.members)
+        print(f"# autotune (N=256, mixed, {args.members} members): {tuned}", file=sys.stderr, flush=True)
     res = run_batched(args.members, args.steps, args.warmup, "mixed",
                       prof_kernel=args.roofline_kernel if args.roofline_kernel >= 0 else None)
     n = res["n"]
@@ -371,8 +507,10 @@ def main():
 
     max_err = rms_err = None
     if not args.no_check:
+        # the same coupling steps as the mixed run, its timing pass included
         ref = run_batched(args.members, args.steps, args.warmup, "fp64",
-                          prof_kernel=args.roofline_kernel if args.roofline_kernel >= 0 else None)
+                          extra_steps=args.steps if res["prof"] else 0)
+        assert ref["t_end"] == res["t_end"], (ref["t_end"], res["t_end"])
         max_err = max(float(np.abs(a - b).max()) for a, b in zip(res["w"], ref["w"]))
         rms_err = float(np.sqrt(np.mean([np.mean((a - b) ** 2) for a, b in zip(res["w"], ref["w"])])))
         print(f"# fp64 batch for the error check: {ref['wall']:.3f} s, max |dw| = {max_err:.3e} m",
@@ -384,6 +522,10 @@ def main():
           file=sys.stderr, flush=True)
 
     roof = roofline(res, args, cells, n)
+    single = {"workload": "one 256^2 ensemble member (N=256), 10-yr isos_update",
+              "fp64": run_single("fp64", tune=not args.no_autotune),
+              "mixed": run_single("mixed", tune=not args.no_autotune)}
+    print(f"# single domain: {single}", file=sys.stderr, flush=True)
 
     serve = None
     if args.serve_requests > 0:
@@ -418,7 +560,7 @@ def main():
         "data": "synthetic (seeded ensemble members 1000+m)",
         "config": {"workload": f"ensemble {args.members} x 256^2 ice grid, dx 16 km, no pad (N={n}), "
                                "coupling step 10 yr, dt_max 0.2 yr",
-                   "members": args.members, "n": n, "precision": "mixed"},
+                   "members": args.members, "n": n, "precision": "mixed", "launch_shape": tuned},
         "plain_fp64": {"value": round(plain_value, 1), "members": args.plain_members,
                        "steps": args.plain_steps},
         "speedup_vs_plain_fp64": round(value / plain_value, 2),
@@ -426,6 +568,7 @@ def main():
         "rms_err_vs_fp64_m": rms_err,
         "step_latency_ms": ({"p50": serve["p50_ms"], "p99": serve["p99_ms"], "of": "serving requests"} if serve
                             else None),
+        "single_update_ms": single,
         "batch_step_ms": {"p50": round(1e3 * percentile(res["lat"], 50), 4),
                           "p99": round(1e3 * percentile(res["lat"], 99), 4)},
         "serving": serve,

@@ -20,7 +20,7 @@ FIELDS = ["z_bed", "z_ss", "dz_ss", "w", "we", "dwdt", "canom_full", "Hice",
 FIELD_ID = {n: i for i, n in enumerate(FIELDS)}
 OUT_FIELDS = FIELDS[:11]
 SCALARS = ["n", "t", "dt", "bsl", "deltaV", "V_af", "V_den", "attempts", "accepts", "D_lith11", "err", "tol",
-           "hbm_bytes", "status"]
+           "hbm_bytes", "status", "nx_ice", "ny_ice"]
 SCALAR_ID = {n: i for i, n in enumerate(SCALARS)}
 
 # every entry point of include/fastiso.h
@@ -31,7 +31,8 @@ EXPORTS = ["fi_last_error", "fi_version", "fi_padded_size", "fi_isos_create", "f
            "fi_bsl_restart_read", "fi_isos_destroy", "fi_bsl_create", "fi_bsl_create_series",
            "fi_bsl_read_series", "fi_bsl_update", "fi_bsl_get",
            "fi_bsl_destroy", "fi_batch_create", "fi_batch_update", "fi_batch_destroy", "fi_sync",
-           "fi_prof_select", "fi_prof_collect", "fi_prof_fused", "fi_bench_copy",
+           "fi_prof_select", "fi_prof_collect", "fi_prof_collect_kernel", "fi_prof_fused", "fi_bench_copy",
+           "fi_isos_write_init", "fi_isos_write_step", "fi_bsl_write_init", "fi_bsl_write_step", "fi_autotune",
            "fi_server_create", "fi_server_admit", "fi_server_release", "fi_server_submit", "fi_server_pending",
            "fi_server_run", "fi_server_completed", "fi_server_failed", "fi_server_stats", "fi_server_destroy"]
 
@@ -119,6 +120,12 @@ def load():
     L.fi_batch_destroy.argtypes = [vp]
     L.fi_prof_select.argtypes = [i]
     L.fi_prof_collect.argtypes = [dp, ctypes.POINTER(ctypes.c_longlong)]
+    L.fi_prof_collect_kernel.argtypes = [i, dp, ctypes.POINTER(ctypes.c_longlong)]
+    L.fi_isos_write_init.argtypes = [vp, dp, dp, ctypes.c_char_p, d]
+    L.fi_isos_write_step.argtypes = [vp, ctypes.c_char_p, d, dp, dp]
+    L.fi_bsl_write_init.argtypes = [vp, ctypes.c_char_p, d]
+    L.fi_bsl_write_step.argtypes = [vp, ctypes.c_char_p, d]
+    L.fi_autotune.argtypes = [i, i, i, i, ctypes.POINTER(i)]
     L.fi_prof_fused.argtypes = []
     L.fi_bench_copy.argtypes = [d, i, dp]
     ll = ctypes.c_longlong

@@ -9,7 +9,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfastiso.so")
-SOURCES = ["rhs.hip", "rhs_fast.hip", "sealevel.hip", "init.hip", "conv.hip", "kelvin.cpp", "rheogrid.cpp", "api.cpp"]
+SOURCES = ["rhs.hip", "rhs_fast.hip", "sealevel.hip", "init.hip", "conv.hip", "kelvin.cpp", "rheogrid.cpp", "api.cpp",
+           "output.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
@@ -54,17 +55,19 @@ FORTRAN_DIR = os.path.join(HERE, "fortran")
 FLANG = os.environ.get("FLANG", "/opt/rocm/bin/amdflang")
 
 
-def build_fortran(verbose=False):
+def build_fortran(verbose=False, single=False):
     """Compile the iso_c_binding module (fortran/fastisostasy.f90) and the
-    Fortran caller drive_disc against libfastiso.so.  Returns the driver path
-    or None when amdflang is not installed."""
+    Fortran caller drive_disc against libfastiso.so, with wp = dp, or with
+    wp = sp (the reference's default precision) into drive_disc_sp when
+    `single`.  Returns the driver path or None when amdflang is not installed."""
     if not os.path.exists(FLANG):
         return None
-    mod = os.path.join(HERE, "_build", "fortran")
+    mod = os.path.join(HERE, "_build", "fortran_sp" if single else "fortran")
     os.makedirs(mod, exist_ok=True)
     obj = os.path.join(mod, "fastisostasy.o")
-    exe = os.path.join(FORTRAN_DIR, "drive_disc")
-    cmds = [[FLANG, "-O2", "-c", os.path.join(FORTRAN_DIR, "fastisostasy.f90"), "-J", mod, "-o", obj],
+    exe = os.path.join(FORTRAN_DIR, "drive_disc_sp" if single else "drive_disc")
+    defs = ["-cpp"] + (["-DFI_WP_SP"] if single else [])
+    cmds = [[FLANG, "-O2"] + defs + ["-c", os.path.join(FORTRAN_DIR, "fastisostasy.f90"), "-J", mod, "-o", obj],
             [FLANG, "-O2", os.path.join(FORTRAN_DIR, "drive_disc.f90"), "-J", mod, obj, "-L" + HERE,
              "-lfastiso", "-Wl,-rpath,$ORIGIN/..", "-o", exe]]
     for cmd in cmds:
@@ -77,3 +80,4 @@ def build_fortran(verbose=False):
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
     print(build_fortran(verbose="-v" in sys.argv))
+    print(build_fortran(verbose="-v" in sys.argv, single=True))

@@ -13,6 +13,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <algorithm>
+#include <chrono>
 #include <deque>
 #include <map>
 #include <mutex>
@@ -38,6 +39,17 @@ int fail(int code, const char* fmt, ...) {
     g_err = buf;
     return code;
 }
+
+}  // namespace
+
+namespace fi {
+int set_error(int code, const char* msg) {
+    g_err = msg;
+    return code;
+}
+}  // namespace fi
+
+namespace {
 
 #define HIPCHK(x)                                                                          \
     do {                                                                                   \
@@ -138,8 +150,29 @@ int fft_threads(const FftPlan& p, int lines, int& threads) {
     return 0;
 }
 
+// Launch shapes picked by the autotuner (fi_autotune) per (N, precision,
+// group size), and the shape under trial while it times candidates (this
+// thread only).  0 fields keep the built-in choice.
+struct TunedShape {
+    int threads_row = 0, C = 0, row_tiles = 0, K = 0;   // power-of-two kernels; K concurrent sub-batches
+    int gL = 0, gC = 0;                                 // mixed-radix kernels: line pairs per row / column block
+};
+std::map<std::tuple<int, int, int>, TunedShape> g_tuned;
+thread_local const TunedShape* t_trial = nullptr;
+
+bool tuned_for(int n, bool fp64, int nmem, TunedShape& out) {
+    if (t_trial) { out = *t_trial; return true; }
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_tuned.find(std::make_tuple(n, (int)fp64, nmem));
+    if (it == g_tuned.end()) return false;
+    out = it->second;
+    return true;
+}
+
 // Row / column blocking for a batch of `nmem` domains of size n.
 int make_shape(int n, int nmem, bool fp64, bool uniform, RhsShape& s) {
+    TunedShape ts;
+    const bool tuned = tuned_for(n, fp64, nmem, ts);
     if (!make_plan(n, s.plan))
         return fail(FI_E_UNSUPPORTED, "padded size N=%d has a prime factor > 7; choose pad so that N is 2^a 3^b 5^c 7^d", n);
     s.n = n;
@@ -166,13 +199,14 @@ int make_shape(int n, int nmem, bool fp64, bool uniform, RhsShape& s) {
         }
         // The fused stencil pass holds its row window in registers per owned
         // column, so several columns per thread spill; with one column per
-        // thread (THREADS = N, possible up to N = 256) even a single domain
+        // thread (THREADS = N, possible up to N = 256, and N = 512 in the
+        // mixed mode with 68 KiB of LDS per block) even a single domain
         // runs 6-15% faster per update than with more, smaller blocks.
-        if (uniform && n <= 256 && row_ok(n)) best_t = n;
-        // FI_ROW_THREADS = 64 | 128 | 256 overrides the row block size (tuning, tests)
+        if (uniform && (n <= 256 || (n == 512 && !fp64)) && row_ok(n)) best_t = n;
+        // FI_ROW_THREADS = 64 | 128 | 256 (| 512 for mixed N = 512) overrides the row block size (tuning, tests)
         if (const char* e = std::getenv("FI_ROW_THREADS")) {
             const int th = std::atoi(e);
-            if ((th == 64 || th == 128 || th == 256) && row_ok(th)) best_t = th;
+            if ((th == 64 || th == 128 || th == 256 || (th == 512 && n == 512 && !fp64)) && row_ok(th)) best_t = th;
         }
         int best_c = 0;
         auto col_ok = [&](int C) {
@@ -188,10 +222,14 @@ int make_shape(int n, int nmem, bool fp64, bool uniform, RhsShape& s) {
             const int C = std::atoi(e);
             if ((C == 1 || C == 2 || C == 4 || C == 8 || C == 16) && col_ok(C)) best_c = C;
         }
+        if (tuned && ts.threads_row && row_ok(ts.threads_row)) best_t = ts.threads_row;
+        if (tuned && ts.C && col_ok(ts.C)) best_c = ts.C;
         if (best_t && best_c) {
             s.fused = uniform && !std::getenv("FI_NO_FUSED_STENCIL");
             s.row_tiles = 1;
             if (const char* e = std::getenv("FI_ROW_TILES")) s.row_tiles = std::atoi(e) == 2 ? 2 : 1;
+            if (tuned && (ts.row_tiles == 1 || ts.row_tiles == 2) && n % (4 * (best_t / tpl)) == 0)
+                s.row_tiles = ts.row_tiles;
             s.threads_row = best_t;
             s.L = best_t / tpl;
             s.C = best_c;
@@ -225,6 +263,11 @@ int make_shape(int n, int nmem, bool fp64, bool uniform, RhsShape& s) {
     if (n >= 256) {
         rb = (int)std::max<long long>(1, (96 + nmem - 1) / nmem);
         cb = (int)std::max<long long>(1, (48 + nmem - 1) / nmem);
+    }
+    if (tuned && ts.gL >= 1 && ts.gC >= 1) {
+        lmax = std::min(lmax, ts.gL);
+        cmax = std::min(cmax, ts.gC);
+        rb = cb = 1;
     }
     // FI_GEN_LINES = "L,C" asks for L / C line pairs per row / column block
     // where the LDS and the divisibility allow them (tuning)
@@ -405,17 +448,12 @@ int gather(Member* d_mem, int nmem, SumBuf& sb) {
     return 0;
 }
 
-// Worker streams: the process uses at most 1 + MAX_SUB streams (the library
-// stream and these three, default priority), matching the 4 hardware queues
-// HIP gives a process by default; more streams would share queues and
-// serialise behind each other by accident.  They carry the concurrent
-// sub-batches of the attempt loop and the concurrent groups of a serving
-// round (never both at once: sub-batches run only on the library stream).
-// One launch of an attempt kernel over a large batch is a single occupancy
-// round in which all blocks load, then compute, in step; sub-batches on
-// separate streams drift apart, so one sub-batch's transforms overlap
-// another's loads.  Results do not depend on the split (reductions are fixed
-// by N).
+// Concurrent sub-batches of the attempt loop (at most three streams besides
+// the library stream, default priority).  One launch of an attempt kernel
+// over a large batch is a single occupancy round in which all blocks load,
+// then compute, in step; sub-batches on separate streams drift apart, so one
+// sub-batch's transforms overlap another's loads.  Results do not depend on
+// the split (reductions are fixed by N).
 constexpr int MAX_SUB = 3;
 struct SubStreams {
     hipStream_t s[MAX_SUB] = {};
@@ -435,16 +473,31 @@ int ensure_sub(int k) {
     return 0;
 }
 
-// number of sub-batches for a batch of nmem domains (FI_SUBBATCHES overrides)
-int sub_count(int nmem) {
+// number of sub-batches for a batch of nmem domains (the autotuner's choice,
+// else 2 from 32 domains; FI_SUBBATCHES overrides)
+int sub_count(int n, bool fp64, int nmem) {
     int k = nmem >= 32 ? 2 : 1;
+    TunedShape ts;
+    if (tuned_for(n, fp64, nmem, ts) && ts.K >= 1) k = ts.K;
     if (const char* e = std::getenv("FI_SUBBATCHES")) k = std::atoi(e);
     return std::max(1, std::min({k, MAX_SUB, nmem}));
 }
 
 std::map<std::tuple<int, int, int, int, int>, RhsShape> g_sub_shapes;
 
-int sub_shape(int n, int nsub, bool fp64, bool uniform, int dt_method, const RhsShape*& out) {
+thread_local RhsShape t_trial_sub[MAX_SUB];
+
+int sub_shape(int n, int nsub, bool fp64, bool uniform, int dt_method, const RhsShape*& out, int k) {
+    if (t_trial) {
+        // a shape under trial is not cached
+        RhsShape& s = t_trial_sub[k];
+        s = RhsShape{};
+        int rc = make_shape(n, nsub, fp64, uniform, s);
+        if (rc) return rc;
+        s.dt_method = dt_method;
+        out = &s;
+        return 0;
+    }
     std::lock_guard<std::mutex> lk(g_mu);
     auto key = std::make_tuple(n, nsub, (int)fp64, (int)uniform, dt_method);
     auto it = g_sub_shapes.find(key);
@@ -509,6 +562,8 @@ int attempt_graph(const Member* d, int nmem, const RhsShape& s, hipStream_t st, 
     if ((int)g_graphs.size() >= GRAPH_CACHE) {
         auto lru = std::min_element(g_graphs.begin(), g_graphs.end(),
                                     [](const GraphEntry& a, const GraphEntry& b) { return a.use < b.use; });
+        // a launch of the evicted exec may still be queued: let its stream drain first
+        (void)hipStreamSynchronize(lru->st);
         (void)hipGraphExecDestroy(lru->exec);
         g_graphs.erase(lru);
     }
@@ -562,6 +617,7 @@ struct UpdateJob {
     const double* times = nullptr;
     double time = 0.0;
     bool stepping = true, graphs = true;
+    bool subbatch = true;        // may split into concurrent sub-batches (library stream only)
     hipStream_t st = nullptr;
     int K = 1, ahead = 1;
     long long total = 0;
@@ -633,13 +689,13 @@ int job_begin(UpdateJob& j) {
     if (need > 0.0) need += (double)first_round_margin();
     j.ahead = (int)std::min(256.0, std::max(1.0, need));
     // concurrent sub-batches only on the library stream (their streams are shared)
-    j.K = shape.method == 3 && j.st == g_stream ? sub_count(nmem) : 1;
+    j.K = shape.method == 3 && j.st == g_stream && j.subbatch ? sub_count(n, h0.fp64, nmem) : 1;
     if (j.K > 1) {
         int rc = ensure_sub(j.K);
         if (rc) return rc;
         for (int k = 0; k <= j.K; ++k) j.sb_lo[k] = (int)((long long)nmem * k / j.K);
         for (int k = 0; k < j.K; ++k) {
-            rc = sub_shape(n, j.sb_lo[k + 1] - j.sb_lo[k], h0.fp64, h0.uniform, shape.dt_method, j.sshape[k]);
+            rc = sub_shape(n, j.sb_lo[k + 1] - j.sb_lo[k], h0.fp64, h0.uniform, shape.dt_method, j.sshape[k], k);
             if (rc) return rc;
         }
     }
@@ -1638,6 +1694,8 @@ int fi_isos_get_scalar(fi_isos* h, int which, double* out) {
     case FI_S_TOL: *out = c.tol; break;
     case FI_S_HBM_BYTES: *out = (double)h->arena_bytes; break;
     case FI_S_STATUS: *out = (double)h->last_status; break;
+    case FI_S_NX_ICE: *out = h->nx_ice; break;
+    case FI_S_NY_ICE: *out = h->ny_ice; break;
     default: return fail(FI_E_ARG, "unknown scalar %d", which);
     }
     return 0;
@@ -1743,8 +1801,13 @@ struct fi_server {
     // (n, fp64, uniform, nmem, method, dt_method)
     std::map<std::tuple<int, int, int, int, int, int>, RhsShape> shapes;
     long long rounds = 0, groups = 0, served = 0;
-    // groups of one round run concurrently on the worker streams, round-robin
-    static constexpr int NSTREAM = MAX_SUB;
+    // Groups of one round run concurrently, round-robin on the server's own
+    // streams (default priority).  Eight streams share the 4 hardware queues;
+    // capping the process at one stream per queue was measured slower on the
+    // serving mix (5044-5277 against 5806 requests/s of batched capacity,
+    // p99 54-59 against 45 ms at 0.8x capacity).
+    static constexpr int NSTREAM = 8;
+    hipStream_t st[NSTREAM] = {};
     std::vector<hipEvent_t> ev;   // one per concurrent group: its last readback
     hipEvent_t start = nullptr;   // library-stream work queued before the round
     std::vector<long long> failed;   // tags of requests whose update failed its checks
@@ -1754,9 +1817,9 @@ namespace {
 
 // every stream the server launches on, drained
 void server_drain(fi_server* s) {
-    (void)s;
     (void)hipStreamSynchronize(g_stream);
-    for (int k = 0; k < g_sub.n; ++k) (void)hipStreamSynchronize(g_sub.s[k]);
+    for (hipStream_t t : s->st)
+        if (t) (void)hipStreamSynchronize(t);
 }
 
 int server_reserve(fi_server* s, int n) {
@@ -1879,8 +1942,8 @@ int fi_server_run(fi_server* s, int* completed) {
             }
     }
     if (concurrent) {
-        rc = ensure_sub(fi_server::NSTREAM);
-        if (rc) return rc;
+        for (int k = 0; k < fi_server::NSTREAM; ++k)
+            if (!s->st[k]) HIPCHK(hipStreamCreateWithFlags(&s->st[k], hipStreamNonBlocking));
         while (s->ev.size() < grp.size()) {
             hipEvent_t e = nullptr;
             HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -1889,7 +1952,7 @@ int fi_server_run(fi_server* s, int* completed) {
         // the round starts after everything already queued on the library stream
         if (!s->start) HIPCHK(hipEventCreateWithFlags(&s->start, hipEventDisableTiming));
         HIPCHK(hipEventRecord(s->start, g_stream));
-        for (int k = 0; k < fi_server::NSTREAM; ++k) HIPCHK(hipStreamWaitEvent(g_sub.s[k], s->start, 0));
+        for (hipStream_t t : s->st) HIPCHK(hipStreamWaitEvent(t, s->start, 0));
     }
     std::vector<UpdateJob> jobs(grp.size());
     std::vector<std::vector<double>> tvec(grp.size());
@@ -1918,7 +1981,7 @@ int fi_server_run(fi_server* s, int* completed) {
     int base = 0;
     for (size_t gi = 0; gi < grp.size(); ++gi) {
         const auto& g = grp[gi];
-        hipStream_t gst = concurrent ? g_sub.s[gi % fi_server::NSTREAM] : g_stream;
+        hipStream_t gst = concurrent ? s->st[gi % fi_server::NSTREAM] : g_stream;
         const int nm = (int)g.size();
         std::vector<fi_isos*> hs;
         std::vector<double> times;
@@ -1974,6 +2037,7 @@ int fi_server_run(fi_server* s, int* completed) {
             j.times = tvec[gi].data();
             j.stepping = true;
             j.graphs = false;
+            j.subbatch = false;
             j.st = gst;
             rc = job_begin(j);
             if (!rc) rc = job_round(j);
@@ -2061,6 +2125,8 @@ int fi_server_destroy(fi_server* s) {
     if (s->d_mem) (void)hipFree(s->d_mem);
     if (s->h_stage) (void)hipHostFree(s->h_stage);
     sum_free(s->sb);
+    for (hipStream_t t : s->st)
+        if (t) (void)hipStreamDestroy(t);
     for (hipEvent_t e : s->ev) (void)hipEventDestroy(e);
     if (s->start) (void)hipEventDestroy(s->start);
     delete s;
@@ -2078,6 +2144,12 @@ int fi_prof_select(int kernel) {
 int fi_prof_collect(double* total_ms, long long* launches) {
     if (!total_ms || !launches) return fail(FI_E_ARG, "NULL argument");
     if (prof_collect(total_ms, launches)) return fail(FI_E_HIP, "event timing failed");
+    return 0;
+}
+
+int fi_prof_collect_kernel(int kernel, double* total_ms, long long* launches) {
+    if (!total_ms || !launches) return fail(FI_E_ARG, "NULL argument");
+    if (prof_collect_kernel(kernel, total_ms, launches)) return fail(FI_E_HIP, "event timing failed");
     return 0;
 }
 
@@ -2107,6 +2179,173 @@ int fi_bench_copy(double mib, int reps, double* gbs) {
     (void)hipFree(a);
     (void)hipFree(b);
     *gbs = 2.0 * (double)bytes * reps / (ms * 1e-3) / 1e9;
+    return 0;
+}
+
+// ------------------------------------------------------------------ autotuner
+// Launch shapes of the attempt kernels for a group of `nmem` domains of
+// size N, picked from timed trial updates on synthetic domains (uniform
+// rheology, a dome load) and kept for the process: every batch, sub-batch and
+// serving group of that (N, precision, size) created afterwards uses them.
+// Results do not depend on the shape (line pairings and reduction orders are
+// fixed by N), only the time does.
+}  // extern "C"
+
+namespace {
+
+fi_params synthetic_params(int precision) {
+    fi_params p{};
+    p.method = 3;
+    p.dt_diagnostics = 10.0;
+    p.mantle = 0; p.lithosphere = 0; p.layering = 1; p.lumping = 5;
+    p.viscosity_scaling_method = 2; p.viscosity_scaling = 1.0;
+    p.nl = 1; p.zl[0] = 100e3; p.viscosities[0] = 1e21; p.eta_ref = 1e21;
+    p.rho_water = 1000.0; p.rho_ice = 910.0; p.rho_seawater = 1028.0; p.rho_uppermantle = 3300.0;
+    p.E = 66.0; p.nu = 0.28; p.g = 9.81; p.r_earth = 6.378e6; p.m_earth = 5.972e24;
+    p.dt_method = 3; p.atol = 1e-2; p.rtol = 1e-4; p.dt_min = 1e-10; p.dt_max = 0.2; p.dt_init = 1.0; p.q = 1.0;
+    p.precision = precision;
+    p.tau = 3000.0;
+    return p;
+}
+
+// candidate shapes, the built-in choice first
+std::vector<TunedShape> tune_candidates(int n, int nmem) {
+    std::vector<TunedShape> c;
+    c.push_back(TunedShape{});
+    const bool p2 = (n & (n - 1)) == 0 && n >= 64 && n <= 1024;
+    if (p2) {
+        for (int th : {64, 128, 256, 512})
+            for (int C : {2, 4, 8, 16})
+                for (int rt : {1, 2})
+                    for (int K = 1; K <= std::min(MAX_SUB, nmem); ++K) {
+                        if (th == 512 && n != 512) continue;   // 512-thread row blocks exist for N = 512
+                        TunedShape t;
+                        t.threads_row = th; t.C = C; t.row_tiles = rt; t.K = K;
+                        c.push_back(t);
+                    }
+    } else {
+        for (int L : {1, 2, 4, 8})
+            for (int C : {1, 2, 4, 8, 16})
+                for (int K = 1; K <= std::min(MAX_SUB, nmem); ++K) {
+                    TunedShape t;
+                    t.gL = L; t.gC = C; t.K = K;
+                    c.push_back(t);
+                }
+    }
+    return c;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fi_autotune(int n, int precision, int nmem, int reps, int* choice) {
+    int rc = ensure_device();
+    if (rc) return rc;
+    if (n < 16 || n % 2 || nmem < 1 || nmem > 1024 || reps < 1 || (precision != FI_FP64 && precision != FI_MIXED))
+        return fail(FI_E_ARG, "fi_autotune: bad arguments");
+    const bool fp64 = precision == FI_FP64;
+    const fi_params p = synthetic_params(precision);
+    const double dx = 16e3;
+    std::vector<fi_isos*> hs;
+    std::vector<fi_bsl*> bs;
+    auto cleanup = [&]() {
+        for (fi_isos* h : hs) fi_isos_destroy(h);
+        for (fi_bsl* b : bs) fi_bsl_destroy(b);
+    };
+    const size_t nc = (size_t)n * n;
+    std::vector<double> z(nc, 1e6), H0(nc, 0.0), H(nc, 0.0);
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) {
+            const double x = (i - 0.5 * (n - 1)) * dx, y = (j - 0.5 * (n - 1)) * dx, R = 0.3 * n * dx;
+            H[(size_t)j * n + i] = std::max(0.0, 2000.0 * (1.0 - (x * x + y * y) / (R * R)));
+        }
+    for (int m = 0; m < nmem; ++m) {
+        fi_isos* h = nullptr;
+        fi_bsl* b = nullptr;
+        rc = fi_bsl_create(0, 0.0, 3.625e14, 0, nullptr, nullptr, 0.0, &b);
+        if (!rc) bs.push_back(b);
+        if (!rc) rc = fi_isos_create(&p, n, n, dx, dx, nullptr, nullptr, nullptr, nullptr, &h);
+        if (!rc) hs.push_back(h);
+        if (!rc) rc = fi_isos_init_state(h, z.data(), H0.data(), 0.0, b);
+        if (rc) { cleanup(); return rc; }
+        if (hs.back()->n != n) { cleanup(); return fail(FI_E_ARG, "fi_autotune: N must be even"); }
+    }
+    std::vector<const double*> Hp(nmem, H.data());
+    double t = 0.0;
+    // seconds per attempted step of the group under candidate c, over `r` timed updates
+    auto trial = [&](const TunedShape& c, int r, double& per) -> int {
+        t_trial = &c;
+        fi_batch* bt = nullptr;
+        per = 1e300;
+        if (fi_batch_create(hs.data(), nmem, &bt)) {   // a shape the kernels cannot take is skipped
+            t_trial = nullptr;
+            return 0;
+        }
+        // one untimed update (first-round estimates, first launch of the kernels)
+        t += 10.0;
+        int e = fi_batch_update(bt, Hp.data(), t, bs.data());
+        long long a0 = 0, a1 = 0;
+        for (fi_isos* h : hs) { Ctl c0{}; if (!e) e = read_ctl(h->hm, c0); a0 += c0.attempts; }
+        const auto s0 = std::chrono::steady_clock::now();
+        for (int q = 0; q < r && !e; ++q) {
+            t += 10.0;
+            e = fi_batch_update(bt, nullptr, t, bs.data());
+        }
+        const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - s0).count();
+        for (fi_isos* h : hs) { Ctl c1{}; if (!e) e = read_ctl(h->hm, c1); a1 += c1.attempts; }
+        fi_batch_destroy(bt);
+        t_trial = nullptr;
+        if (!e) per = sec / (double)std::max(1LL, a1 - a0);
+        return e;
+    };
+    // round 1: every candidate, `reps` updates; round 2: the built-in shape and
+    // the four fastest, 4 x reps updates each, twice in turn, best of the two;
+    // a candidate replaces the built-in shape only when it is 1% faster
+    const std::vector<TunedShape> cand = tune_candidates(n, nmem);
+    std::vector<std::pair<double, size_t>> r1;
+    for (size_t ci = 0; ci < cand.size() && !rc; ++ci) {
+        double per;
+        rc = trial(cand[ci], reps, per);
+        if (!rc && per < 1e299) r1.emplace_back(per, ci);
+    }
+    std::sort(r1.begin(), r1.end());
+    std::vector<size_t> fin{0};
+    for (size_t k = 0; k < r1.size() && fin.size() < 5; ++k)
+        if (r1[k].second != 0) fin.push_back(r1[k].second);
+    std::vector<double> best2(fin.size(), 1e300);
+    for (int pass = 0; pass < 2 && !rc; ++pass)
+        for (size_t k = 0; k < fin.size() && !rc; ++k) {
+            double per;
+            rc = trial(cand[fin[k]], 4 * reps, per);
+            best2[k] = std::min(best2[k], per);
+        }
+    TunedShape pick{};
+    double best = best2[0];
+    for (size_t k = 1; k < fin.size(); ++k)
+        if (best2[k] < 0.99 * best2[0] && best2[k] < best) { best = best2[k]; pick = cand[fin[k]]; }
+    t_trial = nullptr;
+    cleanup();
+    if (rc) return rc;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        g_tuned[std::make_tuple(n, (int)fp64, nmem)] = pick;
+        // the sub-batches of a split group take the same blocking
+        const int K = std::max(1, std::min({pick.K ? pick.K : (nmem >= 32 ? 2 : 1), MAX_SUB, nmem}));
+        if (K > 1)
+            for (int k = 0; k < K; ++k) {
+                const int ns = (int)((long long)nmem * (k + 1) / K) - (int)((long long)nmem * k / K);
+                TunedShape sub = pick;
+                sub.K = 1;
+                g_tuned[std::make_tuple(n, (int)fp64, ns)] = sub;
+            }
+        for (auto it = g_sub_shapes.begin(); it != g_sub_shapes.end();)
+            it = std::get<0>(it->first) == n ? g_sub_shapes.erase(it) : std::next(it);
+    }
+    if (choice) {
+        choice[0] = pick.threads_row; choice[1] = pick.C; choice[2] = pick.row_tiles; choice[3] = pick.K;
+        choice[4] = pick.gL; choice[5] = pick.gC;
+    }
     return 0;
 }
 

@@ -44,9 +44,15 @@ void launch_attempt(const Member* d_mem, int nmem, const RhsShape& s, hipStream_
 void launch_decide(const Member* d_mem, int nmem, const RhsShape& s, hipStream_t st);
 // power-of-two passes: which = 1 row forward, 2 column, 3 row inverse
 bool launch_fast_pass(const Member* d_mem, int nmem, const RhsShape& s, hipStream_t st, int which);
-// event timing of one attempt kernel: 0 stencil (fused: stencil + row DHT), 1 row DHT, 2 column DHT+divide, 3 inverse row DHT+update
+// event timing of one attempt kernel: 0 stencil (fused: stencil + row DHT), 1 row DHT, 2 column DHT+divide,
+// 3 inverse row DHT+update; PROF_ALL times every one of them
+constexpr int PROF_ALL = 99;
 void prof_select(int kernel);
 int prof_collect(double* total_ms, long long* launches);
+// the launches of one kernel (-1: all timed launches)
+int prof_collect_kernel(int kernel, double* total_ms, long long* launches);
+// the library's error message (fi_last_error) for code `code`; returns code
+int set_error(int code, const char* msg);
 int prof_fused();
 bool prof_active();
 // a start/stop event pair when launches of `kernel` are being timed

@@ -436,13 +436,14 @@ __global__ void __launch_bounds__(64) k_decide(const Member* __restrict__ mem, i
 template <typename T>
 static size_t lds_bytes(int n, int lines) { return (size_t)n * (lines + 1) * sizeof(cplx<T>); }   // + twiddles
 
-// Optional HIP-event timing of one kernel of the attempt sequence (for
-// benchmarks): events are recorded on the launch stream around every launch
-// of the selected kernel and summed on demand.
+// Optional HIP-event timing of the attempt kernels (for benchmarks): events
+// ride on every launch of the selected kernel (or of every attempt kernel,
+// PROF_ALL) and are summed per kernel on demand.
 namespace {
 struct Prof {
-    int kernel = -1;
+    int kernel = -1;              // -1 off, 0-3 one kernel, PROF_ALL every attempt kernel
     std::vector<hipEvent_t> ev;   // start/stop pairs
+    std::vector<int> which;       // kernel of each pair
     size_t used = 0;
     bool fused = false;           // last attempt ran the fused stencil + row kernel
 };
@@ -451,12 +452,14 @@ Prof g_prof;
 }  // namespace
 
 bool prof_pair(int kernel, hipEvent_t* start, hipEvent_t* stop) {
-    if (g_prof.kernel < 0 || g_prof.kernel != kernel) return false;
+    if (g_prof.kernel < 0 || (g_prof.kernel != kernel && g_prof.kernel != PROF_ALL)) return false;
     while (g_prof.used + 2 > g_prof.ev.size()) {
         hipEvent_t e;
         if (hipEventCreate(&e) != hipSuccess) return false;
         g_prof.ev.push_back(e);
     }
+    g_prof.which.resize(g_prof.ev.size() / 2);
+    g_prof.which[g_prof.used / 2] = kernel;
     *start = g_prof.ev[g_prof.used++];
     *stop = g_prof.ev[g_prof.used++];
     return true;
@@ -470,10 +473,11 @@ void prof_select(int kernel) {
 int prof_fused() { return g_prof.fused ? 1 : 0; }
 bool prof_active() { return g_prof.kernel >= 0; }
 
-int prof_collect(double* total_ms, long long* launches) {
+int prof_collect_kernel(int kernel, double* total_ms, long long* launches) {
     double tot = 0.0;
     long long n = 0;
     for (size_t i = 0; i + 1 < g_prof.used; i += 2) {
+        if (kernel >= 0 && g_prof.which[i / 2] != kernel) continue;
         float ms = 0.f;
         if (hipEventSynchronize(g_prof.ev[i + 1]) != hipSuccess) return -1;
         if (hipEventElapsedTime(&ms, g_prof.ev[i], g_prof.ev[i + 1]) != hipSuccess) return -1;
@@ -484,6 +488,8 @@ int prof_collect(double* total_ms, long long* launches) {
     *launches = n;
     return 0;
 }
+
+int prof_collect(double* total_ms, long long* launches) { return prof_collect_kernel(-1, total_ms, launches); }
 
 // methods 0-2: one pointwise kernel, decided at once
 void launch_attempt_xlra(const Member* d_mem, int nmem, const RhsShape& s, hipStream_t st) {

@@ -766,10 +766,20 @@ static void rows_fwd(const Member* d, int nmem, hipStream_t st, const void* tw) 
     const size_t lds = row_lds<T, N>(THREADS);
     launch_k(1, kf_row_fwd<T, N, THREADS>, dim3(N / (2 * L), nmem), dim3(THREADS), lds, st, d, tw);
 }
+// LDS of one workgroup on gfx950 (160 KiB per CU); launches above 64 KiB of
+// dynamic LDS declare it to the runtime first
+constexpr size_t FI_LDS_MAX = 160 * 1024;
+
 template <typename T, int N, int THREADS>
 static bool srow_fwd(const Member* d, int nmem, hipStream_t st, const void* tw) {
     using SR = SRow<T, N, THREADS>;
-    if constexpr (SR::CPT >= 1 && SR::CPT * THREADS == N && SR::ROWS >= 4 && SR::lds <= 65536) {
+    if constexpr (SR::CPT >= 1 && SR::CPT * THREADS == N && SR::ROWS >= 4 && SR::lds <= FI_LDS_MAX) {
+        if constexpr (SR::lds > 65536) {
+            static const bool ok = hipFuncSetAttribute((const void*)kf_srow_fwd<T, N, THREADS>,
+                                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                       (int)SR::lds) == hipSuccess;
+            if (!ok) return false;   // the separate stencil and row kernels run instead
+        }
         launch_k(0, kf_srow_fwd<T, N, THREADS>, dim3(N / SR::ROWS, nmem), dim3(THREADS), SR::lds, st, d, tw);
         return true;
     }
@@ -823,6 +833,8 @@ static bool fast_n(const Member* d, int nmem, const RhsShape& s, hipStream_t st,
         case 64: if constexpr (N / 8 <= 64) return srow_fwd<T, N, 64>(d, nmem, st, s.tw); break;
         case 128: if constexpr (N / 8 <= 128) return srow_fwd<T, N, 128>(d, nmem, st, s.tw); break;
         case 256: if constexpr (N / 8 <= 256) return srow_fwd<T, N, 256>(d, nmem, st, s.tw); break;
+        // one column per thread at N = 512 (two columns per thread spill)
+        case 512: if constexpr (N == 512 && sizeof(T) == 4) return srow_fwd<T, N, 512>(d, nmem, st, s.tw); break;
         }
         return false;
     }
@@ -830,6 +842,7 @@ static bool fast_n(const Member* d, int nmem, const RhsShape& s, hipStream_t st,
     case 64: if constexpr (N / 8 <= 64) { which == 1 ? rows_fwd<T, N, 64>(d, nmem, st, s.tw) : rows_inv<T, N, 64>(d, nmem, st, s.tw, ncolblk, s.row_tiles, s.dt_method); return true; } break;
     case 128: if constexpr (N / 8 <= 128) { which == 1 ? rows_fwd<T, N, 128>(d, nmem, st, s.tw) : rows_inv<T, N, 128>(d, nmem, st, s.tw, ncolblk, s.row_tiles, s.dt_method); return true; } break;
     case 256: if constexpr (N / 8 <= 256) { which == 1 ? rows_fwd<T, N, 256>(d, nmem, st, s.tw) : rows_inv<T, N, 256>(d, nmem, st, s.tw, ncolblk, s.row_tiles, s.dt_method); return true; } break;
+    case 512: if constexpr (N == 512 && sizeof(T) == 4) { which == 1 ? rows_fwd<T, N, 512>(d, nmem, st, s.tw) : rows_inv<T, N, 512>(d, nmem, st, s.tw, ncolblk, s.row_tiles, s.dt_method); return true; } break;
     }
     return false;
 }

@@ -27,7 +27,13 @@ module fastisostasy
 
     integer, parameter, public :: sp = kind(1.0)
     integer, parameter, public :: dp = kind(1.d0)
+    ! working precision of the caller's arrays, as the reference's precision
+    ! module (precision.f90:10): sp when built with -DFI_WP_SP, else dp
+#ifdef FI_WP_SP
+    integer, parameter, public :: wp = sp
+#else
     integer, parameter, public :: wp = dp
+#endif
 
     integer, parameter :: FI_MAXL = 8
 
@@ -96,6 +102,7 @@ module fastisostasy
     public :: isos_init, isos_init_ref, isos_init_state, isos_update, isos_end
     public :: isos_restart_write, isos_restart_read
     public :: bsl_init, bsl_update, bsl_end, bsl_restart_write
+    public :: isos_write_init, isos_write_step, bsl_write_init, bsl_write_step
     public :: fi_select_precision
 
     integer, parameter :: FI_Z_BED = 0, FI_Z_SS = 1, FI_DZ_SS = 2, FI_W = 3, FI_WE = 4, FI_DWDT = 5
@@ -207,6 +214,35 @@ module fastisostasy
         function c_fi_isos_restart_read(h, path, time) bind(C, name="fi_isos_restart_read") result(rc)
             import :: c_int, c_ptr, c_double, c_char
             type(c_ptr), value :: h
+            character(kind=c_char), intent(in) :: path(*)
+            real(c_double), value :: time
+            integer(c_int) :: rc
+        end function
+        function c_fi_isos_write_init(h, xc, yc, path, time) bind(C, name="fi_isos_write_init") result(rc)
+            import :: c_int, c_ptr, c_double, c_char
+            type(c_ptr), value :: h
+            real(c_double), intent(in) :: xc(*), yc(*)
+            character(kind=c_char), intent(in) :: path(*)
+            real(c_double), value :: time
+            integer(c_int) :: rc
+        end function
+        function c_fi_isos_write_step(h, path, time, hice, zb) bind(C, name="fi_isos_write_step") result(rc)
+            import :: c_int, c_ptr, c_double, c_char
+            type(c_ptr), value :: h, hice, zb
+            character(kind=c_char), intent(in) :: path(*)
+            real(c_double), value :: time
+            integer(c_int) :: rc
+        end function
+        function c_fi_bsl_write_init(b, path, time) bind(C, name="fi_bsl_write_init") result(rc)
+            import :: c_int, c_ptr, c_double, c_char
+            type(c_ptr), value :: b
+            character(kind=c_char), intent(in) :: path(*)
+            real(c_double), value :: time
+            integer(c_int) :: rc
+        end function
+        function c_fi_bsl_write_step(b, path, time) bind(C, name="fi_bsl_write_step") result(rc)
+            import :: c_int, c_ptr, c_double, c_char
+            type(c_ptr), value :: b
             character(kind=c_char), intent(in) :: path(*)
             real(c_double), value :: time
             integer(c_int) :: rc
@@ -542,6 +578,54 @@ contains
         call check(c_fi_bsl_restart_write(bsl%h, trim(filename)//c_null_char, real(time, c_double)), &
             "bsl_restart_write")
     end subroutine bsl_restart_write
+
+    ! isos_write_init / isos_write_step (src/isostasy_io.f90:197-405) and
+    ! bsl_write_init / bsl_write_step (src/barysealevel.f90:453-494): the
+    ! library's output container (the reference's variables, not NetCDF)
+    subroutine isos_write_init(isos, xc, yc, filename, time_init)
+        type(isos_class), intent(IN) :: isos
+        real(wp), intent(IN) :: xc(:), yc(:)
+        character(len=*), intent(IN) :: filename
+        real(wp), intent(IN) :: time_init
+        if (size(xc) /= isos%domain%nx_ice .or. size(yc) /= isos%domain%ny_ice) &
+            error stop "isos_write_init:: xc / yc do not match the ice grid"
+        call check(c_fi_isos_write_init(isos%h, real(xc, c_double), real(yc, c_double), &
+            trim(filename)//c_null_char, real(time_init, c_double)), "isos_write_init")
+    end subroutine isos_write_init
+
+    subroutine isos_write_step(isos, filename, time, H_ice, z_bed_bench)
+        type(isos_class), intent(IN) :: isos
+        character(len=*), intent(IN) :: filename
+        real(wp), intent(IN) :: time
+        real(wp), intent(IN) :: H_ice(:, :)
+        real(wp), intent(IN), optional :: z_bed_bench(:, :)
+        real(c_double), allocatable, target :: zb(:, :)
+        type(c_ptr) :: pzb
+        pzb = c_null_ptr
+        if (present(z_bed_bench)) then
+            allocate(zb(size(z_bed_bench, 1), size(z_bed_bench, 2)))
+            zb = real(z_bed_bench, c_double)
+            pzb = c_loc(zb)
+        end if
+        call check(c_fi_isos_write_step(isos%h, trim(filename)//c_null_char, real(time, c_double), &
+            c_null_ptr, pzb), "isos_write_step")
+    end subroutine isos_write_step
+
+    subroutine bsl_write_init(bsl, filename, time_init)
+        type(bsl_class), intent(IN) :: bsl
+        character(len=*), intent(IN) :: filename
+        real(wp), intent(IN) :: time_init
+        call check(c_fi_bsl_write_init(bsl%h, trim(filename)//c_null_char, real(time_init, c_double)), &
+            "bsl_write_init")
+    end subroutine bsl_write_init
+
+    subroutine bsl_write_step(bsl, filename, time)
+        type(bsl_class), intent(IN) :: bsl
+        character(len=*), intent(IN) :: filename
+        real(wp), intent(IN) :: time
+        call check(c_fi_bsl_write_step(bsl%h, trim(filename)//c_null_char, real(time, c_double)), &
+            "bsl_write_step")
+    end subroutine bsl_write_step
 
     ! isos_update (src/fastisostasy.f90:731-872); dwdt_corr is unused, as in the reference
     subroutine isos_update(isos, H_ice, time, bsl, dwdt_corr)

@@ -126,6 +126,14 @@ class BslClass:
         """bsl_restart_write (src/barysealevel.f90:426-450)."""
         check(load().fi_bsl_restart_write(self.h, str(path).encode(), float(time)))
 
+    def write_init(self, path, time_init):
+        """bsl_write_init (src/barysealevel.f90:453-465): a new output file."""
+        check(load().fi_bsl_write_init(self.h, str(path).encode(), float(time_init)))
+
+    def write_step(self, path, time):
+        """bsl_write_step (src/barysealevel.f90:467-490): bsl and A_ocean at `time`."""
+        check(load().fi_bsl_write_step(self.h, str(path).encode(), float(time)))
+
     def close(self):
         if getattr(self, "h", None):
             load().fi_bsl_destroy(self.h)
@@ -190,6 +198,18 @@ class IsosClass:
         takes the restart branch."""
         check(load().fi_isos_restart_read(self.h, str(path).encode(), float(time)))
 
+    def write_init(self, xc, yc, path, time_init):
+        """isos_write_init (src/isostasy_io.f90:197-221): a new output file on
+        the ice grid with axes xc, yc (m)."""
+        check(load().fi_isos_write_init(self.h, _dp(_c(xc)), _dp(_c(yc)), str(path).encode(), float(time_init)))
+
+    def write_step(self, path, time, H_ice=None, z_bed_bench=None):
+        """isos_write_step (src/isostasy_io.f90:318-405): the out% fields and dt
+        at `time` (an existing time is overwritten, a new one appended); a
+        benchmark solution adds z_bed_bench and err_z_bed."""
+        check(load().fi_isos_write_step(self.h, str(path).encode(), float(time), _dp(_c(H_ice)),
+                                        _dp(_c(z_bed_bench))))
+
     def full(self, name):
         """Padded-grid (N x N) state field."""
         a = np.empty((self.n, self.n))
@@ -244,6 +264,42 @@ class Batch:
 
     def __del__(self):
         self.close()
+
+
+def read_output(path):
+    """Read an output file of isos_write_* / bsl_write_*: dict with 'time',
+    'xc', 'yc' (km), 'units' and 'long_name' per variable, and each variable
+    as an array [time] or [time, ny, nx]."""
+    import struct
+    with open(path, "rb") as f:
+        raw = f.read()
+    if raw[:8] != b"FIOUTPUT":
+        raise ValueError(f"{path} is not an output file")
+    version, nx, ny, nvar, nrec = struct.unpack_from("<5i", raw, 8)
+    off = 28
+    names, kinds, units, long_names = [], [], {}, {}
+    for _ in range(nvar):
+        name = raw[off:off + 32].split(b"\0")[0].decode()
+        units[name] = raw[off + 32:off + 48].split(b"\0")[0].decode()
+        long_names[name] = raw[off + 48:off + 112].split(b"\0")[0].decode()
+        kinds.append(struct.unpack_from("<i", raw, off + 112)[0])
+        names.append(name)
+        off += 116
+    xc = np.frombuffer(raw, dtype="<f8", count=nx, offset=off)
+    yc = np.frombuffer(raw, dtype="<f8", count=ny, offset=off + 8 * nx)
+    off += 8 * (nx + ny)
+    per = 1 + sum(nx * ny if k == 0 else 1 for k in kinds)
+    recs = np.frombuffer(raw, dtype="<f8", count=per * nrec, offset=off).reshape(nrec, per)
+    out = {"time": recs[:, 0].copy(), "xc": xc.copy(), "yc": yc.copy(), "units": units, "long_name": long_names}
+    col = 1
+    for name, k in zip(names, kinds):
+        if k == 0:
+            out[name] = recs[:, col:col + nx * ny].reshape(nrec, ny, nx).copy()
+            col += nx * ny
+        else:
+            out[name] = recs[:, col].copy()
+            col += 1
+    return out
 
 
 # ---- reference-named procedures (src/fastisostasy.f90:40-50, barysealevel.f90:32-33)

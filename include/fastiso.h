@@ -89,8 +89,9 @@ enum fi_scalar {
     FI_S_N = 0, FI_S_T, FI_S_DT, FI_S_BSL, FI_S_DELTAV, FI_S_V_AF, FI_S_V_DEN,
     FI_S_ATTEMPTS, FI_S_ACCEPTS, FI_S_D_LITH11, FI_S_ERR, FI_S_TOL,
     FI_S_HBM_BYTES,         /* device memory held by the handle (bytes) */
-    FI_S_STATUS             /* status of the handle's last update checks: 0, FI_E_UNSTABLE or FI_E_NAN
+    FI_S_STATUS,            /* status of the handle's last update checks: 0, FI_E_UNSTABLE or FI_E_NAN
                                (tells which members of a failed fi_batch_update failed) */
+    FI_S_NX_ICE, FI_S_NY_ICE  /* the ice grid */
 };
 
 typedef struct fi_isos fi_isos;
@@ -248,6 +249,31 @@ int fi_server_failed(fi_server* s, long long* tags, int max, int* n);
 int fi_server_stats(fi_server* s, long long* rounds, long long* groups, long long* served, double* hbm_used);
 int fi_server_destroy(fi_server* s);
 
+/* Output writers (isos_write_init / isos_write_step, src/isostasy_io.f90:197-405;
+ * bsl_write_init / bsl_write_step, src/barysealevel.f90:453-494): the
+ * reference's variables, units and long names on the ice grid in a binary
+ * container ("FIOUTPUT", no NetCDF library here), time axis starting at
+ * time_init with no data; a step at a time already on the axis overwrites
+ * it, a new time is appended (nc_time_index).  xc, yc: the ice grid's axes
+ * (m, nx_ice / ny_ice values, written in km).  A non-NULL z_bed_bench adds
+ * z_bed_bench and err_z_bed (= out%w - z_bed_bench) to the file; H_ice is
+ * unused, as in the reference (the file gets out%Hice). */
+int fi_isos_write_init(fi_isos* h, const double* xc, const double* yc, const char* path, double time_init);
+int fi_isos_write_step(fi_isos* h, const char* path, double time, const double* H_ice, const double* z_bed_bench);
+int fi_bsl_write_init(fi_bsl* b, const char* path, double time_init);
+int fi_bsl_write_step(fi_bsl* b, const char* path, double time);
+
+/* Autotuner (no reference counterpart): times candidate launch shapes of the
+ * attempt kernels -- threads per row block, line pairs per column block, row
+ * tiles of the inverse pass and concurrent sub-batches on the power-of-two
+ * path; line pairs per row / column block on the mixed-radix path -- on
+ * `nmem` synthetic N x N domains (`reps` timed 10-yr updates each) and keeps
+ * the fastest for the process: batches, sub-batches and serving groups of that
+ * (N, precision, size) created afterwards use it.  Results do not depend on
+ * the shape.  choice (may be NULL) receives {threads_row, C, row_tiles, K,
+ * L_generic, C_generic}, 0 = the built-in choice. */
+int fi_autotune(int n, int precision, int nmem, int reps, int* choice);
+
 /* Synchronise the library stream. */
 int fi_sync(void);
 
@@ -262,6 +288,9 @@ int fi_prof_collect(double* total_ms, long long* launches);
 /* 1 when the last attempt ran the stencil fused into the forward row pass
  * (uniform spacing, power-of-two N); kernel 0 then times the fused kernel. */
 int fi_prof_fused(void);
+/* kernel = 99 in fi_prof_select times every attempt kernel; this sums the
+ * timed launches of one of them (0, 2, 3; -1: all timed launches) */
+int fi_prof_collect_kernel(int kernel, double* total_ms, long long* launches);
 /* Achieved device-to-device copy bandwidth (bytes read + written per second /
  * 1e9) of `reps` copies of `mib` MiB on the library stream: the practical
  * ceiling beside the HBM peak. */

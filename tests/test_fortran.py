@@ -56,6 +56,38 @@ def test_fortran_driver_matches_oracle(tmp_path):
 
 
 @pytest.mark.gpu
+def test_fortran_driver_single_precision_matches_oracle(tmp_path):
+    """The module built with wp = sp, the reference's default working
+    precision (precision.f90:10): the caller's single-precision arrays are
+    widened at the boundary and the results narrowed back, so out%w equals the
+    FP64 oracle to single-precision rounding."""
+    exe = EXE + "_sp"
+    if not os.path.exists(exe):
+        pytest.skip("drive_disc_sp not built")
+    years = 500.0
+    out = tmp_path / "w_sp.bin"
+    r = subprocess.run([exe, NML, "128", "25e3", str(years), str(out)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    raw = np.fromfile(out, dtype=np.float32)
+    assert raw.size == 2 * 128 * 128        # 4-byte reals: the sp build
+    w_f, zb_f = raw[:128 * 128].reshape(128, 128), raw[128 * 128:].reshape(128, 128)
+    wl = W.plain_disc()
+    o = O.OracleIsos(O.make_params(**wl["params"]), wl["nx"], wl["nx"], wl["dx"])
+    b = O.OracleBSL("const", 0.0)
+    o.init_state(wl["z_bed"], np.zeros_like(wl["H_ice"]), 0.0, b)
+    for k in range(1, int(years / 10) + 1):
+        t = 10.0 * k
+        b.update(t)
+        o.update(wl["H_ice"], t, b)
+    w_o = o.get("w")
+    assert np.abs(w_o).max() > 1.0
+    # single-precision rounding of |w| <= ~30 m and of z_bed ~ 1e6 m
+    assert np.abs(w_o - w_f).max() <= 4e-6 * np.abs(w_o).max()
+    assert np.abs(o.get("z_bed") - zb_f).max() <= 0.07
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("method", [1, 2])
 def test_fortran_driver_relaxation_methods(tmp_path, method):
     """The namelist path of the module with method = 1 (LLRA) and 2 (LV-ELRA),

@@ -205,12 +205,13 @@ def _run_gpu(wl, precision, times, fused, threads, monkeypatch):
     ("disc64", "fp64", 64),
     ("ens256", "fp64", 64), ("ens256", "fp64", 128), ("ens256", "fp64", 256),
     ("ens256", "mixed", 64), ("ens256", "mixed", 128), ("ens256", "mixed", 256),
-    ("lv512", "mixed", 128), ("lv512", "mixed", 256)])
+    ("lv512", "mixed", 128), ("lv512", "mixed", 256), ("lv512", "mixed", 512)])
 def test_fused_stencil_matches_separate_kernels(case, prec, threads, monkeypatch):
     """The stencil fused into the forward row pass (uniform spacing,
     power-of-two N) reproduces the separate stencil and row kernels bit for
     bit -- same arithmetic, same rounding -- on every row block, the top and
-    bottom edge blocks included, for 1, 2 and 4 columns per thread."""
+    bottom edge blocks included, for 1, 2 and 4 columns per thread (512-thread
+    blocks at N = 512: one column per thread, 68 KiB of LDS)."""
     if case == "disc64":
         wl, times = small_disc(), [0.0, 10.0, 20.0, 30.0]
     elif case == "ens256":
