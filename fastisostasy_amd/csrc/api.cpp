@@ -180,7 +180,46 @@ int make_shape(int n, int nmem, bool fp64, bool uniform, RhsShape& s) {
     s.uniform = uniform;
     s.method = 3;
     s.dt_method = -1;
-    s.fast = (n & (n - 1)) == 0 && n >= 64 && n <= 1024 && !std::getenv("FI_GENERIC_FFT");
+    const bool pow2 = (n & (n - 1)) == 0 && n >= 64 && n <= 1024;
+    const bool r3 = n == 192 || n == 384;   // 3 * 2^k: register kernels with 384-thread row blocks
+    s.fast = (pow2 || r3) && !std::getenv("FI_GENERIC_FFT");
+    if (s.fast && r3) {
+        // one row-block shape (384 threads); column blocks as for 2^k
+        const int tpl = n / 8;
+        const size_t cb = fp64 ? 16 : 8;
+        const size_t pitch = n + n / 8 + 1;
+        const int L = 384 / tpl;
+        const bool rows = (n + L * pitch) * cb <= 65536 && n % (2 * L) == 0;
+        auto col_ok = [&](int C) {
+            return C * tpl <= 1024 && ((size_t)n + (size_t)n * (C + 1)) * cb <= 65536 && n % (2 * C) == 0;
+        };
+        int best_c = 0;
+        for (int C : {16, 8, 4, 2, 1}) {
+            if (!col_ok(C)) continue;
+            best_c = C;
+            if ((size_t)(n / (2 * C)) * nmem >= 256) break;
+        }
+        if (tuned && ts.C && col_ok(ts.C)) best_c = ts.C;
+        if (const char* e = std::getenv("FI_COL_LINES")) {
+            const int C = std::atoi(e);
+            if ((C == 1 || C == 2 || C == 4 || C == 8 || C == 16) && col_ok(C)) best_c = C;
+        }
+        if (rows && best_c) {
+            s.fused = uniform && !std::getenv("FI_NO_FUSED_STENCIL");   // fused at N = 384 (one column per thread)
+            s.row_tiles = 1;
+            s.threads_row = 384;
+            s.L = L;
+            s.C = best_c;
+            s.threads_col = best_c * tpl;
+            Twiddles t;
+            int rc = get_twiddles(n, t);
+            if (rc) return rc;
+            s.tw = fp64 ? (const void*)t.d64 : (const void*)t.d32;
+            s.tw64 = t.d64;
+            return 0;
+        }
+        s.fast = false;
+    }
     if (s.fast) {
         // register-FFT kernels: TPL = n/8 threads per line, LDS <= 64 KiB
         const int tpl = n / 8;

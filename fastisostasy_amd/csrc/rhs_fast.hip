@@ -37,7 +37,7 @@ __global__ void __launch_bounds__(THREADS) kf_row_fwd(const Member* __restrict__
     const Member& M = mem[blockIdx.y];
     const kptr<Ctl> c = kview(M.ctl);
     if ((c->status != ST_RUN) | (c->k_valid != 0)) return;
-    using PL = P2Plan<N>;
+    using PL = RegPlan<N>;
     constexpr int TPL = PL::TPL, E = PL::E, L = THREADS / TPL;
     extern __shared__ __align__(16) unsigned char smem[];
     cplx<T>* tw_l = (cplx<T>*)smem;
@@ -52,8 +52,7 @@ __global__ void __launch_bounds__(THREADS) kf_row_fwd(const Member* __restrict__
         v[e] = mk<T>(src[(size_t)ra * N + t + TPL * e], src[(size_t)rb * N + t + TPL * e]);
     __syncthreads();
     RowXch<T, N> xc(buf, line);
-    reg_fft<T, N>(v, t, tw_l, xc);
-    reg_mirror<T, N>(v, m, t, xc);
+    reg_fft_mirror<T, N>(v, m, t, tw_l, xc);
     T* __restrict__ dst = (T*)M.g2;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -124,7 +123,7 @@ __device__ __forceinline__ void srow_body(const Member& M, const kptr<PhysDev> p
                                           T (&ev)[SRow<T, N, THREADS>::ROWS][SRow<T, N, THREADS>::CPT],
                                           Issue&& issue) {
     using SR = SRow<T, N, THREADS>;
-    constexpr int TPL = SR::TPL, E = P2Plan<N>::E, L = SR::L, ROWS = SR::ROWS, CPT = SR::CPT;
+    constexpr int TPL = SR::TPL, E = RegPlan<N>::E, L = SR::L, ROWS = SR::ROWS, CPT = SR::CPT;
     cplx<T>* tw_l = (cplx<T>*)smem;
     double* rw = (double*)(tw_l + N);     // w    rows, slot (r + 3) % 3
     double* rxx = rw + 3 * N;             // Mxx  rows, slot (r + 3) % 3
@@ -263,8 +262,7 @@ __device__ __forceinline__ void srow_body(const Member& M, const kptr<PhysDev> p
     for (int e = 0; e < E; ++e) v[e] = mk<T>(sf[2 * line * N + t + TPL * e], sf[(2 * line + 1) * N + t + TPL * e]);
     __syncthreads();
     RowXch<T, N> xc((cplx<T>*)(tw_l + N), line);
-    reg_fft<T, N>(v, t, tw_l, xc);
-    reg_mirror<T, N>(v, mr, t, xc);
+    reg_fft_mirror<T, N>(v, mr, t, tw_l, xc);
     gmut<T> dst = gout<T>(M.g2);
     const int ra = R0 + 2 * line, rb = ra + 1;
 #pragma unroll
@@ -445,7 +443,7 @@ __global__ void __launch_bounds__(C * (N / 8)) kf_col(const Member* __restrict__
         if (commit) copy_ctl(M.ctl, c);
         return;
     }
-    using PL = P2Plan<N>;
+    using PL = RegPlan<N>;
     constexpr int TPL = PL::TPL, E = PL::E;
     extern __shared__ __align__(16) unsigned char smem[];
     cplx<T>* tw_l = (cplx<T>*)smem;
@@ -497,8 +495,7 @@ __global__ void __launch_bounds__(C * (N / 8)) kf_col(const Member* __restrict__
         kb[e] = xk[e].y;
     }
     ColXch<T, N> xc(buf, line, C);
-    reg_fft<T, N>(v, t, tw_l, xc);
-    reg_mirror<T, N>(v, m, t, xc);
+    reg_fft_mirror<T, N>(v, m, t, tw_l, xc);
     // Hartley values and the spectral divide (lv_elva.f90:115)
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -506,8 +503,7 @@ __global__ void __launch_bounds__(C * (N / 8)) kf_col(const Member* __restrict__
         dht_pair(v[e], m[e], ha, hb);
         v[e] = mk<T>(kdiv(ha, ka[e]), kdiv(hb, kb[e]));
     }
-    reg_fft<T, N>(v, t, tw_l, xc);
-    reg_mirror<T, N>(v, m, t, xc);
+    reg_fft_mirror<T, N>(v, m, t, tw_l, xc);
     gmut<V2> dst = gout<V2>(M.g1);
     gmut<double> cor = gout<double>(M.corners);
 #pragma unroll
@@ -536,13 +532,17 @@ __global__ void __launch_bounds__(C * (N / 8)) kf_col(const Member* __restrict__
 
 // Reduce the per-thread row statistics of two rows over the TPL threads of a
 // line with a fixed butterfly (lines wider than a wave combine their wave
-// halves through LDS in wave order), and store them as the row's partials.
-// The pattern depends on N alone, so the error norm of a domain does not
-// depend on how many domains share the launch.
+// halves through LDS in wave order; lines of 3 * 2^j threads combine their
+// three 2^j-lane groups through LDS in group order), and store them as the
+// row's partials.  The pattern depends on N alone, so the error norm of a
+// domain does not depend on how many domains share the launch.
 template <int TPL>
 __device__ __forceinline__ void row_reduce_store(double a1a, double a2a, double s2a, double a1b, double a2b,
                                                  double s2b, int line, int t, gmut<double> pa) {
-    constexpr int W = TPL < 64 ? TPL : 64;
+    constexpr bool P2 = (TPL & (TPL - 1)) == 0;
+    constexpr int G = P2 ? TPL : TPL / 3;                  // lanes reduced by shuffles
+    static_assert(P2 || (G * 3 == TPL && (G & (G - 1)) == 0 && G <= 64), "TPL = 2^j or 3 * 2^j");
+    constexpr int W = G < 64 ? G : 64;
 #pragma unroll
     for (int o = W / 2; o > 0; o >>= 1) {
         a1a = fmax(a1a, __shfl_xor(a1a, o, W)); a2a = fmax(a2a, __shfl_xor(a2a, o, W));
@@ -550,16 +550,16 @@ __device__ __forceinline__ void row_reduce_store(double a1a, double a2a, double 
         a1b = fmax(a1b, __shfl_xor(a1b, o, W)); a2b = fmax(a2b, __shfl_xor(a2b, o, W));
         s2b += __shfl_xor(s2b, o, W);
     }
-    if constexpr (TPL > 64) {
-        constexpr int NW = TPL / 64;
-        __shared__ double red[16][NW][6];
-        if ((t & 63) == 0) {
-            double* r = red[line][t >> 6];
+    constexpr int NG = P2 ? (TPL > 64 ? TPL / 64 : 1) : 3;   // groups combined through LDS
+    if constexpr (NG > 1) {
+        __shared__ double red[16][NG][6];
+        if ((t % W) == 0) {
+            double* r = red[line][t / W];
             r[0] = a1a; r[1] = a2a; r[2] = s2a; r[3] = a1b; r[4] = a2b; r[5] = s2b;
         }
         __syncthreads();
         if (t == 0) {
-            for (int w = 1; w < NW; ++w) {
+            for (int w = 1; w < NG; ++w) {
                 const double* r = red[line][w];
                 a1a = fmax(a1a, r[0]); a2a = fmax(a2a, r[1]); s2a += r[2];
                 a1b = fmax(a1b, r[3]); a2b = fmax(a2b, r[4]); s2b += r[5];
@@ -589,7 +589,7 @@ __global__ void __launch_bounds__(THREADS) kf_row_inv(const Member* __restrict__
     const kptr<Ctl> c = kview(M.ctl);
     if ((c->status != ST_RUN) | (c->skip != 0)) return;
     const kptr<PhysDev> ph = kview(M.phys);
-    using PL = P2Plan<N>;
+    using PL = RegPlan<N>;
     constexpr int TPL = PL::TPL, E = PL::E, L = THREADS / TPL;
     extern __shared__ __align__(16) unsigned char smem[];
     cplx<T>* tw_l = (cplx<T>*)smem;
@@ -655,8 +655,13 @@ __global__ void __launch_bounds__(THREADS) kf_row_inv(const Member* __restrict__
         if (mark) M.ctl->pending = 1;
     } else {
         __shared__ double qs[4];
-        // N*N is a power of two here: the division by it is a multiplication
+        // /(nx*ny*1.) (lv_elva.f90:117): for a power-of-two N, N*N is a power
+        // of two and the division is a multiplication by its exact inverse
         const double S = (double)(float)((double)N * (double)N), rS = 1.0 / S;
+        auto scl = [&](double x) {
+            if constexpr (IsPow2<N>::value) return x * rS;
+            else return x / S;
+        };
         gptr<T> src = gin<T>(M.g1);
         // Loads in the order of use, each group behind a scheduling barrier so
         // that every wait covers only the groups issued before it: corner
@@ -722,22 +727,21 @@ __global__ void __launch_bounds__(THREADS) kf_row_inv(const Member* __restrict__
         }
         __syncthreads();
         // x(1,1) + x(nx,ny) + x(1,ny) + x(nx,1)  (isos_utils.f90:116)
-        const double cc = 0.25 * (qs[0] * rS + qs[3] * rS + qs[2] * rS + qs[1] * rS);
+        const double cc = 0.25 * (scl(qs[0]) + scl(qs[3]) + scl(qs[2]) + scl(qs[1]));
         const double spy = ph->sec_per_year;
         const bool store_k = !(c->t + dt < c->tf);
         RowXch<T, N> xc(buf, line);
 #pragma unroll
         for (int k = 0; k < TILES; ++k) {
             if (k > 0) __syncthreads();   // the exchange buffer is reused
-            reg_fft<T, N>(v[k], t, tw_l, xc);
-            reg_mirror<T, N>(v[k], m, t, xc);
+            reg_fft_mirror<T, N>(v[k], m, t, tw_l, xc);
             double a1a = 0.0, a2a = 0.0, s2a = 0.0, a1b = 0.0, a2b = 0.0, s2b = 0.0;
 #pragma unroll
             for (int e = 0; e < E; ++e) {
                 T ha, hb;
                 dht_pair(v[k][e], m[e], ha, hb);
                 const size_t ca = idx(ra[k], e), cb = idx(ra[k] + 1, e);
-                const double ka = ((double)ha * rS - cc) * spy, kb = ((double)hb * rS - cc) * spy;
+                const double ka = (scl((double)ha) - cc) * spy, kb = (scl((double)hb) - cc) * spy;
                 if (store_k) {
                     dw[ca] = ka;
                     dw[cb] = kb;
@@ -758,7 +762,7 @@ static size_t row_lds(int threads) {
     return ((size_t)N + (size_t)L * RowXch<T, N>::PITCH) * sizeof(cplx<T>);
 }
 template <typename T, int N>
-static size_t col_lds(int C) { return ((size_t)N + (size_t)N * (C + 1)) * sizeof(cplx<T>); }
+constexpr size_t col_lds(int C) { return ((size_t)N + (size_t)N * (C + 1)) * sizeof(cplx<T>); }
 
 template <typename T, int N, int THREADS>
 static void rows_fwd(const Member* d, int nmem, hipStream_t st, const void* tw) {
@@ -818,16 +822,27 @@ static bool fast_n(const Member* d, int nmem, const RhsShape& s, hipStream_t st,
     // committing the control block of the fused pass), 3 row inverse
     if (which == 2 || which == 4) {
         const bool cm = which == 4;
+        // (shapes whose LDS exceeds 64 KiB are never chosen by the host and not instantiated)
+        constexpr auto fits = [](int C) { return C * (N / 8) <= 1024 && col_lds<T, N>(C) <= 65536; };
         switch (s.C) {
         case 1: cols<T, N, 1>(d, nmem, st, s.tw, s.tw64, cm); return true;
-        case 2: if constexpr (2 * (N / 8) <= 1024) { cols<T, N, 2>(d, nmem, st, s.tw, s.tw64, cm); return true; } break;
-        case 4: if constexpr (4 * (N / 8) <= 1024) { cols<T, N, 4>(d, nmem, st, s.tw, s.tw64, cm); return true; } break;
-        case 8: if constexpr (8 * (N / 8) <= 1024) { cols<T, N, 8>(d, nmem, st, s.tw, s.tw64, cm); return true; } break;
-        case 16: if constexpr (16 * (N / 8) <= 1024) { cols<T, N, 16>(d, nmem, st, s.tw, s.tw64, cm); return true; } break;
+        case 2: if constexpr (fits(2)) { cols<T, N, 2>(d, nmem, st, s.tw, s.tw64, cm); return true; } break;
+        case 4: if constexpr (fits(4)) { cols<T, N, 4>(d, nmem, st, s.tw, s.tw64, cm); return true; } break;
+        case 8: if constexpr (fits(8)) { cols<T, N, 8>(d, nmem, st, s.tw, s.tw64, cm); return true; } break;
+        case 16: if constexpr (fits(16)) { cols<T, N, 16>(d, nmem, st, s.tw, s.tw64, cm); return true; } break;
         }
         return false;
     }
     const int ncolblk = s.n / (2 * s.C);
+    if constexpr (Is3Pow2<N>::value) {
+        // N = 3 * 2^k: 384-thread row blocks (8 lines of 48 threads at N = 384,
+        // 16 of 24 at N = 192)
+        if (s.threads_row != 384) return false;
+        if (which == 0) return srow_fwd<T, N, 384>(d, nmem, st, s.tw);
+        which == 1 ? rows_fwd<T, N, 384>(d, nmem, st, s.tw)
+                   : rows_inv<T, N, 384>(d, nmem, st, s.tw, ncolblk, s.row_tiles, s.dt_method);
+        return true;
+    } else {
     if (which == 0) {
         switch (s.threads_row) {
         case 64: if constexpr (N / 8 <= 64) return srow_fwd<T, N, 64>(d, nmem, st, s.tw); break;
@@ -845,6 +860,7 @@ static bool fast_n(const Member* d, int nmem, const RhsShape& s, hipStream_t st,
     case 512: if constexpr (N == 512 && sizeof(T) == 4) { which == 1 ? rows_fwd<T, N, 512>(d, nmem, st, s.tw) : rows_inv<T, N, 512>(d, nmem, st, s.tw, ncolblk, s.row_tiles, s.dt_method); return true; } break;
     }
     return false;
+    }
 }
 
 template <typename T>
@@ -855,6 +871,8 @@ static bool fast_t(const Member* d, int nmem, const RhsShape& s, hipStream_t st,
     case 256: return fast_n<T, 256>(d, nmem, s, st, which);
     case 512: return fast_n<T, 512>(d, nmem, s, st, which);
     case 1024: return fast_n<T, 1024>(d, nmem, s, st, which);
+    case 192: return fast_n<T, 192>(d, nmem, s, st, which);
+    case 384: return fast_n<T, 384>(d, nmem, s, st, which);
     }
     return false;
 }

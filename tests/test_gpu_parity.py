@@ -267,16 +267,21 @@ def test_batch_shared_bsl_matches_consecutive_updates(method):
     assert sb[0].bsl_now == bb[0].bsl_now
 
 
-@pytest.mark.parametrize("prec,nx", [("fp64", 256), ("mixed", 256), ("fp64", 384), ("mixed", 384)])
-def test_results_identical_alone_and_in_batch(prec, nx, monkeypatch):
+@pytest.mark.parametrize("prec,nx,generic", [("fp64", 256, False), ("mixed", 256, False), ("fp64", 384, False),
+                                             ("mixed", 384, False), ("fp64", 384, True), ("mixed", 384, True)])
+def test_results_identical_alone_and_in_batch(prec, nx, generic, monkeypatch):
     """A domain stepped alone and the same domain stepped inside a batch of
     other domains give the same bits, whatever block shapes the two launches
     use: the corner and error-norm reductions follow an order fixed by N, on
-    the power-of-two kernels (N = 256) and on the mixed-radix ones (N = 384,
-    one line pair per block alone, 8 and 16 in the batch)."""
+    the power-of-two register kernels (N = 256), on the register kernels of
+    N = 3 x 2^k (N = 384: one and 16 line pairs per column block) and on the
+    mixed-radix LDS kernels (FI_GENERIC_FFT; one line pair per block alone,
+    8 and 16 in the batch)."""
     target = 5
     others = [11, 12, 13]
-    print(f"alone vs batch: {prec} N={nx}", flush=True)
+    print(f"alone vs batch: {prec} N={nx} generic={generic}", flush=True)
+    if generic:
+        monkeypatch.setenv("FI_GENERIC_FFT", "1")
 
     def make(m):
         wl = W.ensemble_member(m, nx=nx)
@@ -639,6 +644,27 @@ def test_generic_sizes_match_oracle(n_ice, pad_cells, expect_n, prec):
         pass
     assert g.n == o.n == expect_n
     compare(o, g, wtol=1e-6 if prec == "fp64" else 1e-3, rel=None if prec == "fp64" else 1e-5)
+
+
+@pytest.mark.parametrize("n_ice,pad_cells,expect_n", [(128, 32, 192), (320, 32, 384)])
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_three_times_power_of_two_register_kernels(n_ice, pad_cells, expect_n, prec, monkeypatch):
+    """N = 192 and 384 (3 x 2^k) run the register kernels: three N/3-point
+    register FFTs and a radix-3 pass per line (the stencil fused into the
+    forward row pass at N = 384); against the oracle over 300 yr, and against
+    the mixed-radix LDS kernels (FI_GENERIC_FFT) to FP64 rounding."""
+    print(f"3 x 2^k register kernels: N={expect_n} {prec}", flush=True)
+    wl = small_disc(n_ice=n_ice, dx=50e3, pad=pad_cells * 50e3)
+    times = [0.0] + [10.0 * k for k in range(1, 31)]
+    for o, g in run_pair(wl, times, precision=prec):
+        pass
+    assert g.n == o.n == expect_n
+    compare(o, g, wtol=1e-6 if prec == "fp64" else 1e-3, rel=None if prec == "fp64" else 1e-5)
+    monkeypatch.setenv("FI_GENERIC_FFT", "1")
+    for o2, gen in run_pair(wl, times, precision=prec):
+        pass
+    tol = 1e-9 if prec == "fp64" else 1e-3
+    assert np.abs(g.out("w") - gen.out("w")).max() < tol
 
 
 def test_generic_kernels_on_power_of_two(monkeypatch):
