@@ -90,10 +90,11 @@ def autotune(n, precision, nmem, reps=3):
     trial updates; returns the choice (0 = the built-in one)."""
     import ctypes
     from fastisostasy_amd._lib import check, load
-    choice = (ctypes.c_int * 6)()
+    choice = (ctypes.c_int * 8)()
     t0 = time.perf_counter()
     check(load().fi_autotune(n, 0 if precision == "fp64" else 1, nmem, reps, choice))
-    keys = ("threads_row", "col_lines", "row_tiles", "subbatches", "generic_row_lines", "generic_col_lines")
+    keys = ("threads_row", "col_lines", "row_tiles", "subbatches", "generic_row_lines", "generic_col_lines",
+            "srow_rows", "inv_threads")
     out = {k: int(v) for k, v in zip(keys, choice)}
     out["seconds"] = round(time.perf_counter() - t0, 2)
     return out

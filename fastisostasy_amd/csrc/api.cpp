@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <chrono>
 #include <deque>
+#include <functional>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -156,7 +157,26 @@ int fft_threads(const FftPlan& p, int lines, int& threads) {
 struct TunedShape {
     int threads_row = 0, C = 0, row_tiles = 0, K = 0;   // power-of-two kernels; K concurrent sub-batches
     int gL = 0, gC = 0;                                 // mixed-radix kernels: line pairs per row / column block
+    int srow_rows = 0, threads_inv = 0;                 // register kernels: fused-pass rows, inverse-pass threads
 };
+
+// Rows per block of the fused stencil pass with one column per thread
+// (FI_SROW_ROWS = 4 | 8, or the tuned value; 0 keeps 2L rows): more,
+// shorter row blocks for small batches.  Block size of the inverse row pass
+// (FI_INV_THREADS, or the tuned value; 0 keeps threads_row).
+void small_batch_shape(RhsShape& s, bool tuned, const TunedShape& ts, bool one_col_ok,
+                       const std::function<bool(int)>& inv_ok) {
+    s.srow_rows = 0;
+    s.threads_inv = 0;
+    if (s.fused && one_col_ok) {
+        int r = tuned ? ts.srow_rows : 0;
+        if (const char* e = std::getenv("FI_SROW_ROWS")) r = std::atoi(e);
+        if (r == 4 || r == 8) s.srow_rows = r;
+    }
+    int th = tuned ? ts.threads_inv : 0;
+    if (const char* e = std::getenv("FI_INV_THREADS")) th = std::atoi(e);
+    if (th && th != s.threads_row && inv_ok(th)) s.threads_inv = th;
+}
 std::map<std::tuple<int, int, int>, TunedShape> g_tuned;
 thread_local const TunedShape* t_trial = nullptr;
 
@@ -211,6 +231,8 @@ int make_shape(int n, int nmem, bool fp64, bool uniform, RhsShape& s) {
             s.L = L;
             s.C = best_c;
             s.threads_col = best_c * tpl;
+            // (the inverse row pass keeps 384 threads: its corner sums take whole waves)
+            small_batch_shape(s, tuned, ts, n == 384, [](int) { return false; });
             Twiddles t;
             int rc = get_twiddles(n, t);
             if (rc) return rc;
@@ -273,6 +295,11 @@ int make_shape(int n, int nmem, bool fp64, bool uniform, RhsShape& s) {
             s.L = best_t / tpl;
             s.C = best_c;
             s.threads_col = best_c * tpl;
+            const bool one_col = (n <= 256 || (n == 512 && !fp64)) && row_ok(n);
+            small_batch_shape(s, tuned, ts, one_col, [&](int th) {
+                return (th == 64 || th == 128 || th == 256 || (th == 512 && n == 512 && !fp64)) && row_ok(th) &&
+                       (s.row_tiles == 1 || n % (4 * (th / tpl)) == 0);
+            });
             Twiddles t;
             int rc = get_twiddles(n, t);
             if (rc) return rc;
@@ -578,7 +605,8 @@ unsigned long long g_graph_clock = 0;
 bool same_shape(const RhsShape& a, const RhsShape& b) {
     return a.n == b.n && a.L == b.L && a.C == b.C && a.threads_row == b.threads_row &&
            a.threads_col == b.threads_col && a.fp64 == b.fp64 && a.uniform == b.uniform && a.fast == b.fast &&
-           a.fused == b.fused && a.row_tiles == b.row_tiles && a.tw == b.tw && a.tw64 == b.tw64 &&
+           a.fused == b.fused && a.row_tiles == b.row_tiles && a.srow_rows == b.srow_rows &&
+           a.threads_inv == b.threads_inv && a.tw == b.tw && a.tw64 == b.tw64 &&
            a.plan.n == b.plan.n && a.plan.nf == b.plan.nf && a.method == b.method && a.dt_method == b.dt_method;
 }
 
@@ -2252,6 +2280,7 @@ std::vector<TunedShape> tune_candidates(int n, int nmem) {
     std::vector<TunedShape> c;
     c.push_back(TunedShape{});
     const bool p2 = (n & (n - 1)) == 0 && n >= 64 && n <= 1024;
+    const bool r3 = n == 192 || n == 384;
     if (p2) {
         for (int th : {64, 128, 256, 512})
             for (int C : {2, 4, 8, 16})
@@ -2262,6 +2291,13 @@ std::vector<TunedShape> tune_candidates(int n, int nmem) {
                         t.threads_row = th; t.C = C; t.row_tiles = rt; t.K = K;
                         c.push_back(t);
                     }
+    } else if (r3) {
+        for (int C : {1, 2, 4, 8, 16})
+            for (int K = 1; K <= std::min(MAX_SUB, nmem); ++K) {
+                TunedShape t;
+                t.C = C; t.K = K;
+                c.push_back(t);
+            }
     } else {
         for (int L : {1, 2, 4, 8})
             for (int C : {1, 2, 4, 8, 16})
@@ -2271,6 +2307,19 @@ std::vector<TunedShape> tune_candidates(int n, int nmem) {
                     c.push_back(t);
                 }
     }
+    // small batches, where the latency of one block's work sets the launch
+    // time: shorter fused row blocks (one column per thread) and an inverse
+    // row pass with its own block size
+    if ((p2 || r3) && nmem <= 8)
+        for (int rows : {4, 8})
+            for (int inv : {0, 64, 128})
+                for (int C : {1, 2, 4, 8}) {
+                    if (r3 && inv) continue;   // the N = 3 * 2^k inverse pass keeps 384 threads
+                    TunedShape t;
+                    t.threads_row = p2 ? std::min(n, 256) : 0; t.C = C; t.row_tiles = 1; t.K = 1;
+                    t.srow_rows = rows; t.threads_inv = inv;
+                    c.push_back(t);
+                }
     return c;
 }
 
@@ -2383,7 +2432,7 @@ int fi_autotune(int n, int precision, int nmem, int reps, int* choice) {
     }
     if (choice) {
         choice[0] = pick.threads_row; choice[1] = pick.C; choice[2] = pick.row_tiles; choice[3] = pick.K;
-        choice[4] = pick.gL; choice[5] = pick.gC;
+        choice[4] = pick.gL; choice[5] = pick.gC; choice[6] = pick.srow_rows; choice[7] = pick.threads_inv;
     }
     return 0;
 }

@@ -24,6 +24,9 @@ struct RhsShape {
     bool fast;        // power-of-two N: register-FFT kernels (threads_row = block size)
     bool fused;       // fast && uniform: stencil fused into the forward row pass
     int row_tiles;    // row blocks per thread block of the inverse row pass (1 or 2)
+    int srow_rows;    // fast && fused: rows per block of the fused stencil pass with one
+                      // column per thread (4 or 8); 0: 2L rows in threads_row threads
+    int threads_inv;  // fast: block size of the inverse row pass (0: threads_row)
     int method;       // 3 LV-ELVA (transform passes); 0-2 pointwise relaxation RHS
     int dt_method;    // integrator shared by the batch (0-3), -1 if not known
 };

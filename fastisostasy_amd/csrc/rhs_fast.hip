@@ -80,12 +80,18 @@ __global__ void __launch_bounds__(THREADS) kf_row_fwd(const Member* __restrict__
 // Block order is XCD-aware: consecutive workgroup ids go to different XCDs,
 // so the row blocks of one member are mapped to ids that share an XCD and
 // its L2 holds the halo rows of the neighbouring blocks.
-template <typename T, int N, int THREADS> struct SRow {
-    static constexpr int TPL = N / 8, L = THREADS / TPL, ROWS = 2 * L;
+//
+// LF: complex lines (2 LF rows) per block; by default every thread takes part
+// in a line (LF = THREADS / TPL).  A smaller LF gives more, shorter row blocks
+// for small batches, where the march of one block sets the launch's latency;
+// the threads beyond LF lines then transform a copy of line 0, discarded.
+template <typename T, int N, int THREADS, int LF = THREADS / (N / 8)> struct SRow {
+    static constexpr int TPL = N / 8, L = LF, ROWS = 2 * L, LX = THREADS / TPL;
+    static_assert(LF >= 1 && LF <= LX, "lines per block");
     static constexpr int CPT = N / THREADS;
     static constexpr size_t ring_bytes = (size_t)(3 + 3 + 2) * N * sizeof(double);
     static constexpr size_t stencil_bytes = ring_bytes + (size_t)ROWS * N * sizeof(T);
-    static constexpr size_t xch_bytes = (size_t)L * RowXch<T, N>::PITCH * sizeof(cplx<T>);
+    static constexpr size_t xch_bytes = (size_t)LX * RowXch<T, N>::PITCH * sizeof(cplx<T>);
     static constexpr size_t lds = (size_t)N * sizeof(cplx<T>) + (stencil_bytes > xch_bytes ? stencil_bytes : xch_bytes);
 };
 
@@ -114,15 +120,15 @@ __device__ __forceinline__ void xcd_order(int& member, int& blk) {
 // global loads due at march step tau (inputs arrive SR_LA steps ahead of use).
 // Rows are counted in march order: first_edge / last_edge say that the march
 // starts / ends at a domain edge; UP marches run against the row order.
-template <typename T, int N, int THREADS, bool UP, class Issue>
+template <typename T, int N, int THREADS, int LF, bool UP, class Issue>
 __device__ __forceinline__ void srow_body(const Member& M, const kptr<PhysDev> ph, int R0, int tid, bool first_edge,
                                           bool last_edge, unsigned char* smem,
-                                          double (&wv)[SRow<T, N, THREADS>::ROWS + 4][SRow<T, N, THREADS>::CPT],
-                                          T (&dv)[SRow<T, N, THREADS>::ROWS + 2][SRow<T, N, THREADS>::CPT],
-                                          T (&cv)[SRow<T, N, THREADS>::ROWS][SRow<T, N, THREADS>::CPT],
-                                          T (&ev)[SRow<T, N, THREADS>::ROWS][SRow<T, N, THREADS>::CPT],
+                                          double (&wv)[SRow<T, N, THREADS, LF>::ROWS + 4][SRow<T, N, THREADS, LF>::CPT],
+                                          T (&dv)[SRow<T, N, THREADS, LF>::ROWS + 2][SRow<T, N, THREADS, LF>::CPT],
+                                          T (&cv)[SRow<T, N, THREADS, LF>::ROWS][SRow<T, N, THREADS, LF>::CPT],
+                                          T (&ev)[SRow<T, N, THREADS, LF>::ROWS][SRow<T, N, THREADS, LF>::CPT],
                                           Issue&& issue) {
-    using SR = SRow<T, N, THREADS>;
+    using SR = SRow<T, N, THREADS, LF>;
     constexpr int TPL = SR::TPL, E = RegPlan<N>::E, L = SR::L, ROWS = SR::ROWS, CPT = SR::CPT;
     cplx<T>* tw_l = (cplx<T>*)smem;
     double* rw = (double*)(tw_l + N);     // w    rows, slot (r + 3) % 3
@@ -257,12 +263,14 @@ __device__ __forceinline__ void srow_body(const Member& M, const kptr<PhysDev> p
     });
     __syncthreads();
     const int line = tid / TPL, t = tid % TPL;
+    const int ld = (L == SR::LX || line < L) ? line : 0;   // lines beyond L: a discarded copy of line 0
     cplx<T> v[E], mr[E];
 #pragma unroll
-    for (int e = 0; e < E; ++e) v[e] = mk<T>(sf[2 * line * N + t + TPL * e], sf[(2 * line + 1) * N + t + TPL * e]);
+    for (int e = 0; e < E; ++e) v[e] = mk<T>(sf[2 * ld * N + t + TPL * e], sf[(2 * ld + 1) * N + t + TPL * e]);
     __syncthreads();
     RowXch<T, N> xc((cplx<T>*)(tw_l + N), line);
     reg_fft_mirror<T, N>(v, mr, t, tw_l, xc);
+    if (L < SR::LX && line >= L) return;
     gmut<T> dst = gout<T>(M.g2);
     const int ra = R0 + 2 * line, rb = ra + 1;
 #pragma unroll
@@ -280,10 +288,10 @@ __device__ __forceinline__ void srow_body(const Member& M, const kptr<PhysDev> p
 // (both at the start or both at the end) and the second load finds them in
 // L2; marching the same way, one block would load them first and the other
 // a dozen steps later, after L2 had evicted them.
-template <typename T, int N, int THREADS, bool UP>
+template <typename T, int N, int THREADS, int LF, bool UP>
 __device__ __forceinline__ void srow_block(const Member* __restrict__ mem, const void* __restrict__ twv, int mi,
                                            int rblk) {
-    using SR = SRow<T, N, THREADS>;
+    using SR = SRow<T, N, THREADS, LF>;
     constexpr int ROWS = SR::ROWS, CPT = SR::CPT;
     const Member& M = mem[mi];
     const kptr<Ctl> c = kview(M.ctl);
@@ -414,31 +422,40 @@ __device__ __forceinline__ void srow_block(const Member* __restrict__ mem, const
     }
     // the march meets the domain edge it starts from first: top for a downward
     // block, bottom for an upward one
-    srow_body<T, N, THREADS, UP>(M, ph, R0, tid, UP ? bottom : top, UP ? top : bottom, smem, wv, dv, cv, ev, issue);
+    srow_body<T, N, THREADS, LF, UP>(M, ph, R0, tid, UP ? bottom : top, UP ? top : bottom, smem, wv, dv, cv, ev,
+                                     issue);
     write_next();
 }
 
-template <typename T, int N, int THREADS>
+template <typename T, int N, int THREADS, int LF>
 __global__ void __launch_bounds__(THREADS) kf_srow_fwd(const Member* __restrict__ mem, const void* __restrict__ twv) {
-    using SR = SRow<T, N, THREADS>;
+    using SR = SRow<T, N, THREADS, LF>;
     static_assert(SR::CPT >= 1 && SR::CPT * THREADS == N && SR::ROWS >= 4, "fused stencil shape");
     int mi, rblk;
     xcd_order(mi, rblk);
-    if (rblk & 1) srow_block<T, N, THREADS, true>(mem, twv, mi, rblk);
-    else srow_block<T, N, THREADS, false>(mem, twv, mi, rblk);
+    if (rblk & 1) srow_block<T, N, THREADS, LF, true>(mem, twv, mi, rblk);
+    else srow_block<T, N, THREADS, LF, false>(mem, twv, mi, rblk);
 }
 
 // COMMIT: the attempt began with the fused stencil pass, which left the
 // control block in ctl_next; block 0 commits it (the other blocks read
 // ctl_next, so nothing reads the block being written).
+//
+// Block order is XCD-aware (as in the row passes): a domain's column blocks
+// run on one XCD and share its L2 for the 128-byte lines that neighbouring
+// blocks split between them when a block holds fewer than 16 line pairs
+// (8 line pairs: 13.9 us per launch on the 64-member ensemble against 15.0
+// with the plain order; 16: within 2%).
 template <typename T, int N, int C, bool COMMIT>
 __global__ void __launch_bounds__(C * (N / 8)) kf_col(const Member* __restrict__ mem, const void* __restrict__ twv,
                                                      const double2* __restrict__ tw64) {
-    const Member& M = mem[blockIdx.y];
+    int mi, cblk;
+    xcd_order(mi, cblk);
+    const Member& M = mem[mi];
     const kptr<Ctl> c = kview(COMMIT ? M.ctl_next : M.ctl);
     // block 0 commits after its last load: a one-lane store ahead of the loads
     // would make every later wait a full one
-    const bool commit = COMMIT && blockIdx.x == 0 && threadIdx.x == 0;
+    const bool commit = COMMIT && cblk == 0 && threadIdx.x == 0;
     if ((c->status != ST_RUN) | (c->k_valid != 0) | (c->skip != 0)) {
         if (commit) copy_ctl(M.ctl, c);
         return;
@@ -464,7 +481,7 @@ __global__ void __launch_bounds__(C * (N / 8)) kf_col(const Member* __restrict__
         }
     }
     __builtin_amdgcn_sched_barrier(0);
-    const int c0 = blockIdx.x * 2 * C;
+    const int c0 = cblk * 2 * C;
     const int ca = c0 + 2 * line, cb = ca + 1;   // adjacent columns share a complex line
     using V2 = typename vec2<T>::type;
     gptr<V2> src = gin<V2>(M.g2);
@@ -774,18 +791,28 @@ static void rows_fwd(const Member* d, int nmem, hipStream_t st, const void* tw) 
 // dynamic LDS declare it to the runtime first
 constexpr size_t FI_LDS_MAX = 160 * 1024;
 
-template <typename T, int N, int THREADS>
+template <typename T, int N, int THREADS, int LF = THREADS / (N / 8)>
 static bool srow_fwd(const Member* d, int nmem, hipStream_t st, const void* tw) {
-    using SR = SRow<T, N, THREADS>;
+    using SR = SRow<T, N, THREADS, LF>;
     if constexpr (SR::CPT >= 1 && SR::CPT * THREADS == N && SR::ROWS >= 4 && SR::lds <= FI_LDS_MAX) {
         if constexpr (SR::lds > 65536) {
-            static const bool ok = hipFuncSetAttribute((const void*)kf_srow_fwd<T, N, THREADS>,
+            static const bool ok = hipFuncSetAttribute((const void*)kf_srow_fwd<T, N, THREADS, LF>,
                                                        hipFuncAttributeMaxDynamicSharedMemorySize,
                                                        (int)SR::lds) == hipSuccess;
             if (!ok) return false;   // the separate stencil and row kernels run instead
         }
-        launch_k(0, kf_srow_fwd<T, N, THREADS>, dim3(N / SR::ROWS, nmem), dim3(THREADS), SR::lds, st, d, tw);
+        launch_k(0, kf_srow_fwd<T, N, THREADS, LF>, dim3(N / SR::ROWS, nmem), dim3(THREADS), SR::lds, st, d, tw);
         return true;
+    }
+    return false;
+}
+// one column per thread (THREADS = N, or 384 at N = 384) with 4 or 8 rows
+// per block (srow_rows), for small batches
+template <typename T, int N, int THREADS>
+static bool srow_fwd_rows(const Member* d, int nmem, hipStream_t st, const void* tw, int rows) {
+    if constexpr (THREADS / (N / 8) >= 4) {
+        if (rows == 4) return srow_fwd<T, N, THREADS, 2>(d, nmem, st, tw);
+        if (rows == 8) return srow_fwd<T, N, THREADS, 4>(d, nmem, st, tw);
     }
     return false;
 }
@@ -838,12 +865,20 @@ static bool fast_n(const Member* d, int nmem, const RhsShape& s, hipStream_t st,
         // N = 3 * 2^k: 384-thread row blocks (8 lines of 48 threads at N = 384,
         // 16 of 24 at N = 192)
         if (s.threads_row != 384) return false;
-        if (which == 0) return srow_fwd<T, N, 384>(d, nmem, st, s.tw);
+        if (which == 0)
+            return s.srow_rows ? srow_fwd_rows<T, N, 384>(d, nmem, st, s.tw, s.srow_rows)
+                               : srow_fwd<T, N, 384>(d, nmem, st, s.tw);
         which == 1 ? rows_fwd<T, N, 384>(d, nmem, st, s.tw)
                    : rows_inv<T, N, 384>(d, nmem, st, s.tw, ncolblk, s.row_tiles, s.dt_method);
         return true;
     } else {
     if (which == 0) {
+        if (s.srow_rows) {
+            // one column per thread, 4 or 8 rows per block
+            if constexpr (N <= 256 || (N == 512 && sizeof(T) == 4))
+                return srow_fwd_rows<T, N, N>(d, nmem, st, s.tw, s.srow_rows);
+            return false;
+        }
         switch (s.threads_row) {
         case 64: if constexpr (N / 8 <= 64) return srow_fwd<T, N, 64>(d, nmem, st, s.tw); break;
         case 128: if constexpr (N / 8 <= 128) return srow_fwd<T, N, 128>(d, nmem, st, s.tw); break;
@@ -853,7 +888,8 @@ static bool fast_n(const Member* d, int nmem, const RhsShape& s, hipStream_t st,
         }
         return false;
     }
-    switch (s.threads_row) {
+    // the inverse row pass may take its own block size (threads_inv)
+    switch (which == 3 && s.threads_inv ? s.threads_inv : s.threads_row) {
     case 64: if constexpr (N / 8 <= 64) { which == 1 ? rows_fwd<T, N, 64>(d, nmem, st, s.tw) : rows_inv<T, N, 64>(d, nmem, st, s.tw, ncolblk, s.row_tiles, s.dt_method); return true; } break;
     case 128: if constexpr (N / 8 <= 128) { which == 1 ? rows_fwd<T, N, 128>(d, nmem, st, s.tw) : rows_inv<T, N, 128>(d, nmem, st, s.tw, ncolblk, s.row_tiles, s.dt_method); return true; } break;
     case 256: if constexpr (N / 8 <= 256) { which == 1 ? rows_fwd<T, N, 256>(d, nmem, st, s.tw) : rows_inv<T, N, 256>(d, nmem, st, s.tw, ncolblk, s.row_tiles, s.dt_method); return true; } break;

@@ -265,13 +265,16 @@ int fi_bsl_write_step(fi_bsl* b, const char* path, double time);
 
 /* Autotuner (no reference counterpart): times candidate launch shapes of the
  * attempt kernels -- threads per row block, line pairs per column block, row
- * tiles of the inverse pass and concurrent sub-batches on the power-of-two
- * path; line pairs per row / column block on the mixed-radix path -- on
+ * tiles of the inverse pass and concurrent sub-batches on the register-FFT
+ * path, and for batches of up to 8 domains rows per fused stencil block and
+ * the inverse pass's own block size; line pairs per row / column block on the
+ * mixed-radix path -- on
  * `nmem` synthetic N x N domains (`reps` timed 10-yr updates each) and keeps
  * the fastest for the process: batches, sub-batches and serving groups of that
  * (N, precision, size) created afterwards use it.  Results do not depend on
- * the shape.  choice (may be NULL) receives {threads_row, C, row_tiles, K,
- * L_generic, C_generic}, 0 = the built-in choice. */
+ * the shape.  choice (may be NULL, else 8 ints) receives {threads_row, C,
+ * row_tiles, K, L_generic, C_generic, srow_rows, threads_inv}, 0 = the
+ * built-in choice. */
 int fi_autotune(int n, int precision, int nmem, int reps, int* choice);
 
 /* Synchronise the library stream. */

@@ -680,3 +680,29 @@ def test_generic_kernels_on_power_of_two(monkeypatch):
     assert gen.n == 128
     compare(o2, gen, wtol=1e-6)
     assert np.abs(fast.out("w") - gen.out("w")).max() < 1e-9
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_asymmetric_kappa_r_matches_oracle(prec):
+    """A two-layer freqdomain rheology whose upper-layer viscosity follows an
+    off-centre Gaussian: R, hence kappa*R, is even in neither x nor y (the
+    lumping mixes the spatial viscosity with the frequency kappa at the same
+    array index, lv_elva.f90:207-306); against the oracle over 100 yr."""
+    print(f"asymmetric kappa*R: {prec}", flush=True)
+    nx, dx = 96, 25e3
+    n = W.padded_size(nx, dx, 400e3)
+    xc = W.padded_coords(n, dx)
+    X, Y = np.meshgrid(xc, xc)
+    g = np.exp(-((X - 300e3) ** 2 + (Y + 200e3) ** 2) / (2 * 500e3 ** 2))
+    eta = np.stack([10.0 ** (21.0 - 1.5 * g), np.full((n, n), 3e21)])
+    p = W.params(min_pad=400e3, nl=2, zl=[100e3, 300e3], viscosities=[1e21, 3e21], layering="equalized",
+                 lumping="freqdomain", mantle="fields", dt_max=0.05)
+    wl = dict(params=p, nx=nx, dx=dx, H_ice=W.parabolic_dome(nx, dx, 800e3, 2500.0),
+              z_bed=np.full((nx, nx), 1e6), eta_field=eta)
+    for o, gg in run_pair(wl, [0.0] + [10.0 * k for k in range(1, 11)], precision=prec):
+        pass
+    assert gg.n == n == 128
+    R = gg.full("R")
+    # mirror k -> N - k (mod N) in y and in x
+    assert not np.array_equal(R[1:, :], R[:0:-1, :]) and not np.array_equal(R[:, 1:], R[:, :0:-1])
+    compare(o, gg, wtol=1e-6 if prec == "fp64" else 1e-3, rel=None if prec == "fp64" else 1e-5)
