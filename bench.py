@@ -31,10 +31,22 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 KERNEL_NAMES = {0: "k_stencil", 1: "k_row_fwd", 2: "k_col", 3: "k_row_inv"}
 FUSED_NAME = "kf_srow_fwd (stencil + forward row pass)"
-# the power-of-two attempt kernels of the mixed 256^2 batch, as rocprofv3 names them
-FAST_NAMES = {0: "kf_srow_fwd<float, 256, 256>", 2: "kf_col<float, 256, 16, true>",
-              3: "kf_row_inv<float, 256, 256, 1, 3>"}
 PROF_ALL = 99
+
+
+def fast_kernel_names(shape):
+    """Template names (as rocprofv3 prints them) of the register-FFT attempt
+    kernels the last timed attempt ran, from fi_prof_shape: {N, fp64, fused,
+    fast, threads_row, C, row_tiles, srow_rows, threads_inv, dt_method}."""
+    n, fp64, fused, fast, th, c, rt, rows, inv, meth = shape
+    if not (fast and fused):
+        return {}
+    t = "double" if fp64 else "float"
+    tpl = n // 8
+    sth = (n if n % 3 else 384) if rows else th
+    lf = rows // 2 if rows else sth // tpl
+    return {0: f"kf_srow_fwd<{t}, {n}, {sth}, {lf}>", 2: f"kf_col<{t}, {n}, {c}, true>",
+            3: f"kf_row_inv<{t}, {n}, {inv or th}, {rt}, {meth}>"}
 
 
 def measured_traffic(kernel_name, cells):
@@ -149,6 +161,9 @@ def run_batched(nmem, steps, warmup, precision, prof_kernel=None, timed=True, ex
             check(load().fi_prof_collect_kernel(k, ctypes.byref(ms), ctypes.byref(cnt)))
             prof[k] = (ms.value, cnt.value)
         prof["fused"] = bool(load().fi_prof_fused())
+        shape = (ctypes.c_int * 10)()
+        check(load().fi_prof_shape(shape))
+        prof["names"] = fast_kernel_names(list(shape))
         check(load().fi_prof_select(-1))
     for _ in range(extra_steps):
         # untimed steps that bring a run to the end time of another
@@ -403,9 +418,10 @@ def kernel_roofline(prof, k, cells, n, fused):
     avg_s = ms / cnt * 1e-3
     nbytes = kernel_bytes_per_cell(k, 4, fused) * cells
     achieved = nbytes / avg_s / 1e9
-    name = (FUSED_NAME if k == 0 else FAST_NAMES[k].split("<")[0]) if fused else KERNEL_NAMES[k]
-    traffic = measured_traffic(FAST_NAMES[k], cells) if (fused and n == 256 and k in FAST_NAMES) else None
-    return {"kernel": name, "bytes_per_launch": nbytes, "avg_launch_us": round(avg_s * 1e6, 2), "launches": cnt,
+    full = prof.get("names", {}).get(k)
+    name = (FUSED_NAME if k == 0 else full.split("<")[0]) if (fused and full) else KERNEL_NAMES[k]
+    traffic = measured_traffic(full, cells) if full else None
+    return {"kernel": name, "template": full, "bytes_per_launch": nbytes, "avg_launch_us": round(avg_s * 1e6, 2), "launches": cnt,
             "achieved": round(achieved, 1), "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic}
 
 
@@ -418,12 +434,12 @@ def roofline(res, args, cells, n):
     if not prof:
         return None
     fused = prof["fused"]
-    ks = {k: kernel_roofline(prof, k, cells, n, fused) for k in prof if k != "fused"}
+    ks = {k: kernel_roofline(prof, k, cells, n, fused) for k in (0, 2, 3) if k in prof}
     ks = {k: v for k, v in ks.items() if v}
     if not ks:
         return None
     top = ks.get(0) or next(iter(ks.values()))
-    out = {"kernel": top["kernel"], "bound": "hbm", "achieved": top["achieved"], "peak": HBM_PEAK_GBS,
+    out = {"kernel": top["kernel"], "template": top["template"], "bound": "hbm", "achieved": top["achieved"], "peak": HBM_PEAK_GBS,
            "unit": "GB/s", "frac": top["frac"], "traffic": top["traffic"], "avg_launch_us": top["avg_launch_us"],
            "launches": top["launches"], "bytes_per_launch": top["bytes_per_launch"]}
     if len(ks) > 1:

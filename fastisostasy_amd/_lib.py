@@ -31,7 +31,7 @@ EXPORTS = ["fi_last_error", "fi_version", "fi_padded_size", "fi_isos_create", "f
            "fi_bsl_restart_read", "fi_isos_destroy", "fi_bsl_create", "fi_bsl_create_series",
            "fi_bsl_read_series", "fi_bsl_update", "fi_bsl_get",
            "fi_bsl_destroy", "fi_batch_create", "fi_batch_update", "fi_batch_destroy", "fi_sync",
-           "fi_prof_select", "fi_prof_collect", "fi_prof_collect_kernel", "fi_prof_fused", "fi_bench_copy",
+           "fi_prof_select", "fi_prof_collect", "fi_prof_collect_kernel", "fi_prof_fused", "fi_prof_shape", "fi_bench_copy",
            "fi_isos_write_init", "fi_isos_write_step", "fi_bsl_write_init", "fi_bsl_write_step", "fi_autotune",
            "fi_server_create", "fi_server_admit", "fi_server_release", "fi_server_submit", "fi_server_pending",
            "fi_server_run", "fi_server_completed", "fi_server_failed", "fi_server_stats", "fi_server_destroy"]
@@ -127,6 +127,7 @@ def load():
     L.fi_bsl_write_step.argtypes = [vp, ctypes.c_char_p, d]
     L.fi_autotune.argtypes = [i, i, i, i, ctypes.POINTER(i)]
     L.fi_prof_fused.argtypes = []
+    L.fi_prof_shape.argtypes = [ctypes.POINTER(i)]
     L.fi_bench_copy.argtypes = [d, i, dp]
     ll = ctypes.c_longlong
     L.fi_server_create.argtypes = [d, i, ctypes.POINTER(vp)]

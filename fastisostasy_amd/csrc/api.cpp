@@ -2222,6 +2222,12 @@ int fi_prof_collect_kernel(int kernel, double* total_ms, long long* launches) {
 
 int fi_prof_fused(void) { return prof_fused(); }
 
+int fi_prof_shape(int* out) {
+    if (!out) return fail(FI_E_ARG, "NULL argument");
+    prof_shape(out);
+    return 0;
+}
+
 int fi_bench_copy(double mib, int reps, double* gbs) {
     int rc = ensure_device();
     if (rc) return rc;

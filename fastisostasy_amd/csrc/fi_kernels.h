@@ -57,6 +57,7 @@ int prof_collect_kernel(int kernel, double* total_ms, long long* launches);
 // the library's error message (fi_last_error) for code `code`; returns code
 int set_error(int code, const char* msg);
 int prof_fused();
+void prof_shape(int* out);   // 10 ints, fi_prof_shape
 bool prof_active();
 // a start/stop event pair when launches of `kernel` are being timed
 bool prof_pair(int kernel, hipEvent_t* start, hipEvent_t* stop);

@@ -18,6 +18,7 @@
 // Every kernel reads the domain's Ctl and returns at once if the domain is not
 // stepping, so the same launch sequence serves a batch whose members need
 // different numbers of attempts.
+#include <atomic>
 #include <vector>
 
 #include "fi_device.hpp"
@@ -445,7 +446,8 @@ struct Prof {
     std::vector<hipEvent_t> ev;   // start/stop pairs
     std::vector<int> which;       // kernel of each pair
     size_t used = 0;
-    bool fused = false;           // last attempt ran the fused stencil + row kernel
+    std::atomic<bool> fused{false};   // last attempt ran the fused stencil + row kernel
+    RhsShape last{};              // launch shape of the last attempt
 };
 Prof g_prof;
 
@@ -470,7 +472,13 @@ void prof_select(int kernel) {
     g_prof.used = 0;
 }
 
-int prof_fused() { return g_prof.fused ? 1 : 0; }
+int prof_fused() { return g_prof.fused.load(std::memory_order_relaxed) ? 1 : 0; }
+void prof_shape(int* out) {
+    const RhsShape& s = g_prof.last;
+    const int v[10] = {s.n, s.fp64 ? 1 : 0, prof_fused(), s.fast ? 1 : 0, s.threads_row, s.C,
+                       s.row_tiles, s.srow_rows, s.threads_inv, s.dt_method};
+    for (int i = 0; i < 10; ++i) out[i] = v[i];
+}
 bool prof_active() { return g_prof.kernel >= 0; }
 
 int prof_collect_kernel(int kernel, double* total_ms, long long* launches) {
@@ -503,7 +511,8 @@ void launch_attempt_t(const Member* d_mem, int nmem, const RhsShape& s, hipStrea
     dim3 gs((s.n + ST_TX - 1) / ST_TX, (s.n + ST_TY - 1) / ST_TY, nmem);
     // uniform spacing, power-of-two N: stencil and forward row pass in one kernel
     const bool fused = s.fused && s.fast && s.uniform && launch_fast_pass(d_mem, nmem, s, st, 0);
-    g_prof.fused = fused;
+    g_prof.fused.store(fused, std::memory_order_relaxed);
+    if (g_prof.kernel >= 0) g_prof.last = s;   // profiling runs (one host thread): the timed launches' shape
     if (!fused) {
         if (s.uniform) launch_k(0, k_stencil_u<T>, gs, dim3(256), 0, st, d_mem);
         else launch_k(0, k_stencil<T, false>, gs, dim3(256), 0, st, d_mem);

@@ -441,16 +441,17 @@ __global__ void __launch_bounds__(THREADS) kf_srow_fwd(const Member* __restrict_
 // control block in ctl_next; block 0 commits it (the other blocks read
 // ctl_next, so nothing reads the block being written).
 //
-// Block order is XCD-aware (as in the row passes): a domain's column blocks
-// run on one XCD and share its L2 for the 128-byte lines that neighbouring
-// blocks split between them when a block holds fewer than 16 line pairs
-// (8 line pairs: 13.9 us per launch on the 64-member ensemble against 15.0
-// with the plain order; 16: within 2%).
+// Blocks whose rows are narrower than a 128-byte line (2C values of T) are
+// ordered XCD-aware (as in the row passes): a domain's column blocks run on
+// one XCD and the neighbours that split a line share its L2 (8 line pairs of
+// FP32: 13.9 us per launch on the 64-member ensemble against 15.0 in the
+// plain order).  Blocks of whole lines keep the plain order, which measured
+// about 2% faster for them.
 template <typename T, int N, int C, bool COMMIT>
 __global__ void __launch_bounds__(C * (N / 8)) kf_col(const Member* __restrict__ mem, const void* __restrict__ twv,
                                                      const double2* __restrict__ tw64) {
-    int mi, cblk;
-    xcd_order(mi, cblk);
+    int mi = blockIdx.y, cblk = blockIdx.x;
+    if constexpr (2 * C * sizeof(T) < 128) xcd_order(mi, cblk);
     const Member& M = mem[mi];
     const kptr<Ctl> c = kview(COMMIT ? M.ctl_next : M.ctl);
     // block 0 commits after its last load: a one-lane store ahead of the loads
