@@ -34,7 +34,7 @@ EXPORTS = ["fi_last_error", "fi_version", "fi_padded_size", "fi_isos_create", "f
            "fi_prof_select", "fi_prof_collect", "fi_prof_collect_kernel", "fi_prof_fused", "fi_prof_shape", "fi_bench_copy",
            "fi_isos_write_init", "fi_isos_write_step", "fi_bsl_write_init", "fi_bsl_write_step", "fi_autotune",
            "fi_server_create", "fi_server_admit", "fi_server_release", "fi_server_submit", "fi_server_pending",
-           "fi_server_run", "fi_server_completed", "fi_server_failed", "fi_server_stats", "fi_server_destroy"]
+           "fi_server_run", "fi_server_poll", "fi_server_completed", "fi_server_failed", "fi_server_stats", "fi_server_destroy"]
 
 
 class FiParams(ctypes.Structure):
@@ -136,6 +136,7 @@ def load():
     L.fi_server_submit.argtypes = [vp, i, dp, d, ll]
     L.fi_server_pending.argtypes = [vp, ctypes.POINTER(i)]
     L.fi_server_run.argtypes = [vp, ctypes.POINTER(i)]
+    L.fi_server_poll.argtypes = [vp, i, ctypes.POINTER(i)]
     L.fi_server_completed.argtypes = [vp, ctypes.POINTER(ll), i, ctypes.POINTER(i)]
     L.fi_server_failed.argtypes = [vp, ctypes.POINTER(ll), i, ctypes.POINTER(i)]
     L.fi_server_stats.argtypes = [vp, ctypes.POINTER(ll), ctypes.POINTER(ll), ctypes.POINTER(ll), dp]

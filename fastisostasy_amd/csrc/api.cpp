@@ -163,17 +163,21 @@ struct TunedShape {
 // Rows per block of the fused stencil pass with one column per thread
 // (FI_SROW_ROWS = 4 | 8, or the tuned value; 0 keeps 2L rows): more,
 // shorter row blocks for small batches.  Block size of the inverse row pass
-// (FI_INV_THREADS, or the tuned value; 0 keeps threads_row).
-void small_batch_shape(RhsShape& s, bool tuned, const TunedShape& ts, bool one_col_ok,
+// (FI_INV_THREADS, or the tuned value; 0 keeps threads_row).  Untuned, a
+// group whose default row blocks would not give every CU one (256) takes
+// 4-row fused blocks and 64-thread inverse blocks, the shapes the autotuner
+// picks for a single 256^2 domain (0.95 ms per 10-yr update against 1.16).
+void small_batch_shape(RhsShape& s, int nmem, bool tuned, const TunedShape& ts, bool one_col_ok,
                        const std::function<bool(int)>& inv_ok) {
     s.srow_rows = 0;
     s.threads_inv = 0;
+    const bool few = (long long)(s.n / (2 * s.L)) * nmem < 256;
     if (s.fused && one_col_ok) {
-        int r = tuned ? ts.srow_rows : 0;
+        int r = tuned ? ts.srow_rows : (few ? 4 : 0);
         if (const char* e = std::getenv("FI_SROW_ROWS")) r = std::atoi(e);
         if (r == 4 || r == 8) s.srow_rows = r;
     }
-    int th = tuned ? ts.threads_inv : 0;
+    int th = tuned ? ts.threads_inv : (few ? std::max(64, s.n / 8) : 0);
     if (const char* e = std::getenv("FI_INV_THREADS")) th = std::atoi(e);
     if (th && th != s.threads_row && inv_ok(th)) s.threads_inv = th;
 }
@@ -232,7 +236,7 @@ int make_shape(int n, int nmem, bool fp64, bool uniform, RhsShape& s) {
             s.C = best_c;
             s.threads_col = best_c * tpl;
             // (the inverse row pass keeps 384 threads: its corner sums take whole waves)
-            small_batch_shape(s, tuned, ts, n == 384, [](int) { return false; });
+            small_batch_shape(s, nmem, tuned, ts, n == 384, [](int) { return false; });
             Twiddles t;
             int rc = get_twiddles(n, t);
             if (rc) return rc;
@@ -296,7 +300,7 @@ int make_shape(int n, int nmem, bool fp64, bool uniform, RhsShape& s) {
             s.C = best_c;
             s.threads_col = best_c * tpl;
             const bool one_col = (n <= 256 || (n == 512 && !fp64)) && row_ok(n);
-            small_batch_shape(s, tuned, ts, one_col, [&](int th) {
+            small_batch_shape(s, nmem, tuned, ts, one_col, [&](int th) {
                 return (th == 64 || th == 128 || th == 256 || (th == 512 && n == 512 && !fp64)) && row_ok(th) &&
                        (s.row_tiles == 1 || n % (4 * (th / tpl)) == 0);
             });
@@ -1878,6 +1882,23 @@ struct fi_server {
     std::vector<hipEvent_t> ev;   // one per concurrent group: its last readback
     hipEvent_t start = nullptr;   // library-stream work queued before the round
     std::vector<long long> failed;   // tags of requests whose update failed its checks
+    // Continuous scheduling (fi_server_poll): stream k runs at most one group
+    // at a time, staged in its own range of lane_cap Member copies and
+    // summaries (d_mem / h_stage / sb from k * lane_cap); a domain whose
+    // oldest request is in flight stays in its queue until the group retires.
+    struct Lane {
+        bool busy = false;
+        std::vector<int> slots;
+        std::vector<double> times;
+        UpdateJob job;
+        hipEvent_t ev = nullptr;
+        long long seq = 0;
+        std::tuple<int, int, int, int, int, int, int, int> key;   // launch key (fi_server_poll's grouping)
+    };
+    Lane lane[NSTREAM];
+    int lane_cap = 0;
+    long long lane_seq = 0;
+    std::vector<int> inflight;    // per slot: lane + 1 while its front request is in flight, else 0
 };
 
 namespace {
@@ -1970,6 +1991,8 @@ int fi_server_pending(fi_server* s, int* n) {
 int fi_server_run(fi_server* s, int* completed) {
     if (!s || !completed) return fail(FI_E_ARG, "NULL argument");
     *completed = 0;
+    for (const auto& l : s->lane)
+        if (l.busy) return fail(FI_E_STATE, "fi_server_poll work in flight: poll until it has retired");
     // oldest pending request of each domain, oldest first
     std::vector<int> cand;
     for (int i = 0; i < (int)s->slots.size(); ++i)
@@ -2158,6 +2181,230 @@ int fi_server_run(fi_server* s, int* completed) {
     return 0;
 }
 
+// Continuous scheduling: finished rounds are followed up (another round, or
+// the group retires), then the oldest requests of idle domains are grouped as
+// in fi_server_run and launched on idle streams, without waiting for the
+// groups already running.  A domain whose bsl object is shared with a domain
+// in flight (or with one of an earlier group of this call) waits, so that
+// shared objects see their updates in order.
+int fi_server_poll(fi_server* s, int wait, int* completed) {
+    if (!s || !completed) return fail(FI_E_ARG, "NULL argument");
+    *completed = 0;
+    constexpr int NS = fi_server::NSTREAM;
+    s->inflight.resize(s->slots.size(), 0);
+    int first_rc = 0;
+    auto retire = [&](fi_server::Lane& l, int status) {
+        for (int i : l.slots) {
+            fi_server::Slot& sl = s->slots[i];
+            const bool checks = status == FI_E_UNSTABLE || status == FI_E_NAN;
+            const bool bad = status != 0 && (!checks || sl.h->last_status != 0);
+            (bad ? s->failed : s->done).push_back(sl.q.front().tag);
+            sl.q.pop_front();
+            s->inflight[i] = 0;
+            if (!bad) ++*completed;
+            ++s->served;
+        }
+        ++s->groups;
+        l.busy = false;
+        l.slots.clear();
+        if (status && !first_rc) first_rc = status;
+    };
+    // 1. follow up the lanes whose last round has completed (with `wait`, block
+    //    on the oldest busy lane when no round has completed yet)
+    bool lost = false;   // a launch failed: no new groups in this call
+    auto follow = [&](fi_server::Lane& l) {
+        bool more = false;
+        int rc = job_more(l.job, more);
+        if (!rc && more) {
+            rc = job_round(l.job);
+            if (!rc && hipEventRecord(l.ev, l.job.st) != hipSuccess) rc = fail(FI_E_HIP, "hipEventRecord");
+            if (rc) {
+                server_drain(s);
+                more = false;
+                lost = true;
+            }
+        }
+        if (!more) retire(l, rc ? rc : job_end(l.job));
+    };
+    bool progressed = false;
+    for (auto& l : s->lane) {
+        if (!l.busy) continue;
+        const hipError_t q = hipEventQuery(l.ev);
+        if (q == hipErrorNotReady) continue;
+        if (q != hipSuccess) {
+            server_drain(s);
+            return fail(FI_E_HIP, "hipEventQuery: serving lane lost");
+        }
+        follow(l);
+        progressed = true;
+    }
+    // 2. new groups on idle lanes.  One group per launch key in flight at a
+    // time: requests of a key that arrive while its group runs gather into
+    // the next one, so groups grow with the load.  On the serving mix
+    // (40 domains of 4 sizes, Poisson arrivals at 7100 requests/s) this gave
+    // p50/p99 5.8/16.7 ms against 9.9/39.9 with a new group per idle lane, 8.4/30.5
+    // with two per key, and 8.4/29.5 for rounds; at 1.2x the unbatched
+    // capacity 1.85/4.2 against 1.9/4.4 ms.  FI_SERVER_LANES limits the
+    // lanes in use, FI_SERVER_KEY_LANES the groups of one key (tuning).
+    static const int max_lanes = [] {
+        const char* e = std::getenv("FI_SERVER_LANES");
+        const int v = e ? std::atoi(e) : fi_server::NSTREAM;
+        return std::max(1, std::min(v, (int)fi_server::NSTREAM));
+    }();
+    static const int key_lanes = [] {
+        const char* e = std::getenv("FI_SERVER_KEY_LANES");
+        return e ? std::max(1, std::atoi(e)) : 1;
+    }();
+    int idle = 0;
+    for (int k = 0; k < max_lanes; ++k) idle += !s->lane[k].busy;
+    std::vector<int> cand;
+    for (int i = 0; i < (int)s->slots.size(); ++i)
+        if (s->slots[i].h && !s->slots[i].q.empty() && !s->inflight[i]) cand.push_back(i);
+    if (idle > 0 && !cand.empty() && !lost) {
+        // stage ranges: sized while no lane is busy (a reallocation drains the streams)
+        bool any_busy = false;
+        for (const auto& l : s->lane) any_busy |= l.busy;
+        const int want = std::min<int>(s->max_batch, (int)s->slots.size());
+        if (!any_busy && want > s->lane_cap) {
+            int rc = server_reserve(s, NS * want);
+            if (rc) return rc;
+            s->lane_cap = want;
+        }
+        std::stable_sort(cand.begin(), cand.end(),
+                         [&](int a, int b) { return s->slots[a].q.front().tag < s->slots[b].q.front().tag; });
+        // bsl object -> lane in flight (0 .. NS-1), new group (NS + index), or
+        // -1: a domain using it waits in this call, so later ones wait too
+        std::map<const fi_bsl*, int> bsl_owner;
+        for (int k = 0; k < NS; ++k)
+            if (s->lane[k].busy)
+                for (int i : s->lane[k].slots) bsl_owner[s->slots[i].b] = k;
+        auto key_of = [&](int i) {
+            const fi_isos* h = s->slots[i].h;
+            const bool d = diag_due(h, s->slots[i].q.front().time);
+            return std::make_tuple(h->n, (int)h->fp64, (int)h->uniform, h->par.dt_method, h->par.include_elastic,
+                                   h->par.heterogeneous_ssh, (int)d, h->par.method);
+        };
+        std::vector<std::vector<int>> grp;
+        std::vector<decltype(key_of(0))> keys;
+        std::map<decltype(key_of(0)), int> per_key;   // groups of a key in flight or formed here
+        for (const auto& l : s->lane)
+            if (l.busy) ++per_key[l.key];
+        for (int i : cand) {
+            const auto k = key_of(i);
+            size_t g = 0;
+            for (; g < keys.size(); ++g)
+                if (keys[g] == k && (int)grp[g].size() < s->lane_cap) break;
+            auto it = bsl_owner.find(s->slots[i].b);
+            if (it != bsl_owner.end() && it->second != (int)g + NS) {   // shared with other work: wait
+                it->second = -1;
+                continue;
+            }
+            if (g == keys.size()) {
+                if ((int)keys.size() == idle || per_key[k] >= key_lanes) {   // no lane for another group
+                    bsl_owner[s->slots[i].b] = -1;
+                    continue;
+                }
+                keys.push_back(k);
+                grp.emplace_back();
+                ++per_key[k];
+            }
+            grp[g].push_back(i);
+            bsl_owner[s->slots[i].b] = (int)g + NS;
+        }
+        if (!grp.empty()) {
+            for (int k = 0; k < NS; ++k)
+                if (!s->st[k]) HIPCHK(hipStreamCreateWithFlags(&s->st[k], hipStreamNonBlocking));
+            // new groups start after everything already queued on the library stream
+            if (!s->start) HIPCHK(hipEventCreateWithFlags(&s->start, hipEventDisableTiming));
+            HIPCHK(hipEventRecord(s->start, g_stream));
+        }
+        int k = 0;
+        for (size_t gi = 0; gi < grp.size(); ++gi) {
+            while (s->lane[k].busy) ++k;
+            fi_server::Lane& l = s->lane[k];
+            if (!l.ev) HIPCHK(hipEventCreateWithFlags(&l.ev, hipEventDisableTiming));
+            hipStream_t gst = s->st[k];
+            HIPCHK(hipStreamWaitEvent(gst, s->start, 0));
+            const auto& g = grp[gi];
+            const int nm = (int)g.size(), base = k * s->lane_cap;
+            std::vector<fi_isos*> hs;
+            l.times.clear();
+            for (int q = 0; q < nm; ++q) {
+                fi_server::Slot& sl = s->slots[g[q]];
+                fi_isos* h = sl.h;
+                const fi_server::Req& r = sl.q.front();
+                if (!r.H.empty())
+                    HIPCHK(hipMemcpyAsync(h->hm.H_in, r.H.data(), sizeof(double) * r.H.size(), hipMemcpyHostToDevice,
+                                          gst));
+                Member M = h->hm;
+                M.bsl = sl.b->d;
+                M.tf_req = r.time;
+                M.bsl_lead = 1;
+                M.bsl_next = -1;
+                for (int p = q - 1; p >= 0; --p)
+                    if (s->slots[g[p]].b == sl.b) { M.bsl_lead = 0; break; }
+                for (int p = q + 1; p < nm; ++p)
+                    if (s->slots[g[p]].b == sl.b) { M.bsl_next = p; break; }
+                s->h_stage[base + q] = M;
+                hs.push_back(h);
+                l.times.push_back(r.time);
+            }
+            HIPCHK(hipMemcpyAsync(s->d_mem + base, s->h_stage + base, sizeof(Member) * nm, hipMemcpyHostToDevice,
+                                  gst));
+            const fi_isos* h0 = hs[0];
+            auto sk = std::make_tuple(h0->n, (int)h0->fp64, (int)h0->uniform, nm, h0->par.method, h0->par.dt_method);
+            auto it = s->shapes.find(sk);
+            if (it == s->shapes.end()) {
+                RhsShape shp{};
+                int rc = make_shape(h0->n, nm, h0->fp64, h0->uniform, shp);
+                if (rc) { server_drain(s); return rc; }
+                shp.method = h0->par.method;
+                shp.dt_method = h0->par.dt_method;
+                it = s->shapes.emplace(sk, shp).first;
+            }
+            UpdateJob& j = l.job;
+            j = UpdateJob{};
+            j.hs = hs;
+            j.d_mem = s->d_mem + base;
+            j.shape = &it->second;
+            j.sb = SumBuf{s->sb.d + base, s->sb.h + base, nm};
+            j.times = l.times.data();
+            j.stepping = true;
+            j.graphs = false;
+            j.subbatch = false;
+            j.st = gst;
+            int rc = job_begin(j);
+            if (!rc) rc = job_round(j);
+            if (!rc) rc = hipEventRecord(l.ev, gst) == hipSuccess ? 0 : fail(FI_E_HIP, "hipEventRecord");
+            if (rc) {
+                // partly queued launches: let them finish; the group stays queued
+                server_drain(s);
+                return rc;
+            }
+            l.busy = true;
+            l.slots = g;
+            l.key = keys[gi];
+            l.seq = ++s->lane_seq;
+            for (int i : g) s->inflight[i] = k + 1;
+            progressed = true;
+        }
+    }
+    // 3. nothing done and something in flight: wait for the oldest lane
+    if (wait && !progressed) {
+        fi_server::Lane* old = nullptr;
+        for (auto& l : s->lane)
+            if (l.busy && (!old || l.seq < old->seq)) old = &l;
+        if (old) {
+            if (hipEventSynchronize(old->ev) != hipSuccess) {
+                server_drain(s);
+                return fail(FI_E_HIP, "hipEventSynchronize: serving lane lost");
+            }
+            follow(*old);
+        }
+    }
+    return first_rc;
+}
+
 int fi_server_failed(fi_server* s, long long* tags, int max, int* n) {
     if (!s || !n || (max > 0 && !tags)) return fail(FI_E_ARG, "NULL argument");
     const int k = std::min<int>(max, (int)s->failed.size());
@@ -2195,6 +2442,8 @@ int fi_server_destroy(fi_server* s) {
     for (hipStream_t t : s->st)
         if (t) (void)hipStreamDestroy(t);
     for (hipEvent_t e : s->ev) (void)hipEventDestroy(e);
+    for (auto& l : s->lane)
+        if (l.ev) (void)hipEventDestroy(l.ev);
     if (s->start) (void)hipEventDestroy(s->start);
     delete s;
     return 0;

@@ -49,6 +49,14 @@ class Server:
         check(load().fi_server_run(self.h, ctypes.byref(n)))
         return n.value
 
+    def poll(self, wait=False):
+        """Continuous scheduling (fi_server_poll): follow up finished work and
+        launch idle domains' requests on idle streams; returns the number of
+        requests retired by this call."""
+        n = ctypes.c_int()
+        check(load().fi_server_poll(self.h, 1 if wait else 0, ctypes.byref(n)))
+        return n.value
+
     def pending(self):
         n = ctypes.c_int()
         check(load().fi_server_pending(self.h, ctypes.byref(n)))
@@ -80,9 +88,36 @@ class Server:
             self.h = ctypes.c_void_p()
 
 
-def closed_loop_rate(server, slots, t_now, seconds, step=10.0):
+def closed_loop_rate(server, slots, t_now, seconds, step=10.0, continuous=False):
     """Throughput (requests/s) with every domain always holding one pending
-    request.  t_now: per-slot current model times (updated in place)."""
+    request.  t_now: per-slot current model times (updated in place).  With
+    `continuous`, fi_server_poll schedules and a domain's next request is
+    submitted as soon as its last one completes (the work still in flight at
+    the end is drained outside the timed span)."""
+    if continuous:
+        owner = {}
+        tag = 0
+        for k, s in enumerate(slots):
+            t_now[k] += step
+            server.submit(s, t_now[k], tag)
+            owner[tag] = k
+            tag += 1
+        served = 0
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < seconds:
+            server.poll()
+            for done in server.completed():
+                served += 1
+                k = owner.pop(done)
+                t_now[k] += step
+                server.submit(slots[k], t_now[k], tag)
+                owner[tag] = k
+                tag += 1
+        wall = time.perf_counter() - t0
+        while server.pending():
+            server.poll(wait=True)
+        server.completed()
+        return served / wall
     tag = 0
     served = 0
     t0 = time.perf_counter()
@@ -97,11 +132,12 @@ def closed_loop_rate(server, slots, t_now, seconds, step=10.0):
     return served / (time.perf_counter() - t0)
 
 
-def open_loop(server, slots, t_now, arrivals, domains, step=10.0, progress=0):
+def open_loop(server, slots, t_now, arrivals, domains, step=10.0, progress=0, continuous=False):
     """Serve a request stream with arrival times `arrivals` (seconds from the
     start) on domain indices `domains`; returns per-request latencies
     (seconds, arrival to completion) and the wall time.  With `progress` > 0
-    a line goes to stderr every `progress` completed requests."""
+    a line goes to stderr every `progress` completed requests.  With
+    `continuous`, fi_server_poll schedules instead of rounds."""
     n = len(arrivals)
     lat = np.full(n, np.nan)
     i = 0
@@ -122,7 +158,10 @@ def open_loop(server, slots, t_now, arrivals, domains, step=10.0, progress=0):
                 if wait > 0:
                     time.sleep(min(wait, 1e-3))
             continue
-        server.run()
+        if continuous:
+            server.poll()
+        else:
+            server.run()
         t_end = time.perf_counter() - t0
         for tag in server.completed():
             lat[tag] = t_end - arrivals[tag]

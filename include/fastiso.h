@@ -229,6 +229,19 @@ int fi_batch_destroy(fi_batch* b);
  *                      one batched update per group (with max_batch > 1 the
  *                      groups run concurrently on separate streams unless two
  *                      share a bsl object); returns the count served
+ *   fi_server_poll     continuous scheduling instead of rounds: follows up the
+ *                      groups whose last launch round has completed (another
+ *                      round, or retire), then launches the oldest requests of
+ *                      idle domains, grouped as in fi_server_run, on idle
+ *                      streams (one group per stream of the server's 8, one
+ *                      group per launch key at a time, so that requests arriving
+ *                      meanwhile gather into the next), without waiting for the
+ *                      groups still running; with wait != 0 it
+ *                      blocks for the oldest group when nothing else could be
+ *                      done.  A domain whose bsl object another domain in flight
+ *                      uses waits.  completed: requests retired by this call;
+ *                      pending counts requests in flight.  fi_server_run refuses
+ *                      (FI_E_STATE) while polled work is in flight.
  *   fi_server_completed  tags of served requests, in completion order
  *   fi_server_failed     tags of requests whose update ran but failed the
  *                      instability / NaN checks (the domain's FI_S_STATUS
@@ -244,6 +257,7 @@ int fi_server_release(fi_server* s, int slot);
 int fi_server_submit(fi_server* s, int slot, const double* H_ice, double time, long long tag);
 int fi_server_pending(fi_server* s, int* n);
 int fi_server_run(fi_server* s, int* completed);
+int fi_server_poll(fi_server* s, int wait, int* completed);
 int fi_server_completed(fi_server* s, long long* tags, int max, int* n);
 int fi_server_failed(fi_server* s, long long* tags, int max, int* n);
 int fi_server_stats(fi_server* s, long long* rounds, long long* groups, long long* served, double* hbm_used);

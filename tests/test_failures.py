@@ -215,3 +215,47 @@ def test_gpu_server_fails_a_request_once(max_batch):
         assert a.scalar("t") == s.scalar("t") == 10.0
         assert np.array_equal(a.out("w"), s.out("w"))
     srv.close()
+
+
+@pytest.mark.gpu
+def test_gpu_server_poll_fails_a_request_once():
+    """The same failing domain under continuous scheduling (fi_server_poll):
+    the poll that retires its group returns FI_E_UNSTABLE, the failed tag is
+    reported once, the other requests complete, nothing is retried, and every
+    domain's state equals that of single updates."""
+    print("case: server poll with a failing domain", flush=True)
+    from fastisostasy_amd import isostasy as G
+    from fastisostasy_amd import serving as S
+
+    def dom(nx, scale):
+        p = G.make_params(**_params(method=1))
+        g = G.IsosClass(p, nx, nx, DX)
+        gb = G.bsl_init("const", 0.0)
+        g.init_state(np.full((nx, nx), 1e6), np.zeros((nx, nx)), 0.0, gb)
+        return g, gb, W.disc_load(nx, DX) * scale
+
+    spec = [(32, 1.0), (32, 1000.0), (48, 1.0)]
+    served = [dom(nx, s) for nx, s in spec]
+    alone = [dom(nx, s) for nx, s in spec]
+    srv = S.Server(hbm_budget_bytes=8e9, max_batch=8)
+    slots = [srv.admit(g, gb) for g, gb, _ in served]
+    for k, (g, gb, H) in enumerate(served):
+        srv.submit(slots[k], 10.0, k, H_ice=H)
+    codes = []
+    while srv.pending():
+        try:
+            srv.poll(wait=True)
+        except Exception as ex:   # the retiring poll reports the failed checks
+            codes.append(getattr(ex, "code", None))
+    assert codes == [FI_E_UNSTABLE]
+    assert srv.failed() == [1]
+    assert sorted(srv.completed()) == [0, 2]
+    for k, (g, gb, H) in enumerate(alone):
+        if k == 1:
+            _code(lambda: g.update(H, 10.0, gb))
+        else:
+            g.update(H, 10.0, gb)
+    for (a, _, _), (s, _, _) in zip(alone, served):
+        assert a.scalar("t") == s.scalar("t") == 10.0
+        assert np.array_equal(a.out("w"), s.out("w"))
+    srv.close()
