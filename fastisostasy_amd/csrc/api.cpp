@@ -167,7 +167,7 @@ struct TunedShape {
 // group whose default row blocks would not give every CU one (256) takes
 // 4-row fused blocks and 64-thread inverse blocks, the shapes the autotuner
 // picks for a single 256^2 domain (0.95 ms per 10-yr update against 1.16).
-void small_batch_shape(RhsShape& s, int nmem, bool tuned, const TunedShape& ts, bool one_col_ok,
+void small_batch_shape(RhsShape& s, int nmem, bool tuned, const TunedShape& ts, bool one_col_ok, int def_inv,
                        const std::function<bool(int)>& inv_ok) {
     s.srow_rows = 0;
     s.threads_inv = 0;
@@ -177,7 +177,7 @@ void small_batch_shape(RhsShape& s, int nmem, bool tuned, const TunedShape& ts, 
         if (const char* e = std::getenv("FI_SROW_ROWS")) r = std::atoi(e);
         if (r == 4 || r == 8) s.srow_rows = r;
     }
-    int th = tuned ? ts.threads_inv : (few ? std::max(64, s.n / 8) : 0);
+    int th = tuned ? ts.threads_inv : (few ? def_inv : 0);
     if (const char* e = std::getenv("FI_INV_THREADS")) th = std::atoi(e);
     if (th && th != s.threads_row && inv_ok(th)) s.threads_inv = th;
 }
@@ -235,8 +235,8 @@ int make_shape(int n, int nmem, bool fp64, bool uniform, RhsShape& s) {
             s.L = L;
             s.C = best_c;
             s.threads_col = best_c * tpl;
-            // (the inverse row pass keeps 384 threads: its corner sums take whole waves)
-            small_batch_shape(s, nmem, tuned, ts, n == 384, [](int) { return false; });
+            // (inverse row blocks of 384 or 192 threads: the corner sums take whole waves)
+            small_batch_shape(s, nmem, tuned, ts, n == 384, 192, [](int th) { return th == 192; });
             Twiddles t;
             int rc = get_twiddles(n, t);
             if (rc) return rc;
@@ -300,7 +300,7 @@ int make_shape(int n, int nmem, bool fp64, bool uniform, RhsShape& s) {
             s.C = best_c;
             s.threads_col = best_c * tpl;
             const bool one_col = (n <= 256 || (n == 512 && !fp64)) && row_ok(n);
-            small_batch_shape(s, nmem, tuned, ts, one_col, [&](int th) {
+            small_batch_shape(s, nmem, tuned, ts, one_col, std::max(64, n / 8), [&](int th) {
                 return (th == 64 || th == 128 || th == 256 || (th == 512 && n == 512 && !fp64)) && row_ok(th) &&
                        (s.row_tiles == 1 || n % (4 * (th / tpl)) == 0);
             });
@@ -2567,9 +2567,9 @@ std::vector<TunedShape> tune_candidates(int n, int nmem) {
     // row pass with its own block size
     if ((p2 || r3) && nmem <= 8)
         for (int rows : {4, 8})
-            for (int inv : {0, 64, 128})
+            for (int inv : {0, 64, 128, 192})
                 for (int C : {1, 2, 4, 8}) {
-                    if (r3 && inv) continue;   // the N = 3 * 2^k inverse pass keeps 384 threads
+                    if (r3 ? (inv != 0 && inv != 192) : inv == 192) continue;   // 3 * 2^k: 384 or 192 threads
                     TunedShape t;
                     t.threads_row = p2 ? std::min(n, 256) : 0; t.C = C; t.row_tiles = 1; t.K = 1;
                     t.srow_rows = rows; t.threads_inv = inv;

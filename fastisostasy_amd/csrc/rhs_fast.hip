@@ -686,11 +686,10 @@ __global__ void __launch_bounds__(THREADS) kf_row_inv(const Member* __restrict__
         // terms, twiddles, spectrum, X (consumed last).
         // The four corner sums over the N per-column terms of the column pass:
         // quantity q is summed lane-strided, then by a fixed butterfly, by wave
-        // q (wave 0 .. NW-1 take q, q + NW, ... when there are fewer than four
-        // waves; with more, waves 4.. repeat waves 0..3), so the order is set by
-        // N alone.
-        constexpr int NW = THREADS / 64, QPW = NW >= 4 ? 1 : 4 / NW, CPL = N / 64;
-        static_assert(NW >= 1 && (NW >= 4 || 4 % NW == 0) && N % 64 == 0, "corner-sum layout");
+        // q (wave w of NW < 4 takes q = w, w + NW, ... below 4; with more, waves
+        // 4.. repeat waves 0..3), so the order is set by N alone.
+        constexpr int NW = THREADS / 64, QPW = NW >= 4 ? 1 : (4 + NW - 1) / NW, CPL = N / 64;
+        static_assert(NW >= 1 && THREADS % 64 == 0 && N % 64 == 0, "corner-sum layout");
         const int wave = tid >> 6, lane = tid & 63;
         double cq[QPW][CPL];
         {
@@ -698,8 +697,9 @@ __global__ void __launch_bounds__(THREADS) kf_row_inv(const Member* __restrict__
 #pragma unroll
             for (int j = 0; j < QPW; ++j) {
                 const int qi = NW >= 4 ? (wave & 3) : wave + NW * j;
+                const int qc = qi < 4 ? qi : 0;   // (a wave past the four quantities loads one it discards)
 #pragma unroll
-                for (int i = 0; i < CPL; ++i) cq[j][i] = cp[(size_t)(lane + 64 * i) * 4 + qi];
+                for (int i = 0; i < CPL; ++i) cq[j][i] = cp[(size_t)(lane + 64 * i) * 4 + qc];
             }
         }
         __builtin_amdgcn_sched_barrier(0);
@@ -734,7 +734,8 @@ __global__ void __launch_bounds__(THREADS) kf_row_inv(const Member* __restrict__
 #pragma unroll
             for (int i = 0; i < CPL; ++i) q += cq[j][i];
             q = wave_sum(q);
-            if (lane == 0) qs[NW >= 4 ? (wave & 3) : wave + NW * j] = q;
+            const int qi = NW >= 4 ? (wave & 3) : wave + NW * j;
+            if (lane == 0 && qi < 4) qs[qi] = q;
         }
         // (no branch around the store when the threads cover the table evenly:
         // a join here would make the compiler wait for every load in flight)
@@ -869,8 +870,10 @@ static bool fast_n(const Member* d, int nmem, const RhsShape& s, hipStream_t st,
         if (which == 0)
             return s.srow_rows ? srow_fwd_rows<T, N, 384>(d, nmem, st, s.tw, s.srow_rows)
                                : srow_fwd<T, N, 384>(d, nmem, st, s.tw);
-        which == 1 ? rows_fwd<T, N, 384>(d, nmem, st, s.tw)
-                   : rows_inv<T, N, 384>(d, nmem, st, s.tw, ncolblk, s.row_tiles, s.dt_method);
+        if (which == 1) rows_fwd<T, N, 384>(d, nmem, st, s.tw);
+        // the inverse pass also in 192-thread blocks (3 waves) for small batches
+        else if (s.threads_inv == 192) rows_inv<T, N, 192>(d, nmem, st, s.tw, ncolblk, s.row_tiles, s.dt_method);
+        else rows_inv<T, N, 384>(d, nmem, st, s.tw, ncolblk, s.row_tiles, s.dt_method);
         return true;
     } else {
     if (which == 0) {

@@ -31,14 +31,16 @@ def _run(nx, prec, steps=4, seed=7):
     return out
 
 
+# the built-in shapes of one domain are the small-batch ones (4-row fused
+# blocks, inverse blocks of 64 threads, 192 at N = 384); each case sets others
 @pytest.mark.parametrize("nx,prec,env", [
-    (256, "mixed", {"FI_SROW_ROWS": "4", "FI_INV_THREADS": "64", "FI_COL_LINES": "2"}),
+    (256, "mixed", {"FI_SROW_ROWS": "0", "FI_INV_THREADS": "256", "FI_COL_LINES": "16"}),
     (256, "mixed", {"FI_SROW_ROWS": "8", "FI_INV_THREADS": "128"}),
-    (256, "fp64", {"FI_SROW_ROWS": "4", "FI_INV_THREADS": "64", "FI_COL_LINES": "4"}),
-    (128, "mixed", {"FI_SROW_ROWS": "4", "FI_INV_THREADS": "128"}),
-    (384, "mixed", {"FI_SROW_ROWS": "4", "FI_COL_LINES": "2"}),
-    (384, "fp64", {"FI_SROW_ROWS": "8"}),
-    (512, "mixed", {"FI_SROW_ROWS": "4", "FI_INV_THREADS": "256"}),
+    (256, "fp64", {"FI_SROW_ROWS": "0", "FI_INV_THREADS": "256", "FI_COL_LINES": "4"}),
+    (128, "mixed", {"FI_SROW_ROWS": "0", "FI_INV_THREADS": "128"}),
+    (384, "mixed", {"FI_SROW_ROWS": "0", "FI_INV_THREADS": "384", "FI_COL_LINES": "2"}),
+    (384, "fp64", {"FI_SROW_ROWS": "8", "FI_INV_THREADS": "384"}),
+    (512, "mixed", {"FI_SROW_ROWS": "0", "FI_INV_THREADS": "512"}),
 ])
 def test_small_batch_shapes_bitwise(nx, prec, env, monkeypatch):
     print(f"shapes: N={nx} {prec} {env}", flush=True)
