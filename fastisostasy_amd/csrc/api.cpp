@@ -218,8 +218,11 @@ int make_shape(int n, int nmem, bool fp64, bool uniform, RhsShape& s) {
             return C * tpl <= 1024 && ((size_t)n + (size_t)n * (C + 1)) * cb <= 65536 && n % (2 * C) == 0;
         };
         int best_c = 0;
+        // the largest blocks that still give every CU one; no block below a
+        // wave when a wave-sized one exists (one 256^2 domain: 2 line pairs)
         for (int C : {16, 8, 4, 2, 1}) {
             if (!col_ok(C)) continue;
+            if (best_c && C * tpl < 64) break;
             best_c = C;
             if ((size_t)(n / (2 * C)) * nmem >= 256) break;
         }
@@ -277,8 +280,11 @@ int make_shape(int n, int nmem, bool fp64, bool uniform, RhsShape& s) {
         auto col_ok = [&](int C) {
             return C * tpl <= 1024 && ((size_t)n + (size_t)n * (C + 1)) * cb <= 65536 && n % (2 * C) == 0;
         };
+        // the largest blocks that still give every CU one; no block below a
+        // wave when a wave-sized one exists (one 256^2 domain: 2 line pairs)
         for (int C : {16, 8, 4, 2, 1}) {
             if (!col_ok(C)) continue;
+            if (best_c && C * tpl < 64) break;
             best_c = C;
             if ((size_t)(n / (2 * C)) * nmem >= 256) break;
         }
