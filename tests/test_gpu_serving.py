@@ -144,16 +144,18 @@ def test_server_admission_budget():
     srv.close()
 
 
-def test_server_poll_matches_single_updates():
+@pytest.mark.parametrize("max_batch", [1, 64])
+def test_server_poll_matches_single_updates(max_batch):
     """Continuous scheduling (fi_server_poll): the same six-domain mix,
     requests submitted while earlier groups are still running, reproduces
-    single-domain isos_update calls bit for bit; fi_server_run refuses while
-    polled work is in flight."""
-    print("server poll: 6-domain mix", flush=True)
+    single-domain isos_update calls bit for bit (groups of one domain on
+    concurrent streams with max_batch 1); fi_server_run refuses while polled
+    work is in flight."""
+    print(f"server poll: 6-domain mix, max_batch {max_batch}", flush=True)
     spec = [(64, "fp64"), (64, "mixed"), (96, "fp64"), (96, "mixed"), (128, "mixed"), (128, "mixed")]
     alone = [_domain(k, nx, p) for k, (nx, p) in enumerate(spec)]
     served = [_domain(k, nx, p) for k, (nx, p) in enumerate(spec)]
-    srv = S.Server(hbm_budget_bytes=8e9, max_batch=64)
+    srv = S.Server(hbm_budget_bytes=8e9, max_batch=max_batch)
     slots = [srv.admit(h, b) for h, b, _ in served]
     rng = np.random.default_rng(5)
     steps = 6
