@@ -544,7 +544,7 @@ def main():
     print(f"# batched mixed: {args.members} x {n}^2, {args.steps} steps in {res['wall']:.3f} s, "
           f"{res['attempts']:.0f} attempts total", file=sys.stderr, flush=True)
 
-    max_err = rms_err = None
+    max_err = rms_err = fp64_batched = None
     if not args.no_check:
         # the same coupling steps as the mixed run, its timing pass included
         ref = run_batched(args.members, args.steps, args.warmup, "fp64",
@@ -554,6 +554,9 @@ def main():
         rms_err = float(np.sqrt(np.mean([np.mean((a - b) ** 2) for a, b in zip(res["w"], ref["w"])])))
         print(f"# fp64 batch for the error check: {ref['wall']:.3f} s, max |dw| = {max_err:.3e} m",
               file=sys.stderr, flush=True)
+        # the same batch in FP64 throughout (built-in launch shapes), timed the same way
+        fp64_batched = {"value": round(cells * years / ref["wall"], 1),
+                        "ms_per_step": round(1e3 * ref["wall"] / args.steps, 4)}
 
     plain = run_plain(args.plain_members, args.plain_steps, 2)
     plain_value = args.plain_members * plain["n"] ** 2 * 10.0 * args.plain_steps / plain["wall"]
@@ -604,6 +607,7 @@ def main():
                        "steps": args.plain_steps},
         "speedup_vs_plain_fp64": round(value / plain_value, 2),
         "max_err_vs_fp64_m": max_err,
+        "fp64_batched": fp64_batched,
         "rms_err_vs_fp64_m": rms_err,
         "step_latency_ms": ({"p50": serve["continuous"]["p50_ms"], "p99": serve["continuous"]["p99_ms"],
                              "of": "serving requests (continuous scheduling, 1.2 x unbatched capacity)"} if serve
