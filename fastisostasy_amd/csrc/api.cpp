@@ -164,18 +164,24 @@ struct TunedShape {
 // (FI_SROW_ROWS = 4 | 8, or the tuned value; 0 keeps 2L rows): more,
 // shorter row blocks for small batches.  Block size of the inverse row pass
 // (FI_INV_THREADS, or the tuned value; 0 keeps threads_row).  Untuned, a
-// group whose default row blocks would not give every CU one (256) takes
-// 4-row fused blocks and 64-thread inverse blocks, the shapes the autotuner
-// picks for a single 256^2 domain (0.95 ms per 10-yr update against 1.16).
+// group whose 16-row fused blocks or default inverse blocks would not give
+// every CU one (256) takes 4-row fused blocks and 64-thread inverse blocks,
+// the shapes the autotuner picks for a single 256^2 domain (0.95 ms per 10-yr
+// update against 1.16); larger groups whose row blocks hold fewer threads
+// than columns (FP64 at N = 512) take 16-row fused blocks of N threads.
 void small_batch_shape(RhsShape& s, int nmem, bool tuned, const TunedShape& ts, bool one_col_ok, int def_inv,
                        const std::function<bool(int)>& inv_ok) {
     s.srow_rows = 0;
     s.threads_inv = 0;
+    // fewer blocks than CUs: fused blocks of 16 rows, inverse blocks of threads_row
+    const bool few_rows = (long long)(s.n / 16) * nmem < 256;
     const bool few = (long long)(s.n / (2 * s.L)) * nmem < 256;
     if (s.fused && one_col_ok) {
-        int r = tuned ? ts.srow_rows : (few ? 4 : 0);
+        // (16 rows in N threads where the row blocks hold fewer threads than
+        // columns: one column per thread keeps the march's window in registers)
+        int r = tuned && ts.srow_rows ? ts.srow_rows : (few_rows ? 4 : (s.threads_row < s.n ? 16 : 0));
         if (const char* e = std::getenv("FI_SROW_ROWS")) r = std::atoi(e);
-        if (r == 4 || r == 8) s.srow_rows = r;
+        if (r == 4 || r == 8 || r == 16) s.srow_rows = r;
     }
     int th = tuned ? ts.threads_inv : (few ? def_inv : 0);
     if (const char* e = std::getenv("FI_INV_THREADS")) th = std::atoi(e);
@@ -305,7 +311,9 @@ int make_shape(int n, int nmem, bool fp64, bool uniform, RhsShape& s) {
             s.L = best_t / tpl;
             s.C = best_c;
             s.threads_col = best_c * tpl;
-            const bool one_col = (n <= 256 || (n == 512 && !fp64)) && row_ok(n);
+            // one column per thread in the fused pass: N threads (its LDS is
+            // checked where it is launched; beyond 64 KiB it is declared)
+            const bool one_col = n <= 512;
             small_batch_shape(s, nmem, tuned, ts, one_col, std::max(64, n / 8), [&](int th) {
                 return (th == 64 || th == 128 || th == 256 || (th == 512 && n == 512 && !fp64)) && row_ok(th) &&
                        (s.row_tiles == 1 || n % (4 * (th / tpl)) == 0);

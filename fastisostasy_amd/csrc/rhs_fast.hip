@@ -808,14 +808,18 @@ static bool srow_fwd(const Member* d, int nmem, hipStream_t st, const void* tw) 
     }
     return false;
 }
-// one column per thread (THREADS = N, or 384 at N = 384) with 4 or 8 rows
-// per block (srow_rows), for small batches
+// one column per thread (THREADS = N, or 384 at N = 384) with 4, 8 or 16
+// rows per block (srow_rows): 4 and 8 for small batches, 16 where the
+// row blocks' own thread count would give a thread several columns (FP64 at
+// N = 512, whose 256-thread blocks spill the register window)
 template <typename T, int N, int THREADS>
 static bool srow_fwd_rows(const Member* d, int nmem, hipStream_t st, const void* tw, int rows) {
     if constexpr (THREADS / (N / 8) >= 4) {
         if (rows == 4) return srow_fwd<T, N, THREADS, 2>(d, nmem, st, tw);
         if (rows == 8) return srow_fwd<T, N, THREADS, 4>(d, nmem, st, tw);
     }
+    if constexpr (THREADS / (N / 8) >= 8)
+        if (rows == 16) return srow_fwd<T, N, THREADS, 8>(d, nmem, st, tw);
     return false;
 }
 template <typename T, int N, int THREADS, int METH>
@@ -879,8 +883,7 @@ static bool fast_n(const Member* d, int nmem, const RhsShape& s, hipStream_t st,
     if (which == 0) {
         if (s.srow_rows) {
             // one column per thread, 4 or 8 rows per block
-            if constexpr (N <= 256 || (N == 512 && sizeof(T) == 4))
-                return srow_fwd_rows<T, N, N>(d, nmem, st, s.tw, s.srow_rows);
+            if constexpr (N <= 512) return srow_fwd_rows<T, N, N>(d, nmem, st, s.tw, s.srow_rows);
             return false;
         }
         switch (s.threads_row) {

@@ -183,6 +183,7 @@ def test_sealevel_masks_bsl_fp64():
 def _run_gpu(wl, precision, times, fused, threads, monkeypatch):
     from fastisostasy_amd._lib import load
     monkeypatch.setenv("FI_ROW_THREADS", str(threads))
+    monkeypatch.setenv("FI_SROW_ROWS", "0")   # the fused pass in blocks of FI_ROW_THREADS threads
     if fused:
         monkeypatch.delenv("FI_NO_FUSED_STENCIL", raising=False)
     else:

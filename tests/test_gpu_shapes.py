@@ -41,6 +41,8 @@ def _run(nx, prec, steps=4, seed=7):
     (384, "mixed", {"FI_SROW_ROWS": "0", "FI_INV_THREADS": "384", "FI_COL_LINES": "2"}),
     (384, "fp64", {"FI_SROW_ROWS": "8", "FI_INV_THREADS": "384"}),
     (512, "mixed", {"FI_SROW_ROWS": "0", "FI_INV_THREADS": "512"}),
+    (512, "fp64", {"FI_SROW_ROWS": "16", "FI_INV_THREADS": "256"}),
+    (512, "fp64", {"FI_SROW_ROWS": "0", "FI_INV_THREADS": "256", "FI_ROW_THREADS": "256"}),
 ])
 def test_small_batch_shapes_bitwise(nx, prec, env, monkeypatch):
     print(f"shapes: N={nx} {prec} {env}", flush=True)
