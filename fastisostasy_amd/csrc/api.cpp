@@ -603,11 +603,14 @@ int sub_shape(int n, int nsub, bool fp64, bool uniform, int dt_method, const Rhs
 // hipGraphLaunch, so the host issues one call instead of 4 x GRAPH_REPS
 // kernel launches.  Kernel nodes only (no memset or copy nodes); the kernels
 // read the descriptors and the control block at run time, so one graph
-// serves every update of a batch.  Opt-in (FI_GRAPHS=1): measured on the
-// 64-member ensemble they are neutral (the GPU, not the host, bounds the
-// attempt rate) and rocprofv3's kernel tracing crashes the host process
-// inside the graph launch; benchmark kernel timing and the request server
-// (whose groupings change every round) never use them.
+// serves every update of a batch.  Opt-in for batches and single domains
+// (FI_GRAPHS=1): measured on the 64-member ensemble they are neutral (the
+// GPU, not the host, bounds the attempt rate) and rocprofv3's kernel tracing
+// crashed the host process inside a graph launch.  Benchmark kernel timing
+// and round-based serving (whose groupings change every round) never use
+// them; polled serving does for groups of one or two domains, where the host's
+// launches would otherwise set the pace (FI_SERVER_GRAPHS=0 turns that off,
+// e.g. under a kernel tracer).
 constexpr int GRAPH_REPS = 8, GRAPH_CACHE = 128;
 struct GraphEntry {
     const Member* d;
@@ -670,9 +673,11 @@ void purge_graphs(const Member* lo, const Member* hi) {
 }
 
 // `count` attempts: whole graphs, then the remainder as plain launches
-int launch_attempts(const Member* d, int nmem, const RhsShape& s, hipStream_t st, int count, bool graphs) {
+// graphs: 0 plain launches, 1 attempt graphs when FI_GRAPHS is set, 2 attempt graphs
+int launch_attempts(const Member* d, int nmem, const RhsShape& s, hipStream_t st, int count, int graphs) {
     int done = 0;
-    if (graphs && count >= GRAPH_REPS && !prof_active() && std::getenv("FI_GRAPHS")) {
+    const bool use = graphs == 2 || (graphs == 1 && std::getenv("FI_GRAPHS"));
+    if (use && count >= GRAPH_REPS && !prof_active()) {
         hipGraphExec_t ex;
         int rc = attempt_graph(d, nmem, s, st, &ex);
         if (rc) return rc;
@@ -701,7 +706,8 @@ struct UpdateJob {
     SumBuf sb;
     const double* times = nullptr;
     double time = 0.0;
-    bool stepping = true, graphs = true;
+    bool stepping = true;
+    int graphs = 1;              // launch_attempts: 0 plain, 1 graphs under FI_GRAPHS, 2 graphs
     bool subbatch = true;        // may split into concurrent sub-batches (library stream only)
     hipStream_t st = nullptr;
     int K = 1, ahead = 1;
@@ -887,7 +893,7 @@ int job_end(UpdateJob& j) {
 }
 
 int run_update(std::vector<fi_isos*>& hs, Member* d_mem, const RhsShape& shape, SumBuf& sb, double time,
-               bool stepping, const double* times = nullptr, bool graphs = true) {
+               bool stepping, const double* times = nullptr, int graphs = 1) {
     UpdateJob j;
     j.hs = hs;
     j.d_mem = d_mem;
@@ -2384,7 +2390,17 @@ int fi_server_poll(fi_server* s, int wait, int* completed) {
             j.sb = SumBuf{s->sb.d + base, s->sb.h + base, nm};
             j.times = l.times.data();
             j.stepping = true;
-            j.graphs = false;
+            // Attempt graphs for groups of one or two domains: one host call per
+            // 8 attempts instead of 24 launches, so that enqueueing a group does
+            // not hold up the launch of the next (serving mix at 1.2x the
+            // unbatched capacity: p50 1.42 ms against 1.92); each lane stages in
+            // its own range, so a graph serves every later group of its size
+            // and shape on that lane.  FI_SERVER_GRAPHS=0 keeps plain launches.
+            static const bool lane_graphs = [] {
+                const char* e = std::getenv("FI_SERVER_GRAPHS");
+                return !(e && std::atoi(e) == 0);
+            }();
+            j.graphs = lane_graphs && nm <= 2 ? 2 : 0;
             j.subbatch = false;
             j.st = gst;
             int rc = job_begin(j);
