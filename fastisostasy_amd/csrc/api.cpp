@@ -2400,7 +2400,7 @@ int fi_server_poll(fi_server* s, int wait, int* completed) {
                 const char* e = std::getenv("FI_SERVER_GRAPHS");
                 return !(e && std::atoi(e) == 0);
             }();
-            j.graphs = lane_graphs && nm <= 2 ? 2 : 0;
+            j.graphs = lane_graphs && nm <= 2 ? 2 : 0;   // (all group sizes: no gain at high load)
             j.subbatch = false;
             j.st = gst;
             int rc = job_begin(j);
