@@ -622,6 +622,7 @@ struct GraphEntry {
 };
 std::vector<GraphEntry> g_graphs;
 unsigned long long g_graph_clock = 0;
+std::mutex g_graph_mu;   // the cache is shared by every thread's batches and servers
 
 bool same_shape(const RhsShape& a, const RhsShape& b) {
     return a.n == b.n && a.L == b.L && a.C == b.C && a.threads_row == b.threads_row &&
@@ -632,6 +633,7 @@ bool same_shape(const RhsShape& a, const RhsShape& b) {
 }
 
 int attempt_graph(const Member* d, int nmem, const RhsShape& s, hipStream_t st, hipGraphExec_t* out) {
+    std::lock_guard<std::mutex> lk(g_graph_mu);
     ++g_graph_clock;
     for (auto& e : g_graphs)
         if (e.d == d && e.nmem == nmem && e.st == st && same_shape(e.s, s)) {
@@ -662,6 +664,7 @@ int attempt_graph(const Member* d, int nmem, const RhsShape& s, hipStream_t st, 
 
 // graphs over descriptor arrays in [lo, hi) (before that memory is freed)
 void purge_graphs(const Member* lo, const Member* hi) {
+    std::lock_guard<std::mutex> lk(g_graph_mu);
     for (size_t i = 0; i < g_graphs.size();) {
         if (g_graphs[i].d >= lo && g_graphs[i].d < hi) {
             (void)hipGraphExecDestroy(g_graphs[i].exec);

@@ -53,3 +53,47 @@ def test_two_threads_two_handles():
         assert a.scalar("t") == s.scalar("t") == 10.0 * steps
         for f in ("w", "z_bed", "dwdt"):
             assert np.array_equal(a.out(f), s.out(f)), f
+
+
+def test_two_threads_two_polled_servers():
+    """Two request servers, each polled from its own host thread (their
+    attempt graphs share the library's cache): every domain ends bitwise where
+    the same updates run one thread after the other leave it."""
+    from fastisostasy_amd import serving as S
+    print("case: two host threads, two polled servers (N = 64 and 128, mixed)", flush=True)
+    spec = [(0, 48, "mixed"), (1, 48, "mixed"), (2, 96, "mixed"), (3, 96, "mixed")]
+    steps = 6
+
+    def serve(doms, errors):
+        try:
+            srv = S.Server(hbm_budget_bytes=8e9, max_batch=4)
+            slots = [srv.admit(h, b) for h, b, _ in doms]
+            for k in range(1, steps + 1):
+                for (h, b, wl), sl in zip(doms, slots):
+                    srv.submit(sl, 10.0 * k, k, H_ice=wl["H_ice"] if k == 1 else None)
+                while srv.pending():
+                    srv.poll(wait=True)
+            srv.completed()
+            srv.close()
+        except Exception as e:   # reported by the main thread
+            errors.append(e)
+
+    threaded = [_domain(*s) for s in spec]
+    serial = [_domain(*s) for s in spec]
+    errors = []
+    th = [threading.Thread(target=serve, args=(threaded[2 * i:2 * i + 2], errors)) for i in range(2)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join(timeout=120)
+    assert not any(t.is_alive() for t in th)
+    assert not errors, errors
+    _run(serial[0], steps, errors)
+    _run(serial[1], steps, errors)
+    _run(serial[2], steps, errors)
+    _run(serial[3], steps, errors)
+    assert not errors, errors
+    for (a, _, _), (s, _, _) in zip(threaded, serial):
+        assert a.scalar("t") == s.scalar("t") == 10.0 * steps
+        for f in ("w", "z_bed", "dwdt"):
+            assert np.array_equal(a.out(f), s.out(f)), f
