@@ -165,23 +165,26 @@ struct TunedShape {
 // shorter row blocks for small batches.  Block size of the inverse row pass
 // (FI_INV_THREADS, or the tuned value; 0 keeps threads_row).  Untuned, a
 // group whose 16-row fused blocks or default inverse blocks would not give
-// every CU one (256) takes 4-row fused blocks and 64-thread inverse blocks,
-// the shapes the autotuner picks for a single 256^2 domain (0.95 ms per 10-yr
-// update against 1.16); larger groups whose row blocks hold fewer threads
-// than columns (FP64 at N = 512) take 16-row fused blocks of N threads.
+// every CU one (256) takes 4-row (2-row) fused blocks and 64-thread inverse
+// blocks, the shapes the autotuner picks for a single 256^2 domain (0.95 ms
+// per 10-yr update against 1.16); larger groups whose row blocks hold fewer
+// threads than columns (FP64 at N = 512) take 16-row fused blocks of N threads.
 void small_batch_shape(RhsShape& s, int nmem, bool tuned, const TunedShape& ts, bool one_col_ok, int def_inv,
                        const std::function<bool(int)>& inv_ok) {
     s.srow_rows = 0;
     s.threads_inv = 0;
-    // fewer blocks than CUs: fused blocks of 16 rows, inverse blocks of threads_row
-    const bool few_rows = (long long)(s.n / 16) * nmem < 256;
+    // fewer blocks than CUs: fused blocks of 16 (or 4) rows, inverse blocks of threads_row
+    const bool few_rows = (long long)(s.n / 16) * nmem < 256, fewer_rows = (long long)(s.n / 4) * nmem < 256;
     const bool few = (long long)(s.n / (2 * s.L)) * nmem < 256;
     if (s.fused && one_col_ok) {
-        // (16 rows in N threads where the row blocks hold fewer threads than
-        // columns: one column per thread keeps the march's window in registers)
-        int r = tuned && ts.srow_rows ? ts.srow_rows : (few_rows ? 4 : (s.threads_row < s.n ? 16 : 0));
+        // (2-row blocks while 4-row ones would still leave CUs idle: one 256^2
+        // domain's fused pass 6.25 us against 6.92; 16 rows in N threads where
+        // the row blocks hold fewer threads than columns: one column per thread
+        // keeps the march's window in registers)
+        int r = tuned && ts.srow_rows ? ts.srow_rows
+                                      : (fewer_rows ? 2 : few_rows ? 4 : (s.threads_row < s.n ? 16 : 0));
         if (const char* e = std::getenv("FI_SROW_ROWS")) r = std::atoi(e);
-        if (r == 4 || r == 8 || r == 16) s.srow_rows = r;
+        if (r == 2 || r == 4 || r == 8 || r == 16) s.srow_rows = r;
     }
     int th = tuned ? ts.threads_inv : (few ? def_inv : 0);
     if (const char* e = std::getenv("FI_INV_THREADS")) th = std::atoi(e);
@@ -2599,7 +2602,7 @@ std::vector<TunedShape> tune_candidates(int n, int nmem) {
     // time: shorter fused row blocks (one column per thread) and an inverse
     // row pass with its own block size
     if ((p2 || r3) && nmem <= 8)
-        for (int rows : {4, 8})
+        for (int rows : {2, 4, 8})
             for (int inv : {0, 64, 128, 192})
                 for (int C : {1, 2, 4, 8}) {
                     if (r3 ? (inv != 0 && inv != 192) : inv == 192) continue;   // 3 * 2^k: 384 or 192 threads

@@ -430,7 +430,7 @@ __device__ __forceinline__ void srow_block(const Member* __restrict__ mem, const
 template <typename T, int N, int THREADS, int LF>
 __global__ void __launch_bounds__(THREADS) kf_srow_fwd(const Member* __restrict__ mem, const void* __restrict__ twv) {
     using SR = SRow<T, N, THREADS, LF>;
-    static_assert(SR::CPT >= 1 && SR::CPT * THREADS == N && SR::ROWS >= 4, "fused stencil shape");
+    static_assert(SR::CPT >= 1 && SR::CPT * THREADS == N && SR::ROWS >= 2, "fused stencil shape");
     int mi, rblk;
     xcd_order(mi, rblk);
     if (rblk & 1) srow_block<T, N, THREADS, LF, true>(mem, twv, mi, rblk);
@@ -796,7 +796,8 @@ constexpr size_t FI_LDS_MAX = 160 * 1024;
 template <typename T, int N, int THREADS, int LF = THREADS / (N / 8)>
 static bool srow_fwd(const Member* d, int nmem, hipStream_t st, const void* tw) {
     using SR = SRow<T, N, THREADS, LF>;
-    if constexpr (SR::CPT >= 1 && SR::CPT * THREADS == N && SR::ROWS >= 4 && SR::lds <= FI_LDS_MAX) {
+    if constexpr (SR::CPT >= 1 && SR::CPT * THREADS == N && (SR::ROWS >= 4 || THREADS == N || THREADS == 384) &&
+                  SR::lds <= FI_LDS_MAX) {
         if constexpr (SR::lds > 65536) {
             static const bool ok = hipFuncSetAttribute((const void*)kf_srow_fwd<T, N, THREADS, LF>,
                                                        hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -815,6 +816,7 @@ static bool srow_fwd(const Member* d, int nmem, hipStream_t st, const void* tw) 
 template <typename T, int N, int THREADS>
 static bool srow_fwd_rows(const Member* d, int nmem, hipStream_t st, const void* tw, int rows) {
     if constexpr (THREADS / (N / 8) >= 4) {
+        if (rows == 2) return srow_fwd<T, N, THREADS, 1>(d, nmem, st, tw);
         if (rows == 4) return srow_fwd<T, N, THREADS, 2>(d, nmem, st, tw);
         if (rows == 8) return srow_fwd<T, N, THREADS, 4>(d, nmem, st, tw);
     }

@@ -31,11 +31,15 @@ def _run(nx, prec, steps=4, seed=7):
     return out
 
 
-# the built-in shapes of one domain are the small-batch ones (4-row fused
-# blocks, inverse blocks of 64 threads, 192 at N = 384); each case sets others
+# the built-in shapes of one domain are the small-batch ones (2-row fused
+# blocks, whose march's edge windows reach into the halo rows; inverse blocks
+# of 64 threads, 192 at N = 384); each case sets others
 @pytest.mark.parametrize("nx,prec,env", [
     (256, "mixed", {"FI_SROW_ROWS": "0", "FI_INV_THREADS": "256", "FI_COL_LINES": "16"}),
     (256, "mixed", {"FI_SROW_ROWS": "8", "FI_INV_THREADS": "128"}),
+    (256, "mixed", {"FI_SROW_ROWS": "4"}),
+    (128, "fp64", {"FI_SROW_ROWS": "4", "FI_INV_THREADS": "128"}),
+    (384, "mixed", {"FI_SROW_ROWS": "4"}),
     (256, "fp64", {"FI_SROW_ROWS": "0", "FI_INV_THREADS": "256", "FI_COL_LINES": "4"}),
     (128, "mixed", {"FI_SROW_ROWS": "0", "FI_INV_THREADS": "128"}),
     (384, "mixed", {"FI_SROW_ROWS": "0", "FI_INV_THREADS": "384", "FI_COL_LINES": "2"}),
@@ -70,7 +74,7 @@ def test_autotuned_shape_bitwise(monkeypatch):
         check(load().fi_autotune(128, 1, nmem, 1, choice))
         c = list(choice)
         assert c[0] in (0, 64, 128, 256) and c[1] in (0, 1, 2, 4, 8, 16) and c[3] in (0, 1, 2, 3)
-        assert c[6] in (0, 4, 8) and c[7] in (0, 64, 128, 256)
+        assert c[6] in (0, 2, 4, 8, 16) and c[7] in (0, 64, 128, 192, 256)
     tuned = _run(128, "mixed", seed=11)
     assert tuned["attempts"] == ref["attempts"]
     for f in ("w", "dwdt", "z_bed"):
