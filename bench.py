@@ -37,15 +37,16 @@ PROF_ALL = 99
 def fast_kernel_names(shape):
     """Template names (as rocprofv3 prints them) of the register-FFT attempt
     kernels the last timed attempt ran, from fi_prof_shape: {N, fp64, fused,
-    fast, threads_row, C, row_tiles, srow_rows, threads_inv, dt_method}."""
-    n, fp64, fused, fast, th, c, rt, rows, inv, meth = shape
+    fast, threads_row, C, row_tiles, srow_rows, threads_inv, dt_method,
+    column-pass exchange buffers}."""
+    n, fp64, fused, fast, th, c, rt, rows, inv, meth, nbuf = shape
     if not (fast and fused):
         return {}
     t = "double" if fp64 else "float"
     tpl = n // 8
     sth = (n if n % 3 else 384) if rows else th
     lf = rows // 2 if rows else sth // tpl
-    return {0: f"kf_srow_fwd<{t}, {n}, {sth}, {lf}>", 2: f"kf_col<{t}, {n}, {c}, true>",
+    return {0: f"kf_srow_fwd<{t}, {n}, {sth}, {lf}>", 2: f"kf_col<{t}, {n}, {c}, true, {'true' if nbuf == 2 else 'false'}>",
             3: f"kf_row_inv<{t}, {n}, {inv or th}, {rt}, {meth}>"}
 
 
@@ -108,6 +109,7 @@ def autotune(n, precision, nmem, reps=3):
     keys = ("threads_row", "col_lines", "row_tiles", "subbatches", "generic_row_lines", "generic_col_lines",
             "srow_rows", "inv_threads")
     out = {k: int(v) for k, v in zip(keys, choice)}
+    out["built_in"] = all(int(v) == 0 for v in choice)   # the built-in shapes were the fastest
     out["seconds"] = round(time.perf_counter() - t0, 2)
     return out
 
@@ -161,7 +163,7 @@ def run_batched(nmem, steps, warmup, precision, prof_kernel=None, timed=True, ex
             check(load().fi_prof_collect_kernel(k, ctypes.byref(ms), ctypes.byref(cnt)))
             prof[k] = (ms.value, cnt.value)
         prof["fused"] = bool(load().fi_prof_fused())
-        shape = (ctypes.c_int * 10)()
+        shape = (ctypes.c_int * 11)()
         check(load().fi_prof_shape(shape))
         prof["names"] = fast_kernel_names(list(shape))
         check(load().fi_prof_select(-1))

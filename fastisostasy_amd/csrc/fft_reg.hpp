@@ -4,8 +4,12 @@
 // E = 8 elements, element e of thread t being X[t + TPL * e].  Stage 0 reads
 // its inputs in exactly that order, so it is fed straight from global memory;
 // the last stage leaves its outputs in the same order in registers.  The
-// stages in between exchange data through one LDS buffer (one barrier
-// per exchange, plus one before the buffer is reused).  Radices: 8 for every stage but a possible final 4 or 2.
+// stages in between exchange data through LDS: with one buffer, a barrier
+// between the writes and the reads of an exchange and one before the buffer
+// is written again; with two buffers used in turn (ColXch2), only the first
+// -- the barrier of exchange j + 1 already orders exchange j's reads of a
+// buffer before exchange j + 2's writes to it.  Radices: 8 for every stage
+// but a possible final 4 or 2.
 // Twiddles exp(-2 pi i m / N) are read from an LDS copy of the table.
 #pragma once
 #include "fi_device.hpp"
@@ -34,10 +38,11 @@ template <int N> struct RegPlan {
     static_assert(N % E == 0, "line length");
 };
 
-// Exchange buffers (single LDS buffer, two barriers per exchange): where
-// element `pos` of the calling thread's line lives.  Rows: line-major with one
-// pad element per 8 (spreads the stride-R stage writes over the banks);
-// columns: position-major with the lines fastest and one pad per row.
+// Exchange buffers: where element `pos` of the calling thread's line lives,
+// and what ends an exchange once its reads are issued (`release`).  Rows:
+// line-major with one pad element per 8 (spreads the stride-R stage writes
+// over the banks); columns: position-major with the lines fastest and one pad
+// per row.
 template <typename T, int N> struct RowXch {
     static constexpr int PITCH = N + N / 8 + 1;
     cplx<T>* base;
@@ -46,6 +51,8 @@ template <typename T, int N> struct RowXch {
     __device__ __forceinline__ int addr(int pos) const { return line * PITCH + pos + (pos >> 3); }
     __device__ __forceinline__ void put(int pos, cplx<T> v) { base[addr(pos)] = v; }
     __device__ __forceinline__ cplx<T> get(int pos) const { return base[addr(pos)]; }
+    __device__ __forceinline__ void release() { __syncthreads(); }
+    static constexpr bool TURNS = false;
 };
 
 template <typename T, int N> struct ColXch {
@@ -55,6 +62,22 @@ template <typename T, int N> struct ColXch {
     __device__ __forceinline__ int addr(int pos) const { return pos * pitch + line; }
     __device__ __forceinline__ void put(int pos, cplx<T> v) { base[addr(pos)] = v; }
     __device__ __forceinline__ cplx<T> get(int pos) const { return base[addr(pos)]; }
+    __device__ __forceinline__ void release() { __syncthreads(); }
+    static constexpr bool TURNS = false;
+};
+
+// The column layout in two buffers of N (C + 1) values each, used in turn:
+// `release` switches buffers instead of waiting.  The switches happen in
+// straight-line code from a known start, so `cur` folds to a constant.
+template <typename T, int N> struct ColXch2 {
+    cplx<T>* base;
+    int line, pitch, cur;
+    __device__ __forceinline__ ColXch2(cplx<T>* b, int l, int C) : base(b), line(l), pitch(C + 1), cur(0) {}
+    __device__ __forceinline__ int addr(int pos) const { return cur * N * pitch + pos * pitch + line; }
+    __device__ __forceinline__ void put(int pos, cplx<T> v) { base[addr(pos)] = v; }
+    __device__ __forceinline__ cplx<T> get(int pos) const { return base[addr(pos)]; }
+    __device__ __forceinline__ void release() { cur ^= 1; }
+    static constexpr bool TURNS = true;
 };
 
 // TWS: stride of the twiddle table (1: the table of this length; 3: the
@@ -94,7 +117,7 @@ __device__ __forceinline__ void reg_stage(cplx<T>* v, int t, const cplx<T>* __re
         __syncthreads();
 #pragma unroll
         for (int e = 0; e < E; ++e) v[e] = xc.get(t + TPL * e);
-        __syncthreads();
+        xc.release();
     }
 }
 
@@ -113,8 +136,9 @@ __device__ __forceinline__ void reg_fft_p2(cplx<T>* v, int t, const cplx<T>* __r
 }
 
 // Fetch Z[(N - k) mod N] for every element k = t + TPL e of the thread
-// (one LDS exchange).
-template <typename T, int N, class X>
+// (one LDS exchange).  FINAL: the caller does not use the exchange buffer
+// again, and the exchange is not closed.
+template <typename T, int N, bool FINAL, class X>
 __device__ __forceinline__ void reg_mirror_p2(const cplx<T>* v, cplx<T>* m, int t, X& xc) {
     constexpr int E = P2Plan<N>::E, TPL = P2Plan<N>::TPL;
 #pragma unroll
@@ -122,7 +146,7 @@ __device__ __forceinline__ void reg_mirror_p2(const cplx<T>* v, cplx<T>* m, int 
     __syncthreads();
 #pragma unroll
     for (int e = 0; e < E; ++e) m[e] = xc.get((N - (t + TPL * e)) & (N - 1));
-    __syncthreads();
+    if constexpr (!FINAL) xc.release();
 }
 
 // ---------------------------------------------------------------- N = 3 * 2^k
@@ -153,9 +177,12 @@ template <class X> struct SubXch {
     __device__ __forceinline__ SubXch(const X& xx, int o) : x(xx), off(o) {}
     template <class V> __device__ __forceinline__ void put(int pos, V v) { x.put(off + pos, v); }
     __device__ __forceinline__ auto get(int pos) const { return x.get(off + pos); }
+    // a copy cannot switch the caller's buffer: single-buffered exchanges only
+    static_assert(!X::TURNS, "sub-line views of single-buffered exchanges only");
+    __device__ __forceinline__ void release() { x.release(); }
 };
 
-template <typename T, int N, class X>
+template <typename T, int N, bool FINAL, class X>
 __device__ __forceinline__ void r3_fft_mirror(cplx<T>* v, cplx<T>* m, int tt, const cplx<T>* __restrict__ tw, X& xc) {
     constexpr int M = N / 3, E = 8, TPL = N / 8, TM = M / 8;
     const int t = tt / 3, r = tt - 3 * t;
@@ -191,19 +218,21 @@ __device__ __forceinline__ void r3_fft_mirror(cplx<T>* v, cplx<T>* m, int tt, co
         v[e] = xc.get(k);
         m[e] = xc.get(k == 0 ? 0 : N - k);
     }
-    __syncthreads();
+    if constexpr (!FINAL) __syncthreads();
 }
 
 // Forward FFT of the thread's line with the Hartley mirrors, for N = 2^k or
 // 3 * 2^k: on exit v[e] = Z[k], m[e] = Z[(N - k) mod N], k = t + TPL e.
-template <typename T, int N, class X>
+// FINAL: the caller does not use the exchange buffer again (the last
+// exchange's reads need no barrier behind them).
+template <typename T, int N, bool FINAL = false, class X>
 __device__ __forceinline__ void reg_fft_mirror(cplx<T>* v, cplx<T>* m, int t, const cplx<T>* __restrict__ tw, X& xc) {
     if constexpr (IsPow2<N>::value) {
         reg_fft_p2<T, N>(v, t, tw, xc);
-        reg_mirror_p2<T, N>(v, m, t, xc);
+        reg_mirror_p2<T, N, FINAL>(v, m, t, xc);
     } else {
         static_assert(Is3Pow2<N>::value, "register FFT lengths: 2^k or 3 * 2^k");
-        r3_fft_mirror<T, N>(v, m, t, tw, xc);
+        r3_fft_mirror<T, N, FINAL>(v, m, t, tw, xc);
     }
 }
 

@@ -47,6 +47,8 @@ void launch_attempt(const Member* d_mem, int nmem, const RhsShape& s, hipStream_
 void launch_decide(const Member* d_mem, int nmem, const RhsShape& s, hipStream_t st);
 // power-of-two passes: which = 1 row forward, 2 column, 3 row inverse
 bool launch_fast_pass(const Member* d_mem, int nmem, const RhsShape& s, hipStream_t st, int which);
+// LDS exchange buffers (1 or 2) of the column pass launched for shape s
+int col_buffers(const RhsShape& s);
 // event timing of one attempt kernel: 0 stencil (fused: stencil + row DHT), 1 row DHT, 2 column DHT+divide,
 // 3 inverse row DHT+update; PROF_ALL times every one of them
 constexpr int PROF_ALL = 99;
@@ -57,7 +59,7 @@ int prof_collect_kernel(int kernel, double* total_ms, long long* launches);
 // the library's error message (fi_last_error) for code `code`; returns code
 int set_error(int code, const char* msg);
 int prof_fused();
-void prof_shape(int* out);   // 10 ints, fi_prof_shape
+void prof_shape(int* out);   // 11 ints, fi_prof_shape
 bool prof_active();
 // a start/stop event pair when launches of `kernel` are being timed
 bool prof_pair(int kernel, hipEvent_t* start, hipEvent_t* stop);

@@ -475,9 +475,9 @@ void prof_select(int kernel) {
 int prof_fused() { return g_prof.fused.load(std::memory_order_relaxed) ? 1 : 0; }
 void prof_shape(int* out) {
     const RhsShape& s = g_prof.last;
-    const int v[10] = {s.n, s.fp64 ? 1 : 0, prof_fused(), s.fast ? 1 : 0, s.threads_row, s.C,
-                       s.row_tiles, s.srow_rows, s.threads_inv, s.dt_method};
-    for (int i = 0; i < 10; ++i) out[i] = v[i];
+    const int v[11] = {s.n, s.fp64 ? 1 : 0, prof_fused(), s.fast ? 1 : 0, s.threads_row, s.C,
+                       s.row_tiles, s.srow_rows, s.threads_inv, s.dt_method, s.fast ? col_buffers(s) : 1};
+    for (int i = 0; i < 11; ++i) out[i] = v[i];
 }
 bool prof_active() { return g_prof.kernel >= 0; }
 

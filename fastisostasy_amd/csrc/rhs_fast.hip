@@ -15,6 +15,9 @@
 // (line = tid % C, t = tid / C) so that consecutive lanes read consecutive
 // column pairs.  Packing neighbours (not r and r+L) makes every line's
 // rounding independent of the block shape, hence of the batch size.
+#include <cstdlib>
+#include <type_traits>
+
 #include "fft_reg.hpp"
 #include "fi_kernels.h"
 #include "rhs_common.hpp"
@@ -52,7 +55,7 @@ __global__ void __launch_bounds__(THREADS) kf_row_fwd(const Member* __restrict__
         v[e] = mk<T>(src[(size_t)ra * N + t + TPL * e], src[(size_t)rb * N + t + TPL * e]);
     __syncthreads();
     RowXch<T, N> xc(buf, line);
-    reg_fft_mirror<T, N>(v, m, t, tw_l, xc);
+    reg_fft_mirror<T, N, true>(v, m, t, tw_l, xc);
     T* __restrict__ dst = (T*)M.g2;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -269,7 +272,7 @@ __device__ __forceinline__ void srow_body(const Member& M, const kptr<PhysDev> p
     for (int e = 0; e < E; ++e) v[e] = mk<T>(sf[2 * ld * N + t + TPL * e], sf[(2 * ld + 1) * N + t + TPL * e]);
     __syncthreads();
     RowXch<T, N> xc((cplx<T>*)(tw_l + N), line);
-    reg_fft_mirror<T, N>(v, mr, t, tw_l, xc);
+    reg_fft_mirror<T, N, true>(v, mr, t, tw_l, xc);
     if (L < SR::LX && line >= L) return;
     gmut<T> dst = gout<T>(M.g2);
     const int ra = R0 + 2 * line, rb = ra + 1;
@@ -447,20 +450,16 @@ __global__ void __launch_bounds__(THREADS) kf_srow_fwd(const Member* __restrict_
 // FP32: 13.9 us per launch on the 64-member ensemble against 15.0 in the
 // plain order).  Blocks of whole lines keep the plain order, which measured
 // about 2% faster for them.
-template <typename T, int N, int C, bool COMMIT>
+template <typename T, int N, int C, bool COMMIT, bool DB>
 __global__ void __launch_bounds__(C * (N / 8)) kf_col(const Member* __restrict__ mem, const void* __restrict__ twv,
                                                      const double2* __restrict__ tw64) {
     int mi = blockIdx.y, cblk = blockIdx.x;
     if constexpr (2 * C * sizeof(T) < 128) xcd_order(mi, cblk);
     const Member& M = mem[mi];
-    const kptr<Ctl> c = kview(COMMIT ? M.ctl_next : M.ctl);
-    // block 0 commits after its last load: a one-lane store ahead of the loads
-    // would make every later wait a full one
-    const bool commit = COMMIT && cblk == 0 && threadIdx.x == 0;
-    if ((c->status != ST_RUN) | (c->k_valid != 0) | (c->skip != 0)) {
-        if (commit) copy_ctl(M.ctl, c);
-        return;
-    }
+    // the member's pointers in one batch of scalar loads
+    const void* const p_src = M.g2;
+    const void* const p_kr = M.kR;
+    const Ctl* const p_ctl = COMMIT ? M.ctl_next : M.ctl;
     using PL = RegPlan<N>;
     constexpr int TPL = PL::TPL, E = PL::E;
     extern __shared__ __align__(16) unsigned char smem[];
@@ -470,7 +469,9 @@ __global__ void __launch_bounds__(C * (N / 8)) kf_col(const Member* __restrict__
     // Loads in the order of use, each group behind a scheduling barrier so
     // that every wait covers only its own group: twiddles (stored to LDS
     // once the field loads are in flight), the spectrum (first transform),
-    // then kappa*R (needed only after the first transform).
+    // then kappa*R (needed only after the first transform).  They are issued
+    // before the control block is tested (the buffers exist whatever the
+    // domain's state), so the test's load latency is not in front of them.
     constexpr int THREADS = C * (N / 8), TWQ = (2 * N + THREADS - 1) / THREADS;
     T twr[TWQ];
     {
@@ -485,8 +486,8 @@ __global__ void __launch_bounds__(C * (N / 8)) kf_col(const Member* __restrict__
     const int c0 = cblk * 2 * C;
     const int ca = c0 + 2 * line, cb = ca + 1;   // adjacent columns share a complex line
     using V2 = typename vec2<T>::type;
-    gptr<V2> src = gin<V2>(M.g2);
-    gptr<V2> kR = gin<V2>(M.kR);
+    gptr<V2> src = gin<V2>(p_src);
+    gptr<V2> kR = gin<V2>(p_kr);
     cplx<T> v[E], m[E];
     T ka[E], kb[E];
     V2 xs[E], xk[E];
@@ -498,12 +499,24 @@ __global__ void __launch_bounds__(C * (N / 8)) kf_col(const Member* __restrict__
     // cas weights of the corner terms (used by the threads holding rows 0 and N-1)
     const double2 wa = tw64[ca], wb = tw64[cb];
     __builtin_amdgcn_sched_barrier(0);
-    // (no branch around the store when the threads cover the table evenly: a
-    // join here would make the compiler wait for every load in flight)
+    const kptr<Ctl> c = kview(p_ctl);
+    const bool idle = (c->status != ST_RUN) | (c->k_valid != 0) | (c->skip != 0);
+    // The twiddles go to LDS ahead of the test, which keeps their loads ahead
+    // of it too (after the test they would be issued behind the field loads,
+    // and waiting for them would wait for kappa*R as well).  (No branch around
+    // the store when the threads cover the table evenly: a join here would
+    // make the compiler wait for every load in flight.)
 #pragma unroll
     for (int k = 0; k < TWQ; ++k) {
         if constexpr (THREADS * TWQ == 2 * N) ((T*)tw_l)[tid + THREADS * k] = twr[k];
         else if (tid + THREADS * k < 2 * N) ((T*)tw_l)[tid + THREADS * k] = twr[k];
+    }
+    // block 0 commits after its last load: a one-lane store ahead of the loads
+    // would make every later wait a full one
+    const bool commit = COMMIT && cblk == 0 && threadIdx.x == 0;
+    if (idle) {
+        if (commit) copy_ctl(M.ctl, c);
+        return;
     }
     __syncthreads();
 #pragma unroll
@@ -512,7 +525,7 @@ __global__ void __launch_bounds__(C * (N / 8)) kf_col(const Member* __restrict__
         ka[e] = xk[e].x;
         kb[e] = xk[e].y;
     }
-    ColXch<T, N> xc(buf, line, C);
+    std::conditional_t<DB, ColXch2<T, N>, ColXch<T, N>> xc(buf, line, C);
     reg_fft_mirror<T, N>(v, m, t, tw_l, xc);
     // Hartley values and the spectral divide (lv_elva.f90:115)
 #pragma unroll
@@ -521,7 +534,7 @@ __global__ void __launch_bounds__(C * (N / 8)) kf_col(const Member* __restrict__
         dht_pair(v[e], m[e], ha, hb);
         v[e] = mk<T>(kdiv(ha, ka[e]), kdiv(hb, kb[e]));
     }
-    reg_fft_mirror<T, N>(v, m, t, tw_l, xc);
+    reg_fft_mirror<T, N, true>(v, m, t, tw_l, xc);
     gmut<V2> dst = gout<V2>(M.g1);
     gmut<double> cor = gout<double>(M.corners);
 #pragma unroll
@@ -604,9 +617,10 @@ __global__ void __launch_bounds__(THREADS) kf_row_inv(const Member* __restrict__
     int mi, rblk;
     xcd_order(mi, rblk);
     const Member& M = mem[mi];
-    const kptr<Ctl> c = kview(M.ctl);
-    if ((c->status != ST_RUN) | (c->skip != 0)) return;
-    const kptr<PhysDev> ph = kview(M.phys);
+    // the member's pointers in one batch of scalar loads
+    const Ctl* const p_ctl = M.ctl;
+    const void* const p_src = M.g1;
+    const double* const p_cor = M.corners;
     using PL = RegPlan<N>;
     constexpr int TPL = PL::TPL, E = PL::E, L = THREADS / TPL;
     extern __shared__ __align__(16) unsigned char smem[];
@@ -616,6 +630,66 @@ __global__ void __launch_bounds__(THREADS) kf_row_inv(const Member* __restrict__
     int ra[TILES];
 #pragma unroll
     for (int k = 0; k < TILES; ++k) ra[k] = (rblk * TILES + k) * 2 * L + 2 * line;   // rb = ra + 1
+    auto idx = [&](int r, int e) { return (size_t)r * N + t + TPL * e; };
+    // The transform's loads, in the order of use, each group behind a
+    // scheduling barrier so that every wait covers only the groups issued
+    // before it: corner terms, twiddles, spectrum.  They are issued before the
+    // control block is read (the buffers exist whatever the domain's state;
+    // after a rejected attempt they go unused), so its load latency is not in
+    // front of them; X (consumed last) follows once the live buffer is known.
+    // The four corner sums over the N per-column terms of the column pass:
+    // quantity q is summed lane-strided, then by a fixed butterfly, by wave
+    // q (wave w of NW < 4 takes q = w, w + NW, ... below 4; with more, waves
+    // 4.. repeat waves 0..3), so the order is set by N alone.
+    constexpr int NW = THREADS / 64, QPW = NW >= 4 ? 1 : (4 + NW - 1) / NW, CPL = N / 64;
+    static_assert(NW >= 1 && THREADS % 64 == 0 && N % 64 == 0, "corner-sum layout");
+    const int wave = tid >> 6, lane = tid & 63;
+    double cq[QPW][CPL];
+    {
+        gptr<double> cp = gin<double>(p_cor);
+#pragma unroll
+        for (int j = 0; j < QPW; ++j) {
+            const int qi = NW >= 4 ? (wave & 3) : wave + NW * j;
+            const int qc = qi < 4 ? qi : 0;   // (a wave past the four quantities loads one it discards)
+#pragma unroll
+            for (int i = 0; i < CPL; ++i) cq[j][i] = cp[(size_t)(lane + 64 * i) * 4 + qc];
+        }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    constexpr int TWQ = (2 * N + THREADS - 1) / THREADS;
+    T twr[TWQ];
+    {
+        gptr<T> g = gin<T>(twv);
+#pragma unroll
+        for (int k = 0; k < TWQ; ++k) {
+            if constexpr (THREADS * TWQ == 2 * N) twr[k] = g[tid + THREADS * k];
+            else if (tid + THREADS * k < 2 * N) twr[k] = g[tid + THREADS * k];
+        }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    cplx<T> v[TILES][E];
+    {
+        gptr<T> src = gin<T>(p_src);
+#pragma unroll
+        for (int k = 0; k < TILES; ++k)
+#pragma unroll
+            for (int e = 0; e < E; ++e) v[k][e] = mk<T>(src[idx(ra[k], e)], src[idx(ra[k] + 1, e)]);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    const kptr<Ctl> c = kview(p_ctl);
+    const bool idle = (c->status != ST_RUN) | (c->skip != 0);
+    // The twiddles go to LDS ahead of the test, which keeps their loads ahead
+    // of it too (after it they would be issued behind the spectrum and X, and
+    // waiting for them would wait for those as well).  (No branch around the
+    // store when the threads cover the table evenly: a join here would make
+    // the compiler wait for every load in flight.)
+#pragma unroll
+    for (int k = 0; k < TWQ; ++k) {
+        if constexpr (THREADS * TWQ == 2 * N) ((T*)tw_l)[tid + THREADS * k] = twr[k];
+        else if (tid + THREADS * k < 2 * N) ((T*)tw_l)[tid + THREADS * k] = twr[k];
+    }
+    if (idle) return;
+    const kptr<PhysDev> ph = kview(M.phys);
     const int meth = ph->dt_method;
     const double dt = attempt_dt(meth, c->dt, c->t, c->tf, ph->dt_max);
     const int cur = c->cur;
@@ -639,9 +713,6 @@ __global__ void __launch_bounds__(THREADS) kf_row_inv(const Member* __restrict__
         const double d = x2 - x1;
         s2 += d * d;
     };
-    auto idx = [&](int r, int e) { return (size_t)r * N + t + TPL * e; };
-    // All loads are issued before the first wait, in the order of use; X is
-    // independent of the transform and is consumed last.
     double xa[TILES][E], xb[TILES][E];
     if (k_valid) {
         // rejected attempt before: the RHS is unchanged, only dt differs
@@ -680,46 +751,7 @@ __global__ void __launch_bounds__(THREADS) kf_row_inv(const Member* __restrict__
             if constexpr (IsPow2<N>::value) return x * rS;
             else return x / S;
         };
-        gptr<T> src = gin<T>(M.g1);
-        // Loads in the order of use, each group behind a scheduling barrier so
-        // that every wait covers only the groups issued before it: corner
-        // terms, twiddles, spectrum, X (consumed last).
-        // The four corner sums over the N per-column terms of the column pass:
-        // quantity q is summed lane-strided, then by a fixed butterfly, by wave
-        // q (wave w of NW < 4 takes q = w, w + NW, ... below 4; with more, waves
-        // 4.. repeat waves 0..3), so the order is set by N alone.
-        constexpr int NW = THREADS / 64, QPW = NW >= 4 ? 1 : (4 + NW - 1) / NW, CPL = N / 64;
-        static_assert(NW >= 1 && THREADS % 64 == 0 && N % 64 == 0, "corner-sum layout");
-        const int wave = tid >> 6, lane = tid & 63;
-        double cq[QPW][CPL];
-        {
-            gptr<double> cp = gin<double>(M.corners);
-#pragma unroll
-            for (int j = 0; j < QPW; ++j) {
-                const int qi = NW >= 4 ? (wave & 3) : wave + NW * j;
-                const int qc = qi < 4 ? qi : 0;   // (a wave past the four quantities loads one it discards)
-#pragma unroll
-                for (int i = 0; i < CPL; ++i) cq[j][i] = cp[(size_t)(lane + 64 * i) * 4 + qc];
-            }
-        }
-        __builtin_amdgcn_sched_barrier(0);
-        constexpr int TWQ = (2 * N + THREADS - 1) / THREADS;
-        T twr[TWQ];
-        {
-            gptr<T> g = gin<T>(twv);
-#pragma unroll
-            for (int k = 0; k < TWQ; ++k) {
-                if constexpr (THREADS * TWQ == 2 * N) twr[k] = g[tid + THREADS * k];
-                else if (tid + THREADS * k < 2 * N) twr[k] = g[tid + THREADS * k];
-            }
-        }
-        __builtin_amdgcn_sched_barrier(0);
-        cplx<T> v[TILES][E], m[E];
-#pragma unroll
-        for (int k = 0; k < TILES; ++k)
-#pragma unroll
-            for (int e = 0; e < E; ++e) v[k][e] = mk<T>(src[idx(ra[k], e)], src[idx(ra[k] + 1, e)]);
-        __builtin_amdgcn_sched_barrier(0);
+        cplx<T> m[E];
 #pragma unroll
         for (int k = 0; k < TILES; ++k)
 #pragma unroll
@@ -737,13 +769,6 @@ __global__ void __launch_bounds__(THREADS) kf_row_inv(const Member* __restrict__
             const int qi = NW >= 4 ? (wave & 3) : wave + NW * j;
             if (lane == 0 && qi < 4) qs[qi] = q;
         }
-        // (no branch around the store when the threads cover the table evenly:
-        // a join here would make the compiler wait for every load in flight)
-#pragma unroll
-        for (int k = 0; k < TWQ; ++k) {
-            if constexpr (THREADS * TWQ == 2 * N) ((T*)tw_l)[tid + THREADS * k] = twr[k];
-            else if (tid + THREADS * k < 2 * N) ((T*)tw_l)[tid + THREADS * k] = twr[k];
-        }
         __syncthreads();
         // x(1,1) + x(nx,ny) + x(1,ny) + x(nx,1)  (isos_utils.f90:116)
         const double cc = 0.25 * (scl(qs[0]) + scl(qs[3]) + scl(qs[2]) + scl(qs[1]));
@@ -753,7 +778,7 @@ __global__ void __launch_bounds__(THREADS) kf_row_inv(const Member* __restrict__
 #pragma unroll
         for (int k = 0; k < TILES; ++k) {
             if (k > 0) __syncthreads();   // the exchange buffer is reused
-            reg_fft_mirror<T, N>(v[k], m, t, tw_l, xc);
+            reg_fft_mirror<T, N, true>(v[k], m, t, tw_l, xc);
             double a1a = 0.0, a2a = 0.0, s2a = 0.0, a1b = 0.0, a2b = 0.0, s2b = 0.0;
 #pragma unroll
             for (int e = 0; e < E; ++e) {
@@ -781,7 +806,9 @@ static size_t row_lds(int threads) {
     return ((size_t)N + (size_t)L * RowXch<T, N>::PITCH) * sizeof(cplx<T>);
 }
 template <typename T, int N>
-constexpr size_t col_lds(int C) { return ((size_t)N + (size_t)N * (C + 1)) * sizeof(cplx<T>); }
+constexpr size_t col_lds(int C, bool db = false) {
+    return ((size_t)N + (size_t)N * (C + 1) * (db ? 2 : 1)) * sizeof(cplx<T>);
+}
 
 template <typename T, int N, int THREADS>
 static void rows_fwd(const Member* d, int nmem, hipStream_t st, const void* tw) {
@@ -842,13 +869,40 @@ static void rows_inv(const Member* d, int nmem, hipStream_t st, const void* tw, 
     if (dt_method == 3) rows_inv_m<T, N, THREADS, 3>(d, nmem, st, tw, ncolblk, tiles);
     else rows_inv_m<T, N, THREADS, -1>(d, nmem, st, tw, ncolblk, tiles);
 }
+// Two exchange buffers in the column pass (one barrier per exchange instead
+// of two) where the blocks of a full CU (16 waves) still fit its LDS: FP32
+// lines of 2^k, 8 line pairs or more at N = 256 (64 x 256^2, 8 line pairs:
+// 12.2 us per launch against 12.5-12.9 with one buffer, same build and box).
+template <typename T, int N, int C>
+constexpr bool col_two_buffers() {
+    constexpr int blocks_per_cu = 8192 / (C * N) > 1 ? 8192 / (C * N) : 1;
+    return IsPow2<N>::value && sizeof(T) == 4 && col_lds<T, N>(C, true) * blocks_per_cu <= FI_LDS_MAX;
+}
+template <typename T, int N, int C, bool DB>
+static void cols_launch(const Member* d, int nmem, hipStream_t st, const void* tw, const double2* tw64, bool commit) {
+    const size_t lds = col_lds<T, N>(C, DB);
+    if (commit)
+        launch_k(2, kf_col<T, N, C, true, DB>, dim3(N / (2 * C), nmem), dim3(C * (N / 8)), lds, st, d, tw, tw64);
+    else
+        launch_k(2, kf_col<T, N, C, false, DB>, dim3(N / (2 * C), nmem), dim3(C * (N / 8)), lds, st, d, tw, tw64);
+}
 template <typename T, int N, int C>
 static void cols(const Member* d, int nmem, hipStream_t st, const void* tw, const double2* tw64, bool commit) {
-    const size_t lds = col_lds<T, N>(C);
-    if (commit)
-        launch_k(2, kf_col<T, N, C, true>, dim3(N / (2 * C), nmem), dim3(C * (N / 8)), lds, st, d, tw, tw64);
-    else
-        launch_k(2, kf_col<T, N, C, false>, dim3(N / (2 * C), nmem), dim3(C * (N / 8)), lds, st, d, tw, tw64);
+    if constexpr (col_two_buffers<T, N, C>()) {
+        constexpr int lds = (int)col_lds<T, N>(C, true);
+        // launches above 64 KiB of dynamic LDS declare it to the runtime first
+        static const bool ok =
+            lds <= 65536 ||
+            (hipFuncSetAttribute((const void*)kf_col<T, N, C, true, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 lds) == hipSuccess &&
+             hipFuncSetAttribute((const void*)kf_col<T, N, C, false, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 lds) == hipSuccess);
+        if (ok) {
+            cols_launch<T, N, C, true>(d, nmem, st, tw, tw64, commit);
+            return;
+        }
+    }
+    cols_launch<T, N, C, false>(d, nmem, st, tw, tw64, commit);
 }
 
 template <typename T, int N>
@@ -921,6 +975,30 @@ static bool fast_t(const Member* d, int nmem, const RhsShape& s, hipStream_t st,
     }
     return false;
 }
+
+template <typename T, int N>
+static int col_buffers_n(int C) {
+    switch (C) {
+    case 1: return col_two_buffers<T, N, 1>() ? 2 : 1;
+    case 2: return col_two_buffers<T, N, 2>() ? 2 : 1;
+    case 4: return col_two_buffers<T, N, 4>() ? 2 : 1;
+    case 8: return col_two_buffers<T, N, 8>() ? 2 : 1;
+    case 16: return col_two_buffers<T, N, 16>() ? 2 : 1;
+    }
+    return 1;
+}
+template <typename T>
+static int col_buffers_t(int n, int C) {
+    switch (n) {
+    case 64: return col_buffers_n<T, 64>(C);
+    case 128: return col_buffers_n<T, 128>(C);
+    case 256: return col_buffers_n<T, 256>(C);
+    case 512: return col_buffers_n<T, 512>(C);
+    case 1024: return col_buffers_n<T, 1024>(C);
+    }
+    return 1;
+}
+int col_buffers(const RhsShape& s) { return s.fp64 ? col_buffers_t<double>(s.n, s.C) : col_buffers_t<float>(s.n, s.C); }
 
 bool launch_fast_pass(const Member* d_mem, int nmem, const RhsShape& s, hipStream_t st, int which) {
     return s.fp64 ? fast_t<double>(d_mem, nmem, s, st, which) : fast_t<float>(d_mem, nmem, s, st, which);

@@ -305,10 +305,11 @@ int fi_prof_collect(double* total_ms, long long* launches);
 /* 1 when the last attempt ran the stencil fused into the forward row pass
  * (uniform spacing, power-of-two N); kernel 0 then times the fused kernel. */
 int fi_prof_fused(void);
-/* Launch shape of the last attempt (10 ints): {N, fp64, fused, register-FFT
+/* Launch shape of the last attempt (11 ints): {N, fp64, fused, register-FFT
  * kernels, threads per row block, line pairs per column block, row tiles of
  * the inverse pass, rows per fused block (0: 2 x threads / (N/8)), threads of
- * the inverse pass (0: threads per row block), integrator (-1: run time)}. */
+ * the inverse pass (0: threads per row block), integrator (-1: run time),
+ * LDS exchange buffers of the column pass (1 or 2)}. */
 int fi_prof_shape(int* out);
 /* kernel = 99 in fi_prof_select times every attempt kernel; this sums the
  * timed launches of one of them (0, 2, 3; -1: all timed launches) */
