@@ -41,6 +41,16 @@ template <typename T> using gmut = __attribute__((address_space(1))) T*;
 template <typename T> __device__ __forceinline__ gptr<T> gin(const void* p) { return (gptr<T>)p; }
 template <typename T> __device__ __forceinline__ gmut<T> gout(void* p) { return (gmut<T>)p; }
 
+// Requires the given block-uniform values (pointers read from a Member) to be
+// in scalar registers at this point: their scalar loads are then issued as
+// one batch with one wait, ahead of the vector loads that use them, instead
+// of each next to its first use behind the previous one's wait.
+template <typename P> __device__ __forceinline__ void sgpr_ready(P p) { asm volatile("" : : "s"(p)); }
+template <typename... P> __device__ __forceinline__ void scalars_ready(P... p) {
+    const int order[] = {(sgpr_ready(p), 0)...};
+    (void)order;
+}
+
 // ------------------------------------------------------------------ epilogue
 struct EpiOut { double x1max, x2max, d2; };
 

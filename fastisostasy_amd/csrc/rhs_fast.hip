@@ -297,34 +297,24 @@ __device__ __forceinline__ void srow_block(const Member* __restrict__ mem, const
     using SR = SRow<T, N, THREADS, LF>;
     constexpr int ROWS = SR::ROWS, CPT = SR::CPT;
     const Member& M = mem[mi];
-    const kptr<Ctl> c = kview(M.ctl);
+    // the member's pointers in one batch of scalar loads
+    const Ctl* const p_ctl = M.ctl;
+    const double *x0 = M.X[0], *x1 = M.X[1], *wW = M.W;
+    const void *p_D = M.D, *p_C = M.cfull, *p_E = M.eta, *p_part = M.partials;
     const kptr<PhysDev> ph = kview(M.phys);
     const int tid = threadIdx.x;
-    // Apply the previous attempt's pending accept/reject here instead of in a
-    // separate kernel: every wave recomputes it from the same partials (same
-    // order, same result); block 0 writes the resulting control block to
-    // ctl_next, which the column pass commits.  The field rows are loaded
-    // before the decision is known, from the buffer an accepted step leaves
-    // current; a rejected step without a stored RHS reloads the old state.
-    StepState st = load_state(c);
-    const bool pending = c->pending != 0;
-    if (!pending && ((st.status != ST_RUN) | (st.k_valid != 0))) {
-        if (rblk == 0 && tid == 0) {
-            copy_ctl(M.ctl_next, c);
-            M.ctl_next->skip = 0;
-        }
-        return;
-    }
-    const double *x0 = M.X[0], *x1 = M.X[1], *wW = M.W;
-    const double* wpred = pending ? (st.cur ? x0 : x1) : (st.w_is_x ? (st.cur ? x1 : x0) : wW);
     extern __shared__ __align__(16) unsigned char smem[];
     cplx<T>* tw_l = (cplx<T>*)smem;
     const int R0 = rblk * ROWS;
     const bool top = R0 == 0, bottom = R0 + ROWS == N;
-    gptr<T> Dg = gin<T>(M.D), Cg = gin<T>(M.cfull), Eg = gin<T>(M.eta);
+    gptr<T> Dg = gin<T>(p_D), Cg = gin<T>(p_C), Eg = gin<T>(p_E);
     auto crow = [](int r) { return r < 0 ? 0 : (r > N - 1 ? N - 1 : r); };
-    // twiddles and the previous attempt's partials first: the LDS store and
-    // the decision then wait only for their own loads
+    // Loads that do not depend on the domain's control block go first, so the
+    // two dependent scalar loads of its test are not in front of them: the
+    // twiddles and the previous attempt's partials (the LDS store and the
+    // decision then wait only for their own loads), then the rheology rows
+    // of the march prologue.  The buffers exist whatever the domain's state;
+    // an idle domain's loads are dropped.
     constexpr int TWQ = (2 * N + THREADS - 1) / THREADS;
     T twr[TWQ];
     {
@@ -337,12 +327,13 @@ __device__ __forceinline__ void srow_block(const Member* __restrict__ mem, const
             else if (tid + THREADS * k < 2 * N) twr[k] = g[tid + THREADS * k];
         }
     }
+    scalars_ready(p_ctl, x0, x1, wW, p_D, p_C, p_E, p_part);
     // (loaded whether or not a decision is pending: no branch for the
     // compiler to reorder the decision around)
     constexpr int NP = N / 2, PJ = (NP + 63) / 64;   // one partial per line
     double pa[PJ][2];
     {
-        gptr<double> part = gin<double>(M.partials);
+        gptr<double> part = gin<double>(p_part);
         const int lane = tid & 63;
 #pragma unroll
         for (int j = 0; j < PJ; ++j) {
@@ -359,26 +350,73 @@ __device__ __forceinline__ void srow_block(const Member* __restrict__ mem, const
     // tau + 2 + SR_LA, D row tau + SR_LA and canom_full / eta row
     // tau - 2 + SR_LA, each SR_LA steps before its first use.  Row bases are
     // uniform (scalar); the lane offset is one 32-bit value for every row.
-    gptr<double> w = gin<double>(wpred);
+    gptr<double> w = nullptr;   // the state buffer, known once the control block is read
     // global row of march row r (-2 .. ROWS+1)
     auto grow = [&](int r) { return UP ? R0 + ROWS - 1 - r : R0 + r; };
-    auto issue = [&](auto tc) {
+    auto issue_fields = [&](auto tc) {
         constexpr int tau = decltype(tc)::value;
-        constexpr int rw_ = tau + 2 + SR_LA, rd = tau + SR_LA, rc = tau - 2 + SR_LA;
+        constexpr int rd = tau + SR_LA, rc = tau - 2 + SR_LA;
 #pragma unroll
         for (int q = 0; q < CPT; ++q) {
             const unsigned col = tid + THREADS * q;
-            if constexpr (rw_ >= -2 && rw_ <= ROWS + 1) wv[rw_ + 2][q] = (w + (size_t)crow(grow(rw_)) * N)[col];
             if constexpr (rd >= -1 && rd <= ROWS) dv[rd + 1][q] = (Dg + (size_t)crow(grow(rd)) * N)[col];
             if constexpr (rc >= 0 && rc < ROWS) {
                 cv[rc][q] = (Cg + (size_t)grow(rc) * N)[col];
                 ev[rc][q] = (Eg + (size_t)grow(rc) * N)[col];
             }
         }
+    };
+    auto issue_w = [&](auto tc) {
+        constexpr int tau = decltype(tc)::value;
+        constexpr int rw_ = tau + 2 + SR_LA;
+#pragma unroll
+        for (int q = 0; q < CPT; ++q) {
+            const unsigned col = tid + THREADS * q;
+            if constexpr (rw_ >= -2 && rw_ <= ROWS + 1) wv[rw_ + 2][q] = (w + (size_t)crow(grow(rw_)) * N)[col];
+        }
+    };
+    auto issue = [&](auto tc) {
+        issue_w(tc);
+        issue_fields(tc);
         // keep the issue order: each step waits only on loads SR_LA steps old
         __builtin_amdgcn_sched_barrier(0);
     };
-    static_for<-4 - SR_LA, -1>(issue);   // everything due before the first march step
+    // the rheology rows due before the first march step
+    static_for<-4 - SR_LA, -1>([&](auto tc) {
+        issue_fields(tc);
+        __builtin_amdgcn_sched_barrier(0);
+    });
+    // Apply the previous attempt's pending accept/reject here instead of in a
+    // separate kernel: every wave recomputes it from the same partials (same
+    // order, same result); block 0 writes the resulting control block to
+    // ctl_next, which the column pass commits.  The state rows are loaded
+    // before the decision is known, from the buffer an accepted step leaves
+    // current; a rejected step without a stored RHS reloads the old state.
+    const kptr<Ctl> c = kview(p_ctl);
+    StepState st = load_state(c);
+    const bool pending = c->pending != 0;
+    // The twiddles go to LDS ahead of the tests, which keeps their loads ahead
+    // of them too (after a test they would be issued behind the state rows,
+    // and waiting for them would wait for those as well).
+#pragma unroll
+    for (int k = 0; k < TWQ; ++k) {
+        if constexpr (THREADS * TWQ == 2 * N) ((T*)tw_l)[tid + THREADS * k] = twr[k];
+        else if (tid + THREADS * k < 2 * N) ((T*)tw_l)[tid + THREADS * k] = twr[k];
+    }
+    if (!pending && ((st.status != ST_RUN) | (st.k_valid != 0))) {
+        if (rblk == 0 && tid == 0) {
+            copy_ctl(M.ctl_next, c);
+            M.ctl_next->skip = 0;
+        }
+        return;
+    }
+    const double* wpred = pending ? (st.cur ? x0 : x1) : (st.w_is_x ? (st.cur ? x1 : x0) : wW);
+    w = gin<double>(wpred);
+    // the state rows due before the first march step
+    static_for<-4 - SR_LA, -1>([&](auto tc) {
+        issue_w(tc);
+        __builtin_amdgcn_sched_barrier(0);
+    });
     {
         // the order of reduce_partials: lane-strided, then butterflies
         double amax = 0.0, s2 = 0.0;
@@ -417,11 +455,6 @@ __device__ __forceinline__ void srow_block(const Member* __restrict__ mem, const
     if ((st.status != ST_RUN) | (st.k_valid != 0) | skip) {
         write_next();
         return;
-    }
-#pragma unroll
-    for (int k = 0; k < TWQ; ++k) {
-        if constexpr (THREADS * TWQ == 2 * N) ((T*)tw_l)[tid + THREADS * k] = twr[k];
-        else if (tid + THREADS * k < 2 * N) ((T*)tw_l)[tid + THREADS * k] = twr[k];
     }
     // the march meets the domain edge it starts from first: top for a downward
     // block, bottom for an upward one
@@ -621,6 +654,7 @@ __global__ void __launch_bounds__(THREADS) kf_row_inv(const Member* __restrict__
     const Ctl* const p_ctl = M.ctl;
     const void* const p_src = M.g1;
     const double* const p_cor = M.corners;
+    scalars_ready(p_ctl, p_src, p_cor);
     using PL = RegPlan<N>;
     constexpr int TPL = PL::TPL, E = PL::E, L = THREADS / TPL;
     extern __shared__ __align__(16) unsigned char smem[];
