@@ -519,16 +519,28 @@ __global__ void __launch_bounds__(C * (N / 8)) kf_col(const Member* __restrict__
     const int c0 = cblk * 2 * C;
     const int ca = c0 + 2 * line, cb = ca + 1;   // adjacent columns share a complex line
     using V2 = typename vec2<T>::type;
+    // element e of the thread (row t + TPL e, columns ca and cb) lies lo bytes
+    // past the uniform row base e * ESTEP: one 32-bit lane offset for all 8
+    // loads of a field and scalar bases (global_load v, v_off, s_base), not 8
+    // 64-bit lane addresses
+    const unsigned lo = (unsigned)(t * N + ca) * (unsigned)sizeof(T);
+    constexpr size_t ESTEP = (size_t)TPL * N / 2;
+    // (the bases pass through an empty asm that holds them in scalar registers:
+    // otherwise the compiler folds e * ESTEP into 64-bit lane addresses)
+    auto at = [lo](auto base) {
+        asm volatile("" : "+s"(base));
+        return (decltype(base))((gptr<char>)base + lo);
+    };
     gptr<V2> src = gin<V2>(p_src);
     gptr<V2> kR = gin<V2>(p_kr);
     cplx<T> v[E], m[E];
     T ka[E], kb[E];
     V2 xs[E], xk[E];
 #pragma unroll
-    for (int e = 0; e < E; ++e) xs[e] = src[((size_t)(t + TPL * e) * N + ca) >> 1];
+    for (int e = 0; e < E; ++e) xs[e] = *at(src + e * ESTEP);
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int e = 0; e < E; ++e) xk[e] = kR[((size_t)(t + TPL * e) * N + ca) >> 1];
+    for (int e = 0; e < E; ++e) xk[e] = *at(kR + e * ESTEP);
     // cas weights of the corner terms (used by the threads holding rows 0 and N-1)
     const double2 wa = tw64[ca], wb = tw64[cb];
     __builtin_amdgcn_sched_barrier(0);
@@ -578,7 +590,9 @@ __global__ void __launch_bounds__(C * (N / 8)) kf_col(const Member* __restrict__
         V2 o;
         o.x = ha;
         o.y = hb;
-        dst[((size_t)k * N + ca) >> 1] = o;
+        gmut<V2> row = dst + e * ESTEP;
+        asm volatile("" : "+s"(row));
+        *(gmut<V2>)((gmut<char>)row + lo) = o;
         // per-column terms of the four corners of the inverse transform (rows 0
         // and N-1, weights 1 and cas(2 pi col (N-1) / N)); the row pass sums them
         // in an order fixed by N alone
