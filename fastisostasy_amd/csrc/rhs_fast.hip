@@ -934,18 +934,26 @@ static void cols_launch(const Member* d, int nmem, hipStream_t st, const void* t
     else
         launch_k(2, kf_col<T, N, C, false, DB>, dim3(N / (2 * C), nmem), dim3(C * (N / 8)), lds, st, d, tw, tw64);
 }
+// whether the column pass of this shape runs with two buffers (launches above
+// 64 KiB of dynamic LDS declare it to the runtime first; if that fails, one)
 template <typename T, int N, int C>
-static void cols(const Member* d, int nmem, hipStream_t st, const void* tw, const double2* tw64, bool commit) {
+static bool col_two_buffers_ready() {
     if constexpr (col_two_buffers<T, N, C>()) {
         constexpr int lds = (int)col_lds<T, N>(C, true);
-        // launches above 64 KiB of dynamic LDS declare it to the runtime first
         static const bool ok =
             lds <= 65536 ||
             (hipFuncSetAttribute((const void*)kf_col<T, N, C, true, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                  lds) == hipSuccess &&
              hipFuncSetAttribute((const void*)kf_col<T, N, C, false, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                  lds) == hipSuccess);
-        if (ok) {
+        return ok;
+    }
+    return false;
+}
+template <typename T, int N, int C>
+static void cols(const Member* d, int nmem, hipStream_t st, const void* tw, const double2* tw64, bool commit) {
+    if constexpr (col_two_buffers<T, N, C>()) {
+        if (col_two_buffers_ready<T, N, C>()) {
             cols_launch<T, N, C, true>(d, nmem, st, tw, tw64, commit);
             return;
         }
@@ -1024,14 +1032,20 @@ static bool fast_t(const Member* d, int nmem, const RhsShape& s, hipStream_t st,
     return false;
 }
 
+template <typename T, int N, int C>
+static int col_buffers_c() {
+    // (the shapes fast_n launches: at most 1024 threads and 64 KiB with one buffer)
+    if constexpr (C * (N / 8) <= 1024 && col_lds<T, N>(C) <= 65536) return col_two_buffers_ready<T, N, C>() ? 2 : 1;
+    return 1;
+}
 template <typename T, int N>
 static int col_buffers_n(int C) {
     switch (C) {
-    case 1: return col_two_buffers<T, N, 1>() ? 2 : 1;
-    case 2: return col_two_buffers<T, N, 2>() ? 2 : 1;
-    case 4: return col_two_buffers<T, N, 4>() ? 2 : 1;
-    case 8: return col_two_buffers<T, N, 8>() ? 2 : 1;
-    case 16: return col_two_buffers<T, N, 16>() ? 2 : 1;
+    case 1: return col_buffers_c<T, N, 1>();
+    case 2: return col_buffers_c<T, N, 2>();
+    case 4: return col_buffers_c<T, N, 4>();
+    case 8: return col_buffers_c<T, N, 8>();
+    case 16: return col_buffers_c<T, N, 16>();
     }
     return 1;
 }

@@ -45,6 +45,10 @@ def _run(nx, prec, steps=4, seed=7):
     (384, "mixed", {"FI_SROW_ROWS": "0", "FI_INV_THREADS": "384", "FI_COL_LINES": "2"}),
     (384, "fp64", {"FI_SROW_ROWS": "8", "FI_INV_THREADS": "384"}),
     (512, "mixed", {"FI_SROW_ROWS": "0", "FI_INV_THREADS": "512"}),
+    # column blocks that exchange through two LDS buffers (8 line pairs: 39 KiB
+    # at N = 256; 78 KiB at N = 512, declared to the runtime above 64 KiB)
+    (256, "mixed", {"FI_COL_LINES": "8"}),
+    (512, "mixed", {"FI_COL_LINES": "8"}),
     (512, "fp64", {"FI_SROW_ROWS": "16", "FI_INV_THREADS": "256"}),
     (512, "fp64", {"FI_SROW_ROWS": "0", "FI_INV_THREADS": "256", "FI_ROW_THREADS": "256"}),
 ])
