@@ -2678,8 +2678,9 @@ int fi_autotune(int n, int precision, int nmem, int reps, int* choice) {
         if (!e) per = sec / (double)std::max(1LL, a1 - a0);
         return e;
     };
-    // round 1: every candidate, `reps` updates; round 2: the built-in shape and
-    // the four fastest, 4 x reps updates each, twice in turn, best of the two;
+    // first pass: every candidate, `reps` updates; second pass: the built-in
+    // shape and the four fastest, 4 x reps updates each, twice in turn, best
+    // of the two;
     // a candidate replaces the built-in shape only when it is 1% faster
     const std::vector<TunedShape> cand = tune_candidates(n, nmem);
     std::vector<std::pair<double, size_t>> r1;
