@@ -50,6 +50,34 @@ def fast_kernel_names(shape):
             3: f"kf_row_inv<{t}, {n}, {inv or th}, {rt}, {meth}>"}
 
 
+# Algorithmic HBM bytes per cell of the once-per-coupling-step kernels of an
+# LV-ELVA update without convolutions (uniform spacing, elastic and sea-surface
+# convolutions off: the ensemble workload), all FP64 but the FP32 copy of
+# canom_full and the 1-byte mask.  The per-domain scalar kernels (begin, bsl,
+# decide, corner, check, gather) move a few hundred bytes per domain.  The
+# FETCH_SIZE / WRITE_SIZE counters of the 64-member ensemble read 40.2 + 16.0,
+# 65.1 + 45.0 and 40.1 + 24.1 B/cell for these three.
+STEP_BYTES_PER_CELL = {
+    "k_pre_haf": 40 + 16,        # read H_ice (ice grid, embedded each update), z_ss, z_bed, Haf_ref,
+                                 # H_ice_ref; write H_ice, Haf
+    "k_pre_fields": 65 + 45,     # read mask, z_ss_ref, Haf, z_bed, H_ice, Haf_ref, dz_ss, we_ref,
+                                 # canom_mantle; write canom_load, we, dz_ss, z_ss, mask, canom_full (f64, f32)
+    "k_finalize": 40 + 24,       # read w, w_ref, we, z_bed_ref, dz_ss; write now%w, canom_mantle, z_bed
+}
+
+
+def step_roofline(cells, attempts_per_step, ms_per_step, s=4, peak=8000.0):
+    """The coupling step as run (concurrent sub-batches, bookkeeping kernels
+    included): algorithmic bytes of every kernel of a step over its wall time."""
+    attempt_b = sum(kernel_bytes_per_cell(k, s, True) for k in (0, 2, 3))
+    per_cell = sum(STEP_BYTES_PER_CELL.values()) + attempt_b * attempts_per_step
+    b = cells * per_cell
+    gbs = b / (ms_per_step * 1e-3) / 1e9
+    return {"bytes_per_step": int(b), "ms_per_step": ms_per_step, "attempts_per_step": round(attempts_per_step, 2),
+            "bytes_per_cell": {"attempt": attempt_b, **STEP_BYTES_PER_CELL},
+            "achieved": round(gbs, 1), "peak": peak, "unit": "GB/s", "frac": round(gbs / peak, 4)}
+
+
 def measured_traffic(kernel_name, cells):
     """HBM bytes per launch of a kernel from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE measurement (scaled by the copy-kernel calibration), if it
@@ -566,6 +594,8 @@ def main():
           file=sys.stderr, flush=True)
 
     roof = roofline(res, args, cells, n)
+    roof["step"] = step_roofline(cells, res["attempts"] / args.members / (args.steps + args.warmup),
+                                 1e3 * res["wall"] / args.steps)
     single = {"workload": "one 256^2 ensemble member (N=256), 10-yr isos_update",
               "fp64": run_single("fp64", tune=not args.no_autotune),
               "mixed": run_single("mixed", tune=not args.no_autotune)}

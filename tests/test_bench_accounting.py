@@ -62,3 +62,14 @@ def test_roofline_arithmetic():
     assert ks[2]["template"] == "kf_col<float, 256, 8, true, true>" and ks[2]["traffic"] is not None
     tot = sum(k["bytes_per_launch"] for k in ks.values()) / (sum(k["avg_launch_us"] for k in ks.values()) * 1e-6)
     assert abs(tot / 1e9 / 8000.0 - 0.5734) < 2e-3
+
+
+def test_step_roofline_arithmetic():
+    # 64 x 256^2 cells, 50.5 attempts per step of 2.585 ms: 56 B/cell per attempt
+    # and 230 B/cell of once-per-step kernels
+    cells = 64 * 256 * 256
+    r = bench.step_roofline(cells, 50.5, 2.585)
+    assert r["bytes_per_cell"]["attempt"] == 56 and sum(bench.STEP_BYTES_PER_CELL.values()) == 230
+    assert r["bytes_per_step"] == int(cells * (230 + 56 * 50.5))
+    assert abs(r["achieved"] - cells * (230 + 56 * 50.5) / 2.585e-3 / 1e9) < 0.1
+    assert abs(r["frac"] - r["achieved"] / 8000.0) < 1e-4
