@@ -73,7 +73,8 @@ def step_roofline(cells, attempts_per_step, ms_per_step, s=4, peak=8000.0):
     per_cell = sum(STEP_BYTES_PER_CELL.values()) + attempt_b * attempts_per_step
     b = cells * per_cell
     gbs = b / (ms_per_step * 1e-3) / 1e9
-    return {"bytes_per_step": int(b), "ms_per_step": ms_per_step, "attempts_per_step": round(attempts_per_step, 2),
+    return {"bytes_per_step": int(b), "ms_per_step": round(ms_per_step, 4),
+            "attempts_per_step": round(attempts_per_step, 2),
             "bytes_per_cell": {"attempt": attempt_b, **STEP_BYTES_PER_CELL},
             "achieved": round(gbs, 1), "peak": peak, "unit": "GB/s", "frac": round(gbs / peak, 4)}
 
