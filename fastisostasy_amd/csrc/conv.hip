@@ -16,6 +16,7 @@
 //                convolutions.f90:34-39,89 is kept)
 // The four corners of the full linear convolution are exact products of input
 // and kernel corners, so the correction needs no extra pass.
+#include "fft_reg.hpp"
 #include "fi_device.hpp"
 #include "fi_kernels.h"
 
@@ -110,6 +111,67 @@ __global__ void __launch_bounds__(512) c_cols(const Member* __restrict__ mem, Ff
     }
 }
 
+// pass 2 of a convolution (mode 1 of c_cols) on the register engine, for
+// power-of-two M = 2N <= 1024: C columns per block, TPL = M/8 threads per
+// column, element e of thread t being row t + TPL e.  Input rows >= N are
+// zero (elements 4..7 are not loaded); the product with the kernel spectrum
+// is conjugated so that a second forward transform gives the conjugated
+// inverse; only the N crop rows are stored.  Columns past kx = N (the last
+// block's spare lines) transform a copy of column N and store nothing.
+template <typename T, int M, int C>
+__global__ void __launch_bounds__(C * (M / 8)) c_cols_reg(const Member* __restrict__ mem, const void* __restrict__ twv,
+                                                          int which) {
+    constexpr int N = M / 2, H = N + 1, TPL = M / 8, E = 8, THREADS = C * TPL;
+    const Member& Mb = mem[blockIdx.y];
+    extern __shared__ __align__(16) unsigned char smem[];
+    cplx<T>* tw_l = (cplx<T>*)smem;
+    cplx<T>* buf = tw_l + M;
+    const int tid = threadIdx.x, line = tid % C, t = tid / C;
+    const int kx = blockIdx.x * C + line;
+    const bool live = kx < H;
+    const int kc = live ? kx : H - 1;
+    const cplx<T>* __restrict__ g = (const cplx<T>*)twv;
+    cplx<T> twr[(M + THREADS - 1) / THREADS];
+#pragma unroll
+    for (int k = 0; k < (M + THREADS - 1) / THREADS; ++k)
+        if (tid + THREADS * k < M) twr[k] = g[tid + THREADS * k];
+    const cplx<T>* __restrict__ w1 = (const cplx<T>*)Mb.conv_work;
+    const cplx<T>* __restrict__ spec = (const cplx<T>*)(which == 0 ? Mb.FGE : which == 1 ? Mb.FGN : Mb.FGV);
+    cplx<T> v[E], sp[E];
+#pragma unroll
+    for (int e = 0; e < E / 2; ++e) v[e] = w1[(size_t)(t + TPL * e) * H + kc];
+#pragma unroll
+    for (int e = E / 2; e < E; ++e) v[e] = mk<T>((T)0, (T)0);
+#pragma unroll
+    for (int e = 0; e < E; ++e) sp[e] = spec[(size_t)(t + TPL * e) * H + kc];
+#pragma unroll
+    for (int k = 0; k < (M + THREADS - 1) / THREADS; ++k)
+        if (tid + THREADS * k < M) tw_l[tid + THREADS * k] = twr[k];
+    __syncthreads();
+    ColXch<T, M> xc(buf, line, C);
+    reg_fft_p2<T, M>(v, t, tw_l, xc);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        cplx<T> p = cmul(sp[e], v[e]);
+        p.y = -p.y;
+        v[e] = p;
+    }
+    // (the first transform's last exchange ended with a barrier: the buffer is free)
+    reg_fft_p2<T, M>(v, t, tw_l, xc);
+    if (!live) return;
+    cplx<T>* __restrict__ w2 = (cplx<T>*)Mb.conv_work + (size_t)N * H;
+    constexpr int c0 = N / 2 - 1;   // 0-based crop start (convolutions.f90:34-39), N even
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int jj = t + TPL * e - c0;
+        if (jj >= 0 && jj < N) {
+            cplx<T> o = v[e];
+            o.y = -o.y;
+            w2[(size_t)jj * H + kx] = o;
+        }
+    }
+}
+
 // pass 3: cropped rows back to real space, scale, corner correction, crop.
 template <typename T, bool TWL>
 __global__ void __launch_bounds__(512) c_rows_c2r(const Member* __restrict__ mem, FftPlan plan,
@@ -175,12 +237,34 @@ static void rows_r2c(const Member* d_mem, int nmem, const ConvShape& s, int whic
         hipLaunchKernelGGL((c_rows_r2c<T, false>), g, b, lbytes<T>(s.m, s.Lr), st, d_mem, s.plan, s.tw, s.Lr, which);
 }
 
+// the register-engine column pass: the widest column blocks within 1024
+// threads and 64 KiB of LDS (M twiddles and the M x (C + 1) exchange buffer)
+template <typename T, int M>
+static void cols_reg(const Member* d_mem, int nmem, const void* tw, int which, hipStream_t st) {
+    constexpr int H = M / 2 + 1;
+    constexpr auto fits = [](int C) {
+        return C * (M / 8) <= 1024 && ((size_t)M + (size_t)M * (C + 1)) * sizeof(cplx<T>) <= 65536;
+    };
+    constexpr int C = fits(8) ? 8 : fits(4) ? 4 : fits(2) ? 2 : 1;
+    static_assert(fits(C), "register column pass does not fit");
+    const size_t lds = ((size_t)M + (size_t)M * (C + 1)) * sizeof(cplx<T>);
+    hipLaunchKernelGGL((c_cols_reg<T, M, C>), dim3((H + C - 1) / C, nmem), dim3(C * (M / 8)), lds, st, d_mem, tw,
+                       which);
+}
+
 template <typename T>
 static void conv_t(const Member* d_mem, int nmem, const ConvShape& s, int which, hipStream_t st) {
     const int n = s.n, H = n + 1;
     rows_r2c<T>(d_mem, nmem, s, which, st);
-    hipLaunchKernelGGL(c_cols<T>, dim3((H + s.Cc - 1) / s.Cc, nmem), dim3(s.threads_col),
-                       lbytes<T>(s.m, s.Cc + 1), st, d_mem, s.plan, s.tw, s.Cc, 1, which);
+    // (FP64 lines of 512 and 1024 points would spill the register window:
+    // they stay on the LDS Stockham kernel)
+    if (s.m == 128) cols_reg<T, 128>(d_mem, nmem, s.tw, which, st);
+    else if (s.m == 256) cols_reg<T, 256>(d_mem, nmem, s.tw, which, st);
+    else if (s.m == 512 && sizeof(T) == 4) cols_reg<float, 512>(d_mem, nmem, s.tw, which, st);
+    else if (s.m == 1024 && sizeof(T) == 4) cols_reg<float, 1024>(d_mem, nmem, s.tw, which, st);
+    else
+        hipLaunchKernelGGL(c_cols<T>, dim3((H + s.Cc - 1) / s.Cc, nmem), dim3(s.threads_col),
+                           lbytes<T>(s.m, s.Cc + 1), st, d_mem, s.plan, s.tw, s.Cc, 1, which);
     const dim3 g(n / (2 * s.Lr), nmem), b(s.threads_row);
     if (s.rows_twl)
         hipLaunchKernelGGL((c_rows_c2r<T, true>), g, b, lbytes<T>(s.m, s.Lr + 1), st, d_mem, s.plan, s.tw, s.Lr, which);
