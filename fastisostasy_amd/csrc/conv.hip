@@ -9,7 +9,8 @@
 //                complex FFT of length M, split into their half spectra
 //   c_cols       per kx column: FFT over ky (input rows >= N are zero),
 //                multiply by the kernel spectrum, inverse FFT, keep only the
-//                N output rows of the crop window [N/2-1, 3N/2-1)
+//                N output rows of the crop window [N/2-1, 3N/2-1); on the
+//                register FFT engine (c_cols_reg) for M <= 1024 (FP64: M <= 256)
 //   c_rows_c2r   two cropped rows per inverse FFT, scale 1/M^2, subtract the
 //                corner correction of apply_zerobc_at_corners_dp on the full
 //                (2N-1)^2 array, keep the crop columns (the +1-cell shift of
