@@ -9,8 +9,10 @@
 //                complex FFT of length M, split into their half spectra
 //   c_cols       per kx column: FFT over ky (input rows >= N are zero),
 //                multiply by the kernel spectrum, inverse FFT, keep only the
-//                N output rows of the crop window [N/2-1, 3N/2-1); on the
-//                register FFT engine (c_cols_reg) for M <= 1024 (FP64: M <= 256)
+//                N output rows of the crop window [N/2-1, 3N/2-1)
+// For M <= 1024 (FP64: M <= 256) a convolution runs the same three passes on
+// the register FFT engine (c_rows_r2c_reg, c_cols_reg, c_rows_c2r_reg); the
+// LDS Stockham kernels below serve other lengths and the kernel spectra.
 //   c_rows_c2r   two cropped rows per inverse FFT, scale 1/M^2, subtract the
 //                corner correction of apply_zerobc_at_corners_dp on the full
 //                (2N-1)^2 array, keep the crop columns (the +1-cell shift of
@@ -226,8 +228,134 @@ __global__ void __launch_bounds__(512) c_rows_c2r(const Member* __restrict__ mem
     }
 }
 
+// passes 1 and 3 on the register engine (M = 2N <= 1024): a block holds L
+// complex lines of TPL = M/8 threads, line l packing rows 2j and 2j+1
+// (j = block * L + l), element e of thread t being column t + TPL e.
+//
+// pass 1: columns >= N are zero (elements 4..7 are not loaded); the half
+// spectra k = 0 .. N of the two rows from Z[k] and Z[M - k].
+template <typename T, int M, int L>
+__global__ void __launch_bounds__(L * (M / 8)) c_rows_r2c_reg(const Member* __restrict__ mem,
+                                                              const void* __restrict__ twv) {
+    constexpr int N = M / 2, H = N + 1, TPL = M / 8, E = 8, THREADS = L * TPL, TWQ = (M + THREADS - 1) / THREADS;
+    const Member& Mb = mem[blockIdx.y];
+    extern __shared__ __align__(16) unsigned char smem[];
+    cplx<T>* tw_l = (cplx<T>*)smem;
+    cplx<T>* buf = tw_l + M;
+    const int tid = threadIdx.x, line = tid / TPL, t = tid % TPL;
+    const int ra = 2 * (blockIdx.x * L + line), rb = ra + 1;
+    const cplx<T>* __restrict__ g = (const cplx<T>*)twv;
+    cplx<T> twr[TWQ];
+#pragma unroll
+    for (int k = 0; k < TWQ; ++k)
+        if (tid + THREADS * k < M) twr[k] = g[tid + THREADS * k];
+    const double* __restrict__ src = Mb.conv_in;
+    double xa[E / 2], xb[E / 2];
+#pragma unroll
+    for (int e = 0; e < E / 2; ++e) {
+        xa[e] = src[(size_t)ra * N + t + TPL * e];
+        xb[e] = src[(size_t)rb * N + t + TPL * e];
+    }
+#pragma unroll
+    for (int k = 0; k < TWQ; ++k)
+        if (tid + THREADS * k < M) tw_l[tid + THREADS * k] = twr[k];
+    __syncthreads();
+    cplx<T> v[E], m[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = e < E / 2 ? mk<T>((T)xa[e % (E / 2)], (T)xb[e % (E / 2)]) : mk<T>((T)0, (T)0);
+    RowXch<T, M> xc(buf, line);
+    reg_fft_mirror<T, M, true>(v, m, t, tw_l, xc);
+    cplx<T>* __restrict__ dst = (cplx<T>*)Mb.conv_work;
+#pragma unroll
+    for (int e = 0; e <= E / 2; ++e) {
+        const int k = t + TPL * e;
+        if (e < E / 2 || t == 0) {   // k = 0 .. N
+            const cplx<T> zk = v[e], zm = m[e];
+            dst[(size_t)ra * H + k] = mk<T>((T)0.5 * (zk.x + zm.x), (T)0.5 * (zk.y - zm.y));
+            dst[(size_t)rb * H + k] = mk<T>((T)0.5 * (zk.y + zm.y), -(T)0.5 * (zk.x - zm.x));
+        }
+    }
+}
+
+// pass 3: the Hermitian extensions of two cropped half-spectrum rows as one
+// conjugated complex line, one forward transform, scale, corner correction,
+// crop columns [N/2 - 1, 3N/2 - 1).
+template <typename T, int M, int L>
+__global__ void __launch_bounds__(L * (M / 8)) c_rows_c2r_reg(const Member* __restrict__ mem,
+                                                              const void* __restrict__ twv, int which) {
+    constexpr int N = M / 2, H = N + 1, TPL = M / 8, E = 8, THREADS = L * TPL, TWQ = (M + THREADS - 1) / THREADS;
+    const Member& Mb = mem[blockIdx.y];
+    // w_equilibrium is only recomputed for domains that step in this update
+    if (which == 2 && Mb.ctl->status != ST_RUN) return;
+    extern __shared__ __align__(16) unsigned char smem[];
+    cplx<T>* tw_l = (cplx<T>*)smem;
+    cplx<T>* buf = tw_l + M;
+    const int tid = threadIdx.x, line = tid / TPL, t = tid % TPL;
+    const int ra = 2 * (blockIdx.x * L + line), rb = ra + 1;
+    const cplx<T>* __restrict__ g = (const cplx<T>*)twv;
+    cplx<T> twr[TWQ];
+#pragma unroll
+    for (int k = 0; k < TWQ; ++k)
+        if (tid + THREADS * k < M) twr[k] = g[tid + THREADS * k];
+    const cplx<T>* __restrict__ w2 = (const cplx<T>*)Mb.conv_work + (size_t)N * H;
+    cplx<T> v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int k = t + TPL * e, kk = k <= N ? k : M - k;
+        cplx<T> xa = w2[(size_t)ra * H + kk], xb = w2[(size_t)rb * H + kk];
+        if (k > N) { xa.y = -xa.y; xb.y = -xb.y; }
+        // z = xa + i xb, conjugated
+        v[e] = mk<T>(xa.x - xb.y, -(xa.y + xb.x));
+    }
+    // the corner correction of apply_zerobc_at_corners_dp on the full array
+    const double* __restrict__ u = Mb.conv_in;
+    const double* kc = (which == 0) ? Mb.kc_E : (which == 1) ? Mb.kc_N : Mb.kc_V;
+    const double f00 = u[0] * kc[0];
+    const double fee = u[(size_t)N * N - 1] * kc[1];
+    const double fe0 = u[(size_t)(N - 1) * N] * kc[2];
+    const double f0e = u[N - 1] * kc[3];
+    const double cc = 0.25 * (f00 + fee + fe0 + f0e);
+#pragma unroll
+    for (int k = 0; k < TWQ; ++k)
+        if (tid + THREADS * k < M) tw_l[tid + THREADS * k] = twr[k];
+    __syncthreads();
+    RowXch<T, M> xc(buf, line);
+    reg_fft_p2<T, M>(v, t, tw_l, xc);
+    const double scale = (double)M * (double)M;
+    constexpr int c0 = N / 2 - 1;   // N even
+    double* __restrict__ out = (which == 0) ? Mb.we : (which == 1) ? Mb.dz_ss : Mb.weq;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int i = t + TPL * e - c0;
+        if (i >= 0 && i < N) {
+            // conj of the FFT: real part = row a, -(imag) = row b
+            out[(size_t)ra * N + i] = (double)v[e].x / scale - cc;
+            out[(size_t)rb * N + i] = -(double)v[e].y / scale - cc;
+        }
+    }
+}
+
 template <typename T>
 static size_t lbytes(int m, int lines) { return (size_t)m * lines * sizeof(cplx<T>); }
+
+// lines per block of the register row passes: the most within 1024 threads and
+// 64 KiB of LDS (M twiddles and L padded lines) that divide the N / 2 line pairs
+template <typename T, int M>
+constexpr int reg_row_lines() {
+    constexpr auto fits = [](int L) {
+        return L * (M / 8) <= 1024 && ((size_t)M + (size_t)L * RowXch<T, M>::PITCH) * sizeof(cplx<T>) <= 65536 &&
+               (M / 4) % L == 0;
+    };
+    return fits(8) ? 8 : fits(4) ? 4 : fits(2) ? 2 : 1;
+}
+template <typename T, int M>
+static void rows_reg(const Member* d_mem, int nmem, const void* tw, int which, hipStream_t st, bool inverse) {
+    constexpr int L = reg_row_lines<T, M>();
+    const size_t lds = ((size_t)M + (size_t)L * RowXch<T, M>::PITCH) * sizeof(cplx<T>);
+    const dim3 g(M / 4 / L, nmem), b(L * (M / 8));
+    if (inverse) hipLaunchKernelGGL((c_rows_c2r_reg<T, M, L>), g, b, lds, st, d_mem, tw, which);
+    else hipLaunchKernelGGL((c_rows_r2c_reg<T, M, L>), g, b, lds, st, d_mem, tw);
+}
 
 template <typename T>
 static void rows_r2c(const Member* d_mem, int nmem, const ConvShape& s, int which, hipStream_t st) {
@@ -256,9 +384,18 @@ static void cols_reg(const Member* d_mem, int nmem, const void* tw, int which, h
 template <typename T>
 static void conv_t(const Member* d_mem, int nmem, const ConvShape& s, int which, hipStream_t st) {
     const int n = s.n, H = n + 1;
-    rows_r2c<T>(d_mem, nmem, s, which, st);
-    // (FP64 lines of 512 and 1024 points would spill the register window:
-    // they stay on the LDS Stockham kernel)
+    // the register engine for M = 2N <= 1024 (FP64 lines of 512 and 1024
+    // points would spill the register window: they stay on the LDS Stockham
+    // kernels, as do other lengths)
+    const bool reg = s.m == 128 || s.m == 256 || (sizeof(T) == 4 && (s.m == 512 || s.m == 1024));
+    if (reg) {
+        if (s.m == 128) rows_reg<T, 128>(d_mem, nmem, s.tw, which, st, false);
+        else if (s.m == 256) rows_reg<T, 256>(d_mem, nmem, s.tw, which, st, false);
+        else if (s.m == 512) rows_reg<float, 512>(d_mem, nmem, s.tw, which, st, false);
+        else rows_reg<float, 1024>(d_mem, nmem, s.tw, which, st, false);
+    } else {
+        rows_r2c<T>(d_mem, nmem, s, which, st);
+    }
     if (s.m == 128) cols_reg<T, 128>(d_mem, nmem, s.tw, which, st);
     else if (s.m == 256) cols_reg<T, 256>(d_mem, nmem, s.tw, which, st);
     else if (s.m == 512 && sizeof(T) == 4) cols_reg<float, 512>(d_mem, nmem, s.tw, which, st);
@@ -266,6 +403,13 @@ static void conv_t(const Member* d_mem, int nmem, const ConvShape& s, int which,
     else
         hipLaunchKernelGGL(c_cols<T>, dim3((H + s.Cc - 1) / s.Cc, nmem), dim3(s.threads_col),
                            lbytes<T>(s.m, s.Cc + 1), st, d_mem, s.plan, s.tw, s.Cc, 1, which);
+    if (reg) {
+        if (s.m == 128) rows_reg<T, 128>(d_mem, nmem, s.tw, which, st, true);
+        else if (s.m == 256) rows_reg<T, 256>(d_mem, nmem, s.tw, which, st, true);
+        else if (s.m == 512) rows_reg<float, 512>(d_mem, nmem, s.tw, which, st, true);
+        else rows_reg<float, 1024>(d_mem, nmem, s.tw, which, st, true);
+        return;
+    }
     const dim3 g(n / (2 * s.Lr), nmem), b(s.threads_row);
     if (s.rows_twl)
         hipLaunchKernelGGL((c_rows_c2r<T, true>), g, b, lbytes<T>(s.m, s.Lr + 1), st, d_mem, s.plan, s.tw, s.Lr, which);
